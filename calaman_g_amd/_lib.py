@@ -1,0 +1,139 @@
+"""ctypes binding of libcalaman_expm.so (include/calaman_expm.h).
+
+The library is the product: there is no CPU fallback. Loading fails loudly
+when the shared object has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcalaman_expm.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "calaman_expm.h")
+
+# status codes (calaman_expm.h)
+CLM_DOMAIN_BLAS = 0x00010000
+CLM_DOMAIN_SOLVER = 0x00020000
+CLM_DOMAIN_RUNTIME = 0x00030000
+SUCCESS = 0
+NOT_INITIALIZED = CLM_DOMAIN_BLAS | 1
+ALLOC_FAILED = CLM_DOMAIN_BLAS | 3
+INVALID_VALUE = CLM_DOMAIN_BLAS | 7
+EXECUTION_FAILED = CLM_DOMAIN_BLAS | 13
+NOT_SUPPORTED = CLM_DOMAIN_BLAS | 15
+RUNTIME_INVALID_VALUE = CLM_DOMAIN_RUNTIME | 1
+RUNTIME_ERROR = CLM_DOMAIN_RUNTIME | 2
+
+METHOD_TAYLOR_PS = 0
+METHOD_PADE = 1
+DTYPE_F64 = 1
+DTYPE_C128 = 3
+
+NORM_MAX_ABS, NORM_ONE, NORM_INF, NORM_FROBENIUS = 0, 1, 2, 3
+
+
+class ExpmPlan(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
+                ("method", ctypes.c_int), ("ps_block", ctypes.c_int), ("dtype", ctypes.c_int),
+                ("norm1", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PsPlan(ctypes.Structure):
+    _fields_ = [("s", ctypes.c_int), ("r", ctypes.c_int), ("num_powers", ctypes.c_int),
+                ("num_blocks", ctypes.c_int), ("num_gemms", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class DoubleComplex(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class CalamanError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        name = lib().clm_status_string(status).decode()
+        super().__init__(f"{what}: {name} ({status:#x})" if what else f"{name} ({status:#x})")
+
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_sz = ctypes.c_size_t
+_d = ctypes.c_double
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "clm_version": (ctypes.c_char_p, []),
+    "clm_status_string": (ctypes.c_char_p, [_i]),
+    "clm_create": (_i, [ctypes.POINTER(_vp)]),
+    "clm_destroy": (_i, [_vp]),
+    "clm_set_stream": (_i, [_vp, _vp]),
+    "clm_get_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "clm_expm_plan_d": (ExpmPlan, [_d]),
+    "clm_taylor_theta_d": (_d, [_i]),
+    "clm_pade_theta_d": (_d, [_i]),
+    "clm_pade_plan_d": (ExpmPlan, [_d]),
+    "clm_expm_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
+    "clm_expm_buffer_size_z": (_i, [_i, ctypes.POINTER(_sz)]),
+    "clm_expm_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
+    "clm_expm_z": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
+    "clm_matrix_norm1_d": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_d)]),
+    "clm_matrix_norm1_z": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_d)]),
+    "clm_lange_d": (_i, [_vp, _i, _sz, _sz, _vp, _sz, _vp, _vp]),
+    "clm_lange_z": (_i, [_vp, _i, _sz, _sz, _vp, _sz, _vp, _vp]),
+    "clm_lascl_d": (_i, [_vp, _d, _d, _sz, _sz, _vp, _sz]),
+    "clm_lascl_z": (_i, [_vp, _d, _d, _sz, _sz, _vp, _sz]),
+    "clm_paterson_stockmeyer_cost": (_i, [_i, _i]),
+    "clm_paterson_stockmeyer_block_size": (_i, [_i]),
+    "clm_paterson_stockmeyer_plan": (PsPlan, [_i, _i]),
+    "clm_paterson_stockmeyer_power_stride_d": (_sz, [_i]),
+    "clm_paterson_stockmeyer_power_stride_z": (_sz, [_i]),
+    "clm_paterson_stockmeyer_buffer_size_d": (_sz, [_i, _i, _i]),
+    "clm_paterson_stockmeyer_buffer_size_z": (_sz, [_i, _i, _i]),
+    "clm_paterson_stockmeyer_d": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _i]),
+    "clm_paterson_stockmeyer_z": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _i]),
+    "clm_horner_buffer_size_d": (_sz, [_i]),
+    "clm_horner_buffer_size_z": (_sz, [_i]),
+    "clm_horner_d": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "clm_horner_z": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "clm_gemm_d": (_i, [_vp, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
+    "clm_gemm_z": (_i, [_vp, _i, _i, _i, DoubleComplex, _vp, _i, _vp, _i, DoubleComplex, _vp,
+                        _i]),
+}
+
+
+def header_symbols(path: str = HEADER_PATH):
+    """Every function the public C header declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(clm_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(the HIP library is required; there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != SUCCESS:
+        raise CalamanError(status, what)

@@ -1,0 +1,732 @@
+// capi.hip -- host driver and extern "C" entry points of the MI355X
+// matrix-exponential library (declared in include/calaman_expm.h).
+//
+// The host side composes the kernels of gemm.hip and elementwise.hip the way
+// the reference's C++ modules compose vendor BLAS calls:
+//   expm      -- src/expm/expm.cppm:87-174 (validation order, workspace check,
+//                one 1-norm readback, plan, 2^-s scaling, evaluation, s squarings)
+//   PS        -- src/paterson_stockmeyer/interface.cppm:117-422
+//   Horner    -- src/horner/interface.cppm:103-194
+//   lange     -- src/lapack/lange/interface.cppm:82-107
+//   lascl     -- src/lapack/lascl/interface.cppm:79-143
+// The evaluation itself is the truncated Taylor series T_m(A/2^s) by
+// Paterson-Stockmeyer instead of the reference's Pade approximant + LU: same
+// accuracy target (backward error <= u), no LU, and every product is an MFMA GEMM.
+// Argument checks come before any HIP call, so rejected calls never touch a device.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "clm_internal.hpp"
+
+struct clm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace clm {
+namespace {
+
+#define CLM_TRY(expr)                                             \
+  do {                                                            \
+    const clm_status_t clm_try_status_ = (expr);                  \
+    if (clm_try_status_ != CLM_STATUS_SUCCESS) return clm_try_status_; \
+  } while (0)
+#define CLM_TRY_HIP(expr)                                         \
+  do {                                                            \
+    if ((expr) != hipSuccess) return CLM_STATUS_EXECUTION_FAILED; \
+  } while (0)
+
+std::mutex g_ctx_mutex;
+clm_ctx *g_default_ctx[64] = {};
+
+clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
+  if (ctx) {
+    *out = ctx;
+    return CLM_STATUS_SUCCESS;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return CLM_STATUS_NOT_INITIALIZED;
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  if (!g_default_ctx[dev]) {
+    clm_ctx *c = new clm_ctx;
+    c->device = dev;
+    c->stream = nullptr;
+    g_default_ctx[dev] = c;
+  }
+  *out = g_default_ctx[dev];
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+const T *taylor_coefficients(int device);
+template <>
+const double *taylor_coefficients<double>(int device) {
+  return inv_factorials_d(device);
+}
+template <>
+const zd *taylor_coefficients<zd>(int device) {
+  return inv_factorials_z(device);
+}
+
+template <typename T>
+constexpr int dtype_code();
+template <>
+constexpr int dtype_code<double>() {
+  return CLM_DTYPE_F64;
+}
+template <>
+constexpr int dtype_code<zd>() {
+  return CLM_DTYPE_C128;
+}
+
+template <typename T>
+T one() {
+  return ops<T>::from_real(1.0);
+}
+
+// ---- Paterson-Stockmeyer plan (src/paterson_stockmeyer/interface.cppm:117-210) ----
+int ps_cost(int degree, int s) {
+  if (s < 1 || degree < 1) return 0;
+  const int r = degree / s;
+  const int max_power = (r >= 1) ? s : degree;
+  const int num_powers = (max_power >= 2) ? (max_power - 1) : 0;
+  return num_powers + r;
+}
+
+int ps_block_size(int degree) {
+  if (degree < 1) return 1;
+  int best_s = 1, best = ps_cost(degree, 1);
+  for (int s = 2; s <= degree + 1; ++s) {
+    const int c = ps_cost(degree, s);
+    if (c < best) {
+      best = c;
+      best_s = s;
+    }
+  }
+  return best_s;
+}
+
+clm_ps_plan ps_plan(int degree, int s_requested) {
+  clm_ps_plan p{1, 0, 0, 0, 0};
+  if (degree < 0) return p;
+  int s = (s_requested > 0) ? s_requested : ps_block_size(degree);
+  if (s > degree + 1) s = degree + 1;
+  if (s < 1) s = 1;
+  p.s = s;
+  p.r = degree / s;
+  const int max_power = (p.r >= 1) ? s : degree;
+  p.num_powers = (max_power >= 2) ? (max_power - 1) : 0;
+  p.num_blocks = p.num_powers + ((p.r >= 1) ? 1 : 0);
+  p.num_gemms = p.num_powers + p.r;
+  return p;
+}
+
+template <typename T>
+size_t power_stride(int n) {
+  const size_t order = static_cast<size_t>(n < 1 ? 1 : n);
+  return Layout::align(order * order * sizeof(T)) / sizeof(T);
+}
+
+template <typename T>
+size_t ps_buffer_size(int n, int degree, int s_requested) {
+  const int order = n < 1 ? 1 : n;
+  const clm_ps_plan p = ps_plan(degree, s_requested);
+  Layout layout(nullptr);
+  if (p.num_powers > 0) layout.take<T>(static_cast<size_t>(p.num_powers) * power_stride<T>(order));
+  if (p.r >= 1) layout.take<T>(static_cast<size_t>(order) * order);
+  return layout.total();
+}
+
+// ---- Taylor ladder -----------------------------------------------------------------------
+// theta_m: the largest x with sum_{k>m} |h_k| x^(k-1) <= 2^-53 for
+// h(x) = log(exp(-x) T_m(x)) -- the backward-error criterion the reference
+// applies to its Pade rungs (src/expm/plan.cppm:29-41), evaluated by an exact
+// rational series and bisection. The degrees are those where the
+// Paterson-Stockmeyer product count rises by one.
+constexpr int kNumTaylor = 10;
+constexpr int kTaylorDegrees[kNumTaylor] = {1, 2, 3, 5, 8, 11, 15, 19, 24, 29};
+constexpr double kTaylorTheta[kNumTaylor] = {
+    2.220446049250313e-16, 2.580956802971767e-8, 1.386347866119121e-5, 2.400876357887274e-3,
+    4.991228871115323e-2,  2.142358068451711e-1, 6.410835233041199e-1, 1.260381060642639,
+    2.21904886936509,      3.310172839890271};
+
+// the largest power bank any ladder degree needs
+int max_bank_blocks() {
+  int mx = 0;
+  for (int i = 0; i < kNumTaylor; ++i) {
+    const int np = ps_plan(kTaylorDegrees[i], 0).num_powers;
+    if (np > mx) mx = np;
+  }
+  return mx;
+}
+
+clm_expm_plan taylor_plan(double norm1) {
+  int best = -1, best_cost = 0, best_s = 0;
+  for (int i = 0; i < kNumTaylor; ++i) {
+    int s = 0;
+    if (norm1 > kTaylorTheta[i]) {
+      s = static_cast<int>(std::ceil(std::log2(norm1 / kTaylorTheta[i])));
+      if (s < 0) s = 0;
+    }
+    const int cost = ps_plan(kTaylorDegrees[i], 0).num_gemms + s;
+    if (best < 0 || cost < best_cost || (cost == best_cost && s < best_s)) {
+      best = i;
+      best_cost = cost;
+      best_s = s;
+    }
+  }
+  clm_expm_plan p{};
+  p.m = kTaylorDegrees[best];
+  p.s = best_s;
+  p.num_gemms = best_cost;
+  p.method = CLM_METHOD_TAYLOR_PS;
+  p.ps_block = ps_plan(p.m, 0).s;
+  p.dtype = 0;
+  p.norm1 = norm1;
+  return p;
+}
+
+// ---- the reference's Pade ladder (src/expm/plan.cppm:246-326, expm_bridge.h:128-130) ----
+double pade_theta(int m) {
+  switch (m) {
+  case 3: return 1.495585217958292e-2;
+  case 5: return 2.539398330063232e-1;
+  case 7: return 9.504178996162932e-1;
+  case 9: return 2.097847961257067;
+  case 13: return 5.371920351148152;
+  default: return 0.0;
+  }
+}
+int pade_num_gemms(int m) { return ((m == 13) ? 3 : (m - 1) / 2) + 1 + ((m == 13) ? 2 : 0); }
+
+clm_expm_plan pade_plan(double norm1) {
+  clm_expm_plan p{};
+  p.method = CLM_METHOD_PADE;
+  p.norm1 = norm1;
+  const int degrees[5] = {3, 5, 7, 9, 13};
+  for (int m : degrees) {
+    if (norm1 <= pade_theta(m)) {
+      p.m = m;
+      p.s = 0;
+      p.num_gemms = pade_num_gemms(m);
+      return p;
+    }
+  }
+  int s = static_cast<int>(std::ceil(std::log2(norm1 / pade_theta(13))));
+  if (s < 0) s = 0;
+  p.m = 13;
+  p.s = s;
+  p.num_gemms = pade_num_gemms(13) + s;
+  return p;
+}
+
+// ---- expm workspace --------------------------------------------------------------------------
+//   [colsum | n doubles ]  per-column |.| sums; colsum[0] ends as the 1-norm
+//   [As     | n*n T     ]  2^-s A
+//   [bank   | B x stride]  A^2 .. A^(B+1) of the scaled matrix, B = max_bank_blocks()
+//   [S      | n*n T     ]  Paterson-Stockmeyer scratch, then the squaring ping-pong
+// Six n*n blocks at the top of the ladder, against the reference's seven
+// (src/expm/README.md:206-211); nothing for an LU.
+template <typename T>
+struct ExpmWs {
+  double *colsum = nullptr;
+  T *As = nullptr;
+  T *bank = nullptr;
+  T *S = nullptr;
+  size_t stride = 0;
+  size_t carve(void *base, int n) {
+    Layout l(base);
+    const size_t nn = static_cast<size_t>(n) * n;
+    stride = power_stride<T>(n);
+    colsum = l.take<double>(static_cast<size_t>(n));
+    As = l.take<T>(nn);
+    bank = l.take<T>(static_cast<size_t>(max_bank_blocks()) * stride);
+    S = l.take<T>(nn);
+    return l.total();
+  }
+};
+
+template <typename T>
+size_t expm_buffer_size(int n) {
+  ExpmWs<T> ws;
+  return ws.carve(nullptr, n);
+}
+
+// ---- device building blocks ------------------------------------------------------------------
+template <typename T>
+clm_status_t gemm(hipStream_t st, int n, const T *A, long lda, const T *B, long ldb, T *C,
+                  long ldc) {
+  GemmArgs<T> g;
+  g.M = g.N = g.K = n;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.alpha = one<T>();
+  CLM_TRY_HIP(launch_gemm(st, g));
+  return CLM_STATUS_SUCCESS;
+}
+
+// Paterson-Stockmeyer on the device, the reference's structure
+// (src/paterson_stockmeyer/interface.cppm:364-419) with each B_j built inside
+// the epilogue of the Horner product that consumes it.
+template <typename T>
+clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const T *A, long lda,
+                         T *P, long ldp, T *bank, long stride, T *scratch, long ld_scratch,
+                         int s_requested) {
+  const clm_ps_plan p = ps_plan(degree, s_requested);
+  for (int k = 2; k <= p.num_powers + 1; ++k) {
+    const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+    const long ld_left = (k == 2) ? lda : n;
+    CLM_TRY(gemm(st, n, left, ld_left, A, lda, bank + static_cast<long>(k - 2) * stride, n));
+  }
+  const T *X = (p.r < 1) ? nullptr : (p.s >= 2) ? bank + static_cast<long>(p.s - 2) * stride : A;
+  const long ldx = (p.s >= 2) ? n : lda;
+  T *buf[2] = {P, scratch};
+  const long ld[2] = {ldp, ld_scratch};
+  int cur = (p.r % 2 == 0) ? 0 : 1;
+  auto terms = [&](int j) {
+    const int remaining = degree - j * p.s + 1;
+    return remaining < p.s ? remaining : p.s;
+  };
+  CLM_TRY_HIP(launch_build_block(st, n, buf[cur], ld[cur], coef + static_cast<long>(p.r) * p.s,
+                                 terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
+  for (int j = p.r - 1; j >= 0; --j) {
+    const int next = 1 - cur;
+    GemmArgs<T> g;
+    g.M = g.N = g.K = n;
+    g.A = buf[cur];
+    g.lda = ld[cur];
+    g.B = X;
+    g.ldb = ldx;
+    g.C = buf[next];
+    g.ldc = ld[next];
+    g.alpha = one<T>();
+    g.coef = coef + static_cast<long>(j) * p.s;
+    g.nterms = terms(j);
+    g.X1 = A;
+    g.ldx1 = lda;
+    g.bank = bank;
+    g.bank_stride = stride;
+    g.ldbank = n;
+    CLM_TRY_HIP(launch_gemm(st, g));
+    cur = next;
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, double *colsum,
+                        double *norm) {
+  CLM_TRY_HIP(launch_norm1(st, n, A, lda, colsum));
+  double host = 0.0;
+  CLM_TRY_HIP(hipMemcpyAsync(&host, colsum, sizeof(double), hipMemcpyDeviceToHost, st));
+  CLM_TRY_HIP(hipStreamSynchronize(st));
+  *norm = host;
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda, T *dE, int lde,
+                       void *work, size_t bytes, int *dInfo, clm_expm_plan *plan) {
+  if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
+      dE == nullptr)
+    return CLM_STATUS_INVALID_VALUE;
+  if (bytes < expm_buffer_size<T>(n)) return CLM_STATUS_ALLOC_FAILED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  const T *coef = taylor_coefficients<T>(c->device);
+  if (!coef) return CLM_STATUS_EXECUTION_FAILED;
+
+  ExpmWs<T> ws;
+  ws.carve(work, n);
+
+  double norm = 0.0;
+  CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
+  if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
+
+  clm_expm_plan p = taylor_plan(norm);
+  p.dtype = dtype_code<T>();
+  if (plan) *plan = p;
+
+  const T *src = dA;
+  long ld_src = lda;
+  if (p.s > 0) {
+    CLM_TRY_HIP(launch_scale_copy(st, n, std::ldexp(1.0, -p.s), dA, lda, ws.As, n));
+    src = ws.As;
+    ld_src = n;
+  }
+
+  // The polynomial lands where the s squarings that follow end in dE, so no
+  // final copy is needed (the reference copies when s is odd, expm.cppm:167-171).
+  T *target = (p.s % 2 == 0) ? dE : ws.S;
+  const long ld_target = (p.s % 2 == 0) ? lde : n;
+  T *other = (p.s % 2 == 0) ? ws.S : dE;
+  const long ld_other = (p.s % 2 == 0) ? n : lde;
+
+  CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+                         static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+
+  T *cur = target, *nxt = other;
+  long ldc = ld_target, ldn = ld_other;
+  for (int k = 0; k < p.s; ++k) {
+    CLM_TRY(gemm(st, n, cur, ldc, cur, ldc, nxt, ldn));
+    std::swap(cur, nxt);
+    std::swap(ldc, ldn);
+  }
+  if (dInfo) CLM_TRY_HIP(launch_fill_int(st, dInfo, 2, 0));
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t ps_impl(clm_ctx *ctx, int n, const T *dCoeffs, int degree, const T *dA, int lda,
+                     T *dP, int ldp, void *work, size_t work_bytes, int s_requested) {
+  if (n < 1 || n > kMaxDim || degree < 0 || lda < n || ldp < n || s_requested < 0)
+    return CLM_STATUS_NOT_INITIALIZED;
+  if (dCoeffs == nullptr || dA == nullptr || dP == nullptr) return CLM_STATUS_NOT_INITIALIZED;
+  const clm_ps_plan p = ps_plan(degree, s_requested);
+  if (p.num_blocks > 0 &&
+      (work == nullptr || work_bytes < ps_buffer_size<T>(n, degree, s_requested)))
+    return CLM_STATUS_NOT_INITIALIZED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  Layout l(work);
+  const size_t stride = power_stride<T>(n);
+  T *bank = nullptr, *scratch = nullptr;
+  if (p.num_powers > 0) bank = l.take<T>(static_cast<size_t>(p.num_powers) * stride);
+  if (p.r >= 1) scratch = l.take<T>(static_cast<size_t>(n) * n);
+  return ps_evaluate<T>(c->stream, n, dCoeffs, degree, dA, lda, dP, ldp, bank,
+                        static_cast<long>(stride), scratch, n, s_requested);
+}
+
+template <typename T>
+size_t horner_buffer_size(int n) {
+  const size_t order = static_cast<size_t>(n < 1 ? 1 : n);
+  Layout l(nullptr);
+  l.take<T>(order * order);
+  return l.total();
+}
+
+// Horner (src/horner/interface.cppm:156-194): P <- c_d I, then P <- P A + c_k I,
+// the diagonal add riding in the product's epilogue.
+template <typename T>
+clm_status_t horner_impl(clm_ctx *ctx, int n, const T *dCoeffs, int degree, const T *dA, int lda,
+                         T *dP, int ldp, void *work, size_t work_bytes) {
+  if (!(n >= 1 && n <= kMaxDim && degree >= 0 && lda >= n && ldp >= n))
+    return CLM_STATUS_NOT_INITIALIZED;
+  if (dCoeffs == nullptr || dA == nullptr || dP == nullptr) return CLM_STATUS_NOT_INITIALIZED;
+  if (!(degree == 0 || (work != nullptr && work_bytes >= horner_buffer_size<T>(n))))
+    return CLM_STATUS_NOT_INITIALIZED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  hipStream_t st = c->stream;
+  T *buf[2] = {dP, static_cast<T *>(work)};
+  const long ld[2] = {ldp, n};
+  int cur = (degree % 2 == 0) ? 0 : 1;
+  CLM_TRY_HIP(launch_set_scaled_identity(st, n, buf[cur], ld[cur], dCoeffs + degree));
+  for (int k = degree - 1; k >= 0; --k) {
+    const int next = 1 - cur;
+    GemmArgs<T> g;
+    g.M = g.N = g.K = n;
+    g.A = buf[cur];
+    g.lda = ld[cur];
+    g.B = dA;
+    g.ldb = lda;
+    g.C = buf[next];
+    g.ldc = ld[next];
+    g.alpha = one<T>();
+    g.coef = dCoeffs + k;
+    g.nterms = 1;
+    CLM_TRY_HIP(launch_gemm(st, g));
+    cur = next;
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t gemm_impl(clm_ctx *ctx, int m, int n, int k, T alpha, const T *dA, int lda,
+                       const T *dB, int ldb, T beta, T *dC, int ldc) {
+  if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) ||
+      ldc < (m > 1 ? m : 1))
+    return CLM_STATUS_INVALID_VALUE;
+  if (m == 0 || n == 0) return CLM_STATUS_SUCCESS;
+  if (dC == nullptr || (k > 0 && (dA == nullptr || dB == nullptr))) return CLM_STATUS_INVALID_VALUE;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  GemmArgs<T> g;
+  g.M = m;
+  g.N = n;
+  g.K = k;
+  g.A = dA;
+  g.lda = lda;
+  g.B = dB;
+  g.ldb = ldb;
+  g.C = dC;
+  g.ldc = ldc;
+  g.alpha = alpha;
+  g.beta = beta;
+  if (!ops<T>::is_zero(beta)) {
+    g.Cin = dC;
+    g.ldcin = ldc;
+  }
+  CLM_TRY_HIP(launch_gemm(c->stream, g));
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t norm1_impl(clm_stream_t stream, int n, const T *dA, int lda, double *dColsum,
+                        double *norm) {
+  if (n < 1 || n > kMaxDim || lda < n || dA == nullptr || dColsum == nullptr || norm == nullptr)
+    return CLM_STATUS_INVALID_VALUE;
+  return read_norm1(static_cast<hipStream_t>(stream), n, dA, lda, dColsum, norm);
+}
+
+template <typename T>
+clm_status_t lange_impl(clm_stream_t stream, int which, size_t m, size_t n, const T *dA,
+                        size_t lda, double *dResult, double *dScratch) {
+  if (which < CLM_NORM_MAX_ABS || which > CLM_NORM_FROBENIUS || dResult == nullptr ||
+      lda < (m > 1 ? m : 1))
+    return CLM_STATUS_RUNTIME_INVALID_VALUE;
+  if (m > 0 && n > 0 && (dA == nullptr || dScratch == nullptr))
+    return CLM_STATUS_RUNTIME_INVALID_VALUE;
+  if (n > 0x7fffffffu) return CLM_STATUS_RUNTIME_INVALID_VALUE;
+  CLM_TRY_HIP(launch_lange(static_cast<hipStream_t>(stream), which, m, n, dA, lda, dResult,
+                           dScratch));
+  return CLM_STATUS_SUCCESS;
+}
+
+// ?lascl's guarded multiplier loop, verbatim in structure from netlib via the
+// reference (src/lapack/lascl/interface.cppm:101-140); each factor is one pass.
+template <typename T>
+clm_status_t lascl_impl(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n, T *dA,
+                        size_t lda) {
+  if (cfrom == 0.0 || std::isnan(cfrom) || std::isnan(cto)) return CLM_STATUS_RUNTIME_INVALID_VALUE;
+  if (m == 0 || n == 0) return CLM_STATUS_SUCCESS;
+  if (dA == nullptr || lda < m) return CLM_STATUS_RUNTIME_INVALID_VALUE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double smlnum = 2.2250738585072014e-308;  // DBL_MIN, DLAMCH('S')
+  const double bignum = 1.0 / smlnum;
+  double cfromc = cfrom, ctoc = cto;
+  bool done = false;
+  do {
+    const double cfrom1 = cfromc * smlnum;
+    double mul;
+    if (cfrom1 == cfromc) {
+      mul = ctoc / cfromc;
+      done = true;
+    } else {
+      const double cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        mul = ctoc;
+        done = true;
+        cfromc = 1.0;
+      } else if (std::fabs(cfrom1) > std::fabs(ctoc) && ctoc != 0.0) {
+        mul = smlnum;
+        done = false;
+        cfromc = cfrom1;
+      } else if (std::fabs(cto1) > std::fabs(cfromc)) {
+        mul = bignum;
+        done = false;
+        ctoc = cto1;
+      } else {
+        mul = ctoc / cfromc;
+        done = true;
+      }
+    }
+    // grid rows carry columns; launch in runs of at most 65535 columns
+    for (size_t j0 = 0; j0 < n; j0 += 65535) {
+      const size_t nc = (n - j0 < 65535) ? n - j0 : 65535;
+      if (launch_lascl(st, m, nc, mul, dA + j0 * lda, lda) != hipSuccess)
+        return CLM_STATUS_RUNTIME_ERROR;
+    }
+  } while (!done);
+  return CLM_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace clm
+
+using clm::zd;
+
+static_assert(sizeof(clm_double_complex) == sizeof(zd), "complex layout");
+
+extern "C" {
+
+const char *clm_version(void) { return "calaman-expm-mi355x 0.1 (gfx950)"; }
+
+const char *clm_status_string(clm_status_t s) {
+  switch (s) {
+  case CLM_STATUS_SUCCESS: return "CLM_STATUS_SUCCESS";
+  case CLM_STATUS_NOT_INITIALIZED: return "CLM_STATUS_NOT_INITIALIZED";
+  case CLM_STATUS_ALLOC_FAILED: return "CLM_STATUS_ALLOC_FAILED";
+  case CLM_STATUS_INVALID_VALUE: return "CLM_STATUS_INVALID_VALUE";
+  case CLM_STATUS_EXECUTION_FAILED: return "CLM_STATUS_EXECUTION_FAILED";
+  case CLM_STATUS_NOT_SUPPORTED: return "CLM_STATUS_NOT_SUPPORTED";
+  case CLM_STATUS_RUNTIME_INVALID_VALUE: return "CLM_STATUS_RUNTIME_INVALID_VALUE";
+  case CLM_STATUS_RUNTIME_ERROR: return "CLM_STATUS_RUNTIME_ERROR";
+  default: return "CLM_STATUS_UNKNOWN";
+  }
+}
+
+clm_status_t clm_create(clm_ctx **ctx) {
+  if (ctx == nullptr) return CLM_STATUS_INVALID_VALUE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CLM_STATUS_NOT_INITIALIZED;
+  clm_ctx *c = new clm_ctx;
+  c->device = dev;
+  c->stream = nullptr;
+  *ctx = c;
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_destroy(clm_ctx *ctx) {
+  delete ctx;
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_set_stream(clm_ctx *ctx, clm_stream_t stream) {
+  if (ctx == nullptr) return CLM_STATUS_INVALID_VALUE;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream) {
+  if (ctx == nullptr || stream == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *stream = ctx->stream;
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_expm_plan clm_expm_plan_d(double norm1) { return clm::taylor_plan(norm1); }
+
+double clm_taylor_theta_d(int m) {
+  for (int i = 0; i < clm::kNumTaylor; ++i)
+    if (clm::kTaylorDegrees[i] == m) return clm::kTaylorTheta[i];
+  return 0.0;
+}
+
+double clm_pade_theta_d(int m) { return clm::pade_theta(m); }
+clm_expm_plan clm_pade_plan_d(double norm1) { return clm::pade_plan(norm1); }
+
+clm_status_t clm_expm_buffer_size_d(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_buffer_size<double>(n);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_buffer_size_z(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_buffer_size<zd>(n);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                        double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                        clm_expm_plan *plan) {
+  return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
+                                lwork_bytes, dInfo, plan);
+}
+
+clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_double_complex *dA,
+                        int lda, clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes,
+                        int *dInfo, clm_expm_plan *plan) {
+  return clm::expm_impl<zd>(ctx, static_cast<hipStream_t>(stream), n,
+                            reinterpret_cast<const zd *>(dA), lda, reinterpret_cast<zd *>(dE), lde,
+                            dWork, lwork_bytes, dInfo, plan);
+}
+
+clm_status_t clm_matrix_norm1_d(clm_stream_t stream, int n, const double *dA, int lda,
+                                double *dColsum, double *norm) {
+  return clm::norm1_impl<double>(stream, n, dA, lda, dColsum, norm);
+}
+
+clm_status_t clm_matrix_norm1_z(clm_stream_t stream, int n, const clm_double_complex *dA, int lda,
+                                double *dColsum, double *norm) {
+  return clm::norm1_impl<zd>(stream, n, reinterpret_cast<const zd *>(dA), lda, dColsum, norm);
+}
+
+clm_status_t clm_lange_d(clm_stream_t stream, int which, size_t m, size_t n, const double *dA,
+                         size_t lda, double *dResult, double *dScratch) {
+  return clm::lange_impl<double>(stream, which, m, n, dA, lda, dResult, dScratch);
+}
+
+clm_status_t clm_lange_z(clm_stream_t stream, int which, size_t m, size_t n,
+                         const clm_double_complex *dA, size_t lda, double *dResult,
+                         double *dScratch) {
+  return clm::lange_impl<zd>(stream, which, m, n, reinterpret_cast<const zd *>(dA), lda, dResult,
+                             dScratch);
+}
+
+clm_status_t clm_lascl_d(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
+                         double *dA, size_t lda) {
+  return clm::lascl_impl<double>(stream, cfrom, cto, m, n, dA, lda);
+}
+
+clm_status_t clm_lascl_z(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
+                         clm_double_complex *dA, size_t lda) {
+  return clm::lascl_impl<zd>(stream, cfrom, cto, m, n, reinterpret_cast<zd *>(dA), lda);
+}
+
+int clm_paterson_stockmeyer_cost(int degree, int s) { return clm::ps_cost(degree, s); }
+int clm_paterson_stockmeyer_block_size(int degree) { return clm::ps_block_size(degree); }
+clm_ps_plan clm_paterson_stockmeyer_plan(int degree, int s_requested) {
+  return clm::ps_plan(degree, s_requested);
+}
+size_t clm_paterson_stockmeyer_power_stride_d(int n) { return clm::power_stride<double>(n); }
+size_t clm_paterson_stockmeyer_power_stride_z(int n) { return clm::power_stride<zd>(n); }
+size_t clm_paterson_stockmeyer_buffer_size_d(int n, int degree, int s_requested) {
+  return clm::ps_buffer_size<double>(n, degree, s_requested);
+}
+size_t clm_paterson_stockmeyer_buffer_size_z(int n, int degree, int s_requested) {
+  return clm::ps_buffer_size<zd>(n, degree, s_requested);
+}
+
+clm_status_t clm_paterson_stockmeyer_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
+                                       const double *dA, int lda, double *dP, int ldp,
+                                       void *dWork, size_t work_bytes, int s_requested) {
+  return clm::ps_impl<double>(ctx, n, dCoeffs, degree, dA, lda, dP, ldp, dWork, work_bytes,
+                              s_requested);
+}
+
+clm_status_t clm_paterson_stockmeyer_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs,
+                                       int degree, const clm_double_complex *dA, int lda,
+                                       clm_double_complex *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested) {
+  return clm::ps_impl<zd>(ctx, n, reinterpret_cast<const zd *>(dCoeffs), degree,
+                          reinterpret_cast<const zd *>(dA), lda, reinterpret_cast<zd *>(dP), ldp,
+                          dWork, work_bytes, s_requested);
+}
+
+size_t clm_horner_buffer_size_d(int n) { return clm::horner_buffer_size<double>(n); }
+size_t clm_horner_buffer_size_z(int n) { return clm::horner_buffer_size<zd>(n); }
+
+clm_status_t clm_horner_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
+                          const double *dA, int lda, double *dP, int ldp, void *dWork,
+                          size_t work_bytes) {
+  return clm::horner_impl<double>(ctx, n, dCoeffs, degree, dA, lda, dP, ldp, dWork, work_bytes);
+}
+
+clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs, int degree,
+                          const clm_double_complex *dA, int lda, clm_double_complex *dP, int ldp,
+                          void *dWork, size_t work_bytes) {
+  return clm::horner_impl<zd>(ctx, n, reinterpret_cast<const zd *>(dCoeffs), degree,
+                              reinterpret_cast<const zd *>(dA), lda, reinterpret_cast<zd *>(dP),
+                              ldp, dWork, work_bytes);
+}
+
+clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
+                        const double *dB, int ldb, double beta, double *dC, int ldc) {
+  return clm::gemm_impl<double>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+}
+
+clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex alpha,
+                        const clm_double_complex *dA, int lda, const clm_double_complex *dB,
+                        int ldb, clm_double_complex beta, clm_double_complex *dC, int ldc) {
+  return clm::gemm_impl<zd>(ctx, m, n, k, zd{alpha.re, alpha.im}, reinterpret_cast<const zd *>(dA),
+                            lda, reinterpret_cast<const zd *>(dB), ldb, zd{beta.re, beta.im},
+                            reinterpret_cast<zd *>(dC), ldc);
+}
+
+}  // extern "C"

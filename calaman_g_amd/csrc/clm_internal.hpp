@@ -1,0 +1,144 @@
+// clm_internal.hpp -- types, workspace layout and kernel launchers shared by the
+// HIP translation units of the MI355X matrix-exponential library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "calaman_expm.h"
+
+namespace clm {
+
+// n*n must fit an int in the reference's flat-index kernels (src/expm/detail.cppm:25-28);
+// the bound is kept so the two libraries accept the same orders.
+constexpr int kMaxDim = 46340;
+
+// Interleaved double complex, 16-byte aligned so one global/LDS access moves it.
+struct alignas(16) zd {
+  double re, im;
+};
+
+__host__ __device__ inline zd make_zd(double r, double i) { return zd{r, i}; }
+
+// ---- element arithmetic (real and complex share the kernels' templates) ----
+template <typename T>
+struct ops;
+
+template <>
+struct ops<double> {
+  using real = double;
+  __host__ __device__ static double zero() { return 0.0; }
+  __host__ __device__ static double from_real(double r) { return r; }
+  __host__ __device__ static double add(double a, double b) { return a + b; }
+  __host__ __device__ static double mul(double a, double b) { return a * b; }
+  // c * x + acc, the order the reference block build uses
+  __host__ __device__ static double mad(double c, double x, double acc) { return c * x + acc; }
+  __host__ __device__ static double scale(double x, double r) { return x * r; }
+  __host__ __device__ static bool is_zero(double a) { return a == 0.0; }
+};
+
+template <>
+struct ops<zd> {
+  using real = double;
+  __host__ __device__ static zd zero() { return zd{0.0, 0.0}; }
+  __host__ __device__ static zd from_real(double r) { return zd{r, 0.0}; }
+  __host__ __device__ static zd add(zd a, zd b) { return zd{a.re + b.re, a.im + b.im}; }
+  __host__ __device__ static zd mul(zd a, zd b) {
+    return zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  }
+  __host__ __device__ static zd mad(zd c, zd x, zd acc) { return add(mul(c, x), acc); }
+  __host__ __device__ static zd scale(zd x, double r) { return zd{x.re * r, x.im * r}; }
+  __host__ __device__ static bool is_zero(zd a) { return a.re == 0.0 && a.im == 0.0; }
+};
+
+// ---- workspace carve -----------------------------------------------------------
+// One caller-owned buffer carved into 256-byte aligned regions; a null base
+// sizes without carving, so a size query and the routine share one layout
+// (the reference's WorkspaceLayout, src/common/workspace.cppm:69-159).
+class Layout {
+ public:
+  explicit Layout(void *base) : base_(static_cast<char *>(base)) {}
+  template <typename U>
+  U *take(size_t count) {
+    offset_ = align(offset_);
+    U *p = base_ ? reinterpret_cast<U *>(base_ + offset_) : nullptr;
+    offset_ += count * sizeof(U);
+    return p;
+  }
+  size_t total() const { return align(offset_); }
+  static size_t align(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+ private:
+  char *base_;
+  size_t offset_ = 0;
+};
+
+// ---- GEMM ---------------------------------------------------------------------------
+// C = alpha * A * B [+ beta * Cin] [+ B_blk], column-major NN, where the optional
+// block term is the Paterson-Stockmeyer block
+//   B_blk = coef[0] I + coef[1] X1 + sum_{t>=2} coef[t] bank[t-2]
+// built in the epilogue (coef on the device). With nterms == 1 it is the Horner
+// step's coef[0] I.
+template <typename T>
+struct GemmArgs {
+  int M = 0, N = 0, K = 0;
+  const T *A = nullptr;
+  long lda = 0;
+  const T *B = nullptr;
+  long ldb = 0;
+  T *C = nullptr;
+  long ldc = 0;
+  T alpha{};
+  T beta{};
+  const T *Cin = nullptr;  // read when non-null (may alias C)
+  long ldcin = 0;
+  const T *coef = nullptr;  // block coefficients, device
+  int nterms = 0;
+  const T *X1 = nullptr;
+  long ldx1 = 0;
+  const T *bank = nullptr;
+  long bank_stride = 0;
+  long ldbank = 0;
+};
+
+hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g);
+hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g);
+template <typename T>
+hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);
+
+// ---- element-wise and reduction kernels ------------------------------------------------
+// 1-norm in two launches (colsums, NaN-propagating max into dColsum[0]).
+template <typename T>
+hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, double *colsum);
+// lange: which in CLM_NORM_*; result in *d_result.
+template <typename T>
+hipError_t launch_lange(hipStream_t stream, int which, size_t m, size_t n, const T *A, size_t lda,
+                        double *d_result, double *d_scratch);
+// out = f * in (n x n), f a real factor
+template <typename T>
+hipError_t launch_scale_copy(hipStream_t stream, int n, double f, const T *in, long ldi, T *out,
+                             long ldo);
+// A(i,j) *= mul, m x n in place
+template <typename T>
+hipError_t launch_lascl(hipStream_t stream, size_t m, size_t n, double mul, T *A, size_t lda);
+// B = coef[0] I + coef[1] X1 + sum_t coef[t] bank[t-2]
+template <typename T>
+hipError_t launch_build_block(hipStream_t stream, int n, T *Bout, long ldb, const T *coef,
+                              int nterms, const T *X1, long ldx1, const T *bank, long stride,
+                              long ldbank);
+// P = (*alpha) I
+template <typename T>
+hipError_t launch_set_scaled_identity(hipStream_t stream, int n, T *P, long ldp, const T *alpha);
+// dst = src (n x n)
+template <typename T>
+hipError_t launch_copy(hipStream_t stream, int n, const T *src, long lds, T *dst, long ldd);
+// d[0..count) = v
+hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v);
+
+// Device tables of 1/k!, k = 0..32 (correctly rounded), for the Taylor coefficients.
+const double *inv_factorials_d(int device);
+const zd *inv_factorials_z(int device);
+
+}  // namespace clm

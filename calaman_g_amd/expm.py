@@ -1,0 +1,319 @@
+"""Python host mirror of calaman_G's matrix-exponential interface over the C ABI.
+
+Names and argument meaning follow the reference's modules
+(src/expm/interface.cppm, src/paterson_stockmeyer/interface.cppm,
+src/horner/interface.cppm, src/lapack/lange, src/lapack/lascl); device memory
+and streams come from PyTorch (ROCm), the arithmetic from libcalaman_expm.so.
+
+Two layers:
+
+* ``*_cm`` functions take torch tensors purely as device buffers holding a
+  COLUMN-MAJOR matrix with a leading dimension -- the reference's calling
+  convention, one-to-one with the C entry points. A column-major n x n matrix
+  with leading dimension ld lives in a tensor of shape (n, ld) (row j holds
+  column j), or any tensor with at least ld*(n-1)+n elements.
+* convenience functions (``expm``, ``matrix_norm1``, ...) take ordinary
+  (row-major) torch matrices and return ordinary matrices; they transpose into
+  column-major storage so the plan is chosen from the 1-norm of the matrix
+  itself, as the reference does.
+
+Errors are raised as ``CalamanError`` carrying the library status.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _lib
+from ._lib import CalamanError, ExpmPlan, check, lib
+
+_F64 = torch.float64
+_C128 = torch.complex128
+
+
+def _suffix(dtype: torch.dtype) -> str:
+    if dtype == _F64:
+        return "d"
+    if dtype == _C128:
+        return "z"
+    raise TypeError(f"unsupported dtype {dtype}; float64 and complex128 are supported")
+
+
+def _stream_ptr(stream: Optional[torch.cuda.Stream]) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _require_device(t: torch.Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device (cuda/HIP) tensor")
+
+
+def to_cm(a: torch.Tensor) -> torch.Tensor:
+    """Column-major storage of a (row-major) matrix: returns a tensor whose memory
+    is a in column-major order (shape (ncols, nrows), row j = column j)."""
+    return a.t().contiguous()
+
+
+def from_cm(buf: torch.Tensor, n_rows: int, n_cols: int, ld: Optional[int] = None) -> torch.Tensor:
+    """The n_rows x n_cols matrix held column-major in buf (leading dimension ld)."""
+    ld = n_rows if ld is None else ld
+    flat = buf.reshape(-1)[: ld * (n_cols - 1) + n_rows] if n_cols > 0 else buf.reshape(-1)[:0]
+    return flat.as_strided((n_rows, n_cols), (1, ld))
+
+
+# ---- plans and sizes ------------------------------------------------------------------
+def expm_plan(norm1: float) -> dict:
+    """The plan expm() executes for a given 1-norm (the role of expm_plan<T>)."""
+    return lib().clm_expm_plan_d(float(norm1)).as_dict()
+
+
+def taylor_theta(m: int) -> float:
+    return float(lib().clm_taylor_theta_d(int(m)))
+
+
+def pade_theta(m: int) -> float:
+    """pade_theta<double> (src/expm/plan.cppm:246-281)."""
+    return float(lib().clm_pade_theta_d(int(m)))
+
+
+def pade_plan(norm1: float) -> dict:
+    """The reference's own expm_plan<double> (src/expm/plan.cppm:308-326)."""
+    return lib().clm_pade_plan_d(float(norm1)).as_dict()
+
+
+def expm_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    out = ctypes.c_size_t(0)
+    check(getattr(lib(), f"clm_expm_buffer_size_{_suffix(dtype)}")(int(n), ctypes.byref(out)),
+          "expm_buffer_size")
+    return int(out.value)
+
+
+def ps_plan(degree: int, s_requested: int = 0) -> dict:
+    return lib().clm_paterson_stockmeyer_plan(int(degree), int(s_requested)).as_dict()
+
+
+def ps_cost(degree: int, s: int) -> int:
+    return int(lib().clm_paterson_stockmeyer_cost(int(degree), int(s)))
+
+
+def ps_block_size(degree: int) -> int:
+    return int(lib().clm_paterson_stockmeyer_block_size(int(degree)))
+
+
+def ps_buffer_size(n: int, degree: int, s_requested: int = 0, dtype: torch.dtype = _F64) -> int:
+    f = getattr(lib(), f"clm_paterson_stockmeyer_buffer_size_{_suffix(dtype)}")
+    return int(f(int(n), int(degree), int(s_requested)))
+
+
+def ps_power_stride(n: int, dtype: torch.dtype = _F64) -> int:
+    f = getattr(lib(), f"clm_paterson_stockmeyer_power_stride_{_suffix(dtype)}")
+    return int(f(int(n)))
+
+
+def horner_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    return int(getattr(lib(), f"clm_horner_buffer_size_{_suffix(dtype)}")(int(n)))
+
+
+# ---- context -------------------------------------------------------------------------------
+class Context:
+    """clm_ctx: the stand-in for the reference's BLAS/solver handles; carries the
+    stream Paterson-Stockmeyer, Horner and gemm enqueue on."""
+
+    def __init__(self, stream: Optional[torch.cuda.Stream] = None):
+        h = ctypes.c_void_p()
+        check(lib().clm_create(ctypes.byref(h)), "clm_create")
+        self.handle = h
+        self.set_stream(stream)
+
+    def set_stream(self, stream: Optional[torch.cuda.Stream]) -> None:
+        check(lib().clm_set_stream(self.handle, ctypes.c_void_p(_stream_ptr(stream))),
+              "clm_set_stream")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().clm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts: dict = {}
+
+
+def _ctx_for(stream: Optional[torch.cuda.Stream]) -> Context:
+    key = (torch.cuda.current_device(), _stream_ptr(stream))
+    c = _contexts.get(key)
+    if c is None:
+        c = Context(stream)
+        _contexts[key] = c
+    return c
+
+
+# ---- column-major layer (one-to-one with the C entry points) ---------------------------------
+def expm_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int, dWork: torch.Tensor,
+            dInfo: Optional[torch.Tensor] = None, stream: Optional[torch.cuda.Stream] = None,
+            ctx: Optional[Context] = None) -> tuple:
+    """clm_expm_{d,z}; returns (status, plan dict)."""
+    plan = ExpmPlan()
+    f = getattr(lib(), f"clm_expm_{_suffix(dA.dtype)}")
+    st = f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA),
+           int(lda), _ptr(dE), int(lde), _ptr(dWork), dWork.numel() * dWork.element_size(),
+           _ptr(dInfo), ctypes.byref(plan))
+    return int(st), plan.as_dict()
+
+
+def matrix_norm1_cm(n: int, dA: torch.Tensor, lda: int, dColsum: torch.Tensor,
+                    stream: Optional[torch.cuda.Stream] = None) -> float:
+    out = ctypes.c_double(0.0)
+    f = getattr(lib(), f"clm_matrix_norm1_{_suffix(dA.dtype)}")
+    check(f(ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA), int(lda), _ptr(dColsum),
+            ctypes.byref(out)), "matrix_norm1")
+    return float(out.value)
+
+
+def lange_cm(which: int, m: int, n: int, dA: torch.Tensor, lda: int, dResult: torch.Tensor,
+             dScratch: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> int:
+    f = getattr(lib(), f"clm_lange_{_suffix(dA.dtype)}")
+    return int(f(ctypes.c_void_p(_stream_ptr(stream)), int(which), int(m), int(n), _ptr(dA),
+                 int(lda), _ptr(dResult), _ptr(dScratch)))
+
+
+def lascl_cm(cfrom: float, cto: float, m: int, n: int, dA: torch.Tensor, lda: int,
+             stream: Optional[torch.cuda.Stream] = None) -> int:
+    f = getattr(lib(), f"clm_lascl_{_suffix(dA.dtype)}")
+    return int(f(ctypes.c_void_p(_stream_ptr(stream)), float(cfrom), float(cto), int(m), int(n),
+                 _ptr(dA), int(lda)))
+
+
+def paterson_stockmeyer_cm(ctx: Context, n: int, dCoeffs: torch.Tensor, degree: int,
+                           dA: torch.Tensor, lda: int, dP: torch.Tensor, ldp: int,
+                           dWork: Optional[torch.Tensor], s_requested: int = 0) -> int:
+    f = getattr(lib(), f"clm_paterson_stockmeyer_{_suffix(dA.dtype)}")
+    wb = 0 if dWork is None else dWork.numel() * dWork.element_size()
+    return int(f(ctx.handle, int(n), _ptr(dCoeffs), int(degree), _ptr(dA), int(lda), _ptr(dP),
+                 int(ldp), _ptr(dWork), wb, int(s_requested)))
+
+
+def horner_cm(ctx: Context, n: int, dCoeffs: torch.Tensor, degree: int, dA: torch.Tensor,
+              lda: int, dP: torch.Tensor, ldp: int, dWork: Optional[torch.Tensor]) -> int:
+    f = getattr(lib(), f"clm_horner_{_suffix(dA.dtype)}")
+    wb = 0 if dWork is None else dWork.numel() * dWork.element_size()
+    return int(f(ctx.handle, int(n), _ptr(dCoeffs), int(degree), _ptr(dA), int(lda), _ptr(dP),
+                 int(ldp), _ptr(dWork), wb))
+
+
+def gemm_cm(ctx: Context, m: int, n: int, k: int, alpha, dA: torch.Tensor, lda: int,
+            dB: torch.Tensor, ldb: int, beta, dC: torch.Tensor, ldc: int) -> int:
+    if dA.dtype == _C128:
+        a = _lib.DoubleComplex(complex(alpha).real, complex(alpha).imag)
+        b = _lib.DoubleComplex(complex(beta).real, complex(beta).imag)
+        return int(lib().clm_gemm_z(ctx.handle, m, n, k, a, _ptr(dA), lda, _ptr(dB), ldb, b,
+                                    _ptr(dC), ldc))
+    return int(lib().clm_gemm_d(ctx.handle, m, n, k, float(alpha), _ptr(dA), lda, _ptr(dB), ldb,
+                                float(beta), _ptr(dC), ldc))
+
+
+# ---- convenience layer (row-major torch matrices) ----------------------------------------------
+def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
+         return_plan: bool = False):
+    """exp(a) for a square float64/complex128 device matrix."""
+    _require_device(a, "a")
+    n = a.shape[-1]
+    if a.dim() != 2 or a.shape[0] != n:
+        raise ValueError("expm expects a square matrix")
+    acm = to_cm(a)
+    ecm = torch.empty_like(acm)
+    work = torch.empty(expm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+    info = torch.empty(2, dtype=torch.int32, device=a.device)
+    st, plan = expm_cm(n, acm, n, ecm, n, work, info, stream)
+    check(st, "expm")
+    e = ecm.t()
+    return (e, plan) if return_plan else e
+
+
+def matrix_norm1(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> float:
+    _require_device(a, "a")
+    n = a.shape[0]
+    acm = to_cm(a)
+    colsum = torch.empty(n, dtype=_F64, device=a.device)
+    return matrix_norm1_cm(n, acm, n, colsum, stream)
+
+
+def lange(which: str, a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+    """The 'max_abs' | 'one' | 'inf' | 'frobenius' norm of a, as a device scalar."""
+    code = {"max_abs": 0, "one": 1, "inf": 2, "frobenius": 3}[which]
+    m, n = a.shape
+    acm = to_cm(a)
+    out = torch.empty(1, dtype=_F64, device=a.device)
+    scratch = torch.empty(max(m, n, 1), dtype=_F64, device=a.device)
+    check(lange_cm(code, m, n, acm, max(m, 1), out, scratch, stream), "lange")
+    return out[0]
+
+
+def lascl(cfrom: float, cto: float, a: torch.Tensor,
+          stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+    """a * (cto / cfrom) through ?lascl's guarded multipliers (returns a new matrix)."""
+    m, n = a.shape
+    acm = to_cm(a)
+    check(lascl_cm(cfrom, cto, m, n, acm, max(m, 1), stream), "lascl")
+    return acm.t()
+
+
+def paterson_stockmeyer(coeffs: torch.Tensor, a: torch.Tensor, s_requested: int = 0,
+                        stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+    n = a.shape[0]
+    degree = coeffs.numel() - 1
+    ctx = _ctx_for(stream)
+    acm = to_cm(a)
+    pcm = torch.empty_like(acm)
+    nbytes = ps_buffer_size(n, degree, s_requested, a.dtype)
+    work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=a.device)
+    check(paterson_stockmeyer_cm(ctx, n, coeffs.contiguous(), degree, acm, n, pcm, n, work,
+                                 s_requested), "paterson_stockmeyer")
+    return pcm.t()
+
+
+def horner(coeffs: torch.Tensor, a: torch.Tensor,
+           stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+    n = a.shape[0]
+    degree = coeffs.numel() - 1
+    ctx = _ctx_for(stream)
+    acm = to_cm(a)
+    pcm = torch.empty_like(acm)
+    work = torch.empty(horner_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+    check(horner_cm(ctx, n, coeffs.contiguous(), degree, acm, n, pcm, n, work), "horner")
+    return pcm.t()
+
+
+def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torch.Tensor] = None,
+         stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+    m, k = a.shape
+    k2, n = b.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    ctx = _ctx_for(stream)
+    acm, bcm = to_cm(a), to_cm(b)
+    ccm = to_cm(c) if c is not None else torch.zeros((n, m), dtype=a.dtype, device=a.device)
+    check(gemm_cm(ctx, m, n, k, alpha, acm, max(m, 1), bcm, max(k, 1), beta, ccm, max(m, 1)),
+          "gemm")
+    return ccm.t()
+
+
+__all__ = [
+    "CalamanError", "Context", "expm", "expm_cm", "expm_plan", "expm_buffer_size", "taylor_theta",
+    "pade_theta", "pade_plan", "matrix_norm1", "matrix_norm1_cm", "lange", "lange_cm", "lascl",
+    "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
+    "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",
+    "horner_buffer_size", "to_cm", "from_cm",
+]

@@ -1,0 +1,228 @@
+/*
+ * calaman_expm.h -- C ABI of the MI355X (gfx950) matrix-exponential library.
+ *
+ * Drop-in boundary for calaman_G's matrix-exponential path. Each entry point
+ * names the reference interface it replaces (paths relative to the calaman_G
+ * source root). The reference's C++ templates over {float, double,
+ * complex<float>, complex<double>} become type-suffixed C functions here
+ * (d = double, z = complex<double>); the C++ header calaman/expm.hpp restores
+ * the template spelling on top of these.
+ *
+ * Conventions kept from the reference:
+ *   - matrices are column-major with a leading dimension >= n; only the n x n
+ *     block of an output is written, padding rows are left untouched;
+ *   - the caller owns every device buffer; the workspace is sized by the
+ *     *_buffer_size query and passed in, nothing on the expm path allocates;
+ *   - all work is enqueued on one stream; clm_expm_* and clm_matrix_norm1_*
+ *     synchronize that stream once (to read the 1-norm that picks the plan);
+ *     every other entry point is asynchronous;
+ *   - a bad argument or a non-finite 1-norm is CLM_STATUS_INVALID_VALUE, an
+ *     undersized workspace CLM_STATUS_ALLOC_FAILED (src/expm/expm.cppm:77-80);
+ *     Paterson-Stockmeyer and Horner report every bad argument as
+ *     CLM_STATUS_NOT_INITIALIZED (src/paterson_stockmeyer/interface.cppm:333-345,
+ *     src/horner/interface.cppm:160-164); lascl reports a bad ratio in the
+ *     runtime domain (src/lapack/lascl/interface.cppm:90-95).
+ *
+ * No argument-checking path touches the device: a rejected call returns before
+ * any HIP call, so it can be exercised on a host without a GPU.
+ */
+#ifndef CALAMAN_EXPM_H
+#define CALAMAN_EXPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- element types ---------------------------------------------------------
+ * Interleaved complex, layout-compatible with std::complex<T>,
+ * hipDoubleComplex/hipFloatComplex and numpy complex128/complex64. */
+typedef struct clm_double_complex {
+  double re, im;
+} clm_double_complex;
+
+/* A HIP stream (hipStream_t) as an opaque pointer; NULL is the null stream. */
+typedef void *clm_stream_t;
+
+/* ---- status ------------------------------------------------------------------
+ * calaman::Status{domain, code} (src/error_handling/error_handling.cppm:88-180)
+ * flattened into one int: 0 is success in every domain, otherwise
+ * domain | code. */
+typedef int clm_status_t;
+#define CLM_DOMAIN_MASK 0x7fff0000
+#define CLM_DOMAIN_BLAS 0x00010000
+#define CLM_DOMAIN_SOLVER 0x00020000
+#define CLM_DOMAIN_RUNTIME 0x00030000
+#define CLM_STATUS_SUCCESS 0
+#define CLM_STATUS_NOT_INITIALIZED (CLM_DOMAIN_BLAS | 1)
+#define CLM_STATUS_ALLOC_FAILED (CLM_DOMAIN_BLAS | 3)
+#define CLM_STATUS_INVALID_VALUE (CLM_DOMAIN_BLAS | 7)
+#define CLM_STATUS_EXECUTION_FAILED (CLM_DOMAIN_BLAS | 13)
+#define CLM_STATUS_NOT_SUPPORTED (CLM_DOMAIN_BLAS | 15)
+#define CLM_STATUS_RUNTIME_INVALID_VALUE (CLM_DOMAIN_RUNTIME | 1)
+#define CLM_STATUS_RUNTIME_ERROR (CLM_DOMAIN_RUNTIME | 2)
+
+/* Status name, e.g. "CLM_STATUS_INVALID_VALUE"; never NULL. */
+const char *clm_status_string(clm_status_t status);
+
+/* ---- context -----------------------------------------------------------------
+ * Replaces the reference's (wwrblasHandle_t, wwrsolverDnHandle_t) pair. A
+ * context is bound to the HIP device current at creation and carries the
+ * stream that clm_paterson_stockmeyer_* / clm_horner_* / clm_gemm_* enqueue on
+ * (the reference takes that stream from the BLAS handle,
+ * src/paterson_stockmeyer/interface.cppm:349-350). Everywhere a clm_ctx* is
+ * taken, NULL means the library's default context for the current device
+ * (null stream). One context per host thread, as with vendor handles. */
+typedef struct clm_ctx clm_ctx;
+clm_status_t clm_create(clm_ctx **ctx);
+clm_status_t clm_destroy(clm_ctx *ctx);
+clm_status_t clm_set_stream(clm_ctx *ctx, clm_stream_t stream);
+clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream);
+
+/* ---- plans -------------------------------------------------------------------
+ * ExpmPlan{m, s, num_gemms} (src/expm/plan.cppm:94-98), extended with the
+ * method, the Paterson-Stockmeyer block size, the element type and the 1-norm
+ * the plan was chosen from. */
+#define CLM_METHOD_TAYLOR_PS 0 /* Taylor series by Paterson-Stockmeyer (this library) */
+#define CLM_METHOD_PADE 1      /* diagonal Pade + LU (the reference's route; plan only) */
+#define CLM_DTYPE_F64 1
+#define CLM_DTYPE_C128 3
+typedef struct clm_expm_plan {
+  int m;         /* degree: Taylor degree, or Pade degree for CLM_METHOD_PADE */
+  int s;         /* squarings: the matrix is evaluated at A / 2^s */
+  int num_gemms; /* matrix products the evaluation issues */
+  int method;    /* CLM_METHOD_* */
+  int ps_block;  /* Paterson-Stockmeyer block size (0 for Pade) */
+  int dtype;     /* CLM_DTYPE_* (0 when only predicted from a norm) */
+  double norm1;  /* the 1-norm the plan was chosen from */
+} clm_expm_plan;
+
+/* The plan clm_expm_d/z execute for a given 1-norm (the role of expm_plan<T>,
+ * src/expm/plan.cppm:308-326): over the Taylor degrees
+ * {1,2,3,5,8,11,15,19,24,29}, minimise Paterson-Stockmeyer products + s with
+ * s = max(0, ceil(log2(norm1 / theta_m))), ties to the smaller s. */
+clm_expm_plan clm_expm_plan_d(double norm1);
+/* Backward-error threshold theta_m of the degree-m Taylor approximant at
+ * u = 2^-53 (0 for a degree off the ladder); the counterpart of pade_theta<T>
+ * (src/expm/plan.cppm:246-281). */
+double clm_taylor_theta_d(int m);
+/* The reference's own Pade ladder, restated for callers that predict its cost:
+ * pade_theta<double> and expm_plan<double> (src/expm/plan.cppm:246-326). */
+double clm_pade_theta_d(int m);
+clm_expm_plan clm_pade_plan_d(double norm1);
+
+/* ---- expm --------------------------------------------------------------------
+ * expm_bufferSize<T> (src/expm/buffer_size.cppm:242-262): bytes of device
+ * workspace clm_expm_* needs for order n (n >= 1). */
+clm_status_t clm_expm_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_buffer_size_z(int n, size_t *bytes);
+
+/* expm<T> (src/expm/expm.cppm:87-174): E = exp(A) by scaling and squaring.
+ *   ctx      NULL or a context (stands in for both reference handles)
+ *   stream   the stream all work is enqueued on; synchronized once
+ *   dA       n x n input, leading dimension lda >= n; must not overlap dE
+ *   dE       n x n output, leading dimension lde >= n
+ *   dWork    >= clm_expm_buffer_size_* bytes of device memory, 256-B aligned
+ *   dInfo    device int[2]; the reference's getrf/getrs info. The Taylor
+ *            route needs no LU, so both are written 0.
+ *   plan     optional: the executed plan
+ * n must lie in [1, 46340] (src/expm/detail.cppm:25-28). */
+clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                        double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                        clm_expm_plan *plan);
+clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_double_complex *dA,
+                        int lda, clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes,
+                        int *dInfo, clm_expm_plan *plan);
+
+/* ---- 1-norm --------------------------------------------------------------------
+ * matrix_norm1<T> (src/expm/norm1.cppm:417-428): max_j sum_i |A(i,j)| with the
+ * true complex modulus; NaN propagates. dColsum is device scratch of n
+ * doubles. Synchronizes the stream and writes the norm to *norm (host). */
+clm_status_t clm_matrix_norm1_d(clm_stream_t stream, int n, const double *dA, int lda,
+                                double *dColsum, double *norm);
+clm_status_t clm_matrix_norm1_z(clm_stream_t stream, int n, const clm_double_complex *dA, int lda,
+                                double *dColsum, double *norm);
+
+/* lange<T> (src/lapack/lange/interface.cppm:82-107): the `which` norm of an
+ * m x n matrix into the DEVICE scalar dResult, asynchronously; 0 for an empty
+ * matrix. which: CLM_NORM_MAX_ABS, CLM_NORM_ONE, CLM_NORM_INF,
+ * CLM_NORM_FROBENIUS (src/common/enums.h:154-159). The reference allocates its
+ * per-column scratch on the stream; here the caller passes dScratch of
+ * max(m, n) doubles. */
+#define CLM_NORM_MAX_ABS 0
+#define CLM_NORM_ONE 1
+#define CLM_NORM_INF 2
+#define CLM_NORM_FROBENIUS 3
+clm_status_t clm_lange_d(clm_stream_t stream, int which, size_t m, size_t n, const double *dA,
+                         size_t lda, double *dResult, double *dScratch);
+clm_status_t clm_lange_z(clm_stream_t stream, int which, size_t m, size_t n,
+                         const clm_double_complex *dA, size_t lda, double *dResult,
+                         double *dScratch);
+
+/* lascl<T> TYPE='G' (src/lapack/lascl/interface.cppm:79-143): A *= cto/cfrom
+ * through netlib's over/underflow-guarded chain of multipliers, in place,
+ * asynchronously. A zero or NaN cfrom, or a NaN cto, is
+ * CLM_STATUS_RUNTIME_INVALID_VALUE. */
+clm_status_t clm_lascl_d(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
+                         double *dA, size_t lda);
+clm_status_t clm_lascl_z(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
+                         clm_double_complex *dA, size_t lda);
+
+/* ---- Paterson-Stockmeyer -----------------------------------------------------------
+ * PatersonStockmeyerPlan and its helpers
+ * (src/paterson_stockmeyer/interface.cppm:117-228). */
+typedef struct clm_ps_plan {
+  int s;          /* block size */
+  int r;          /* Horner steps over X = A^s */
+  int num_powers; /* powers held: A^2 .. A^(num_powers+1) */
+  int num_blocks; /* n x n workspace blocks */
+  int num_gemms;  /* matrix products */
+} clm_ps_plan;
+int clm_paterson_stockmeyer_cost(int degree, int s);
+int clm_paterson_stockmeyer_block_size(int degree);
+clm_ps_plan clm_paterson_stockmeyer_plan(int degree, int s_requested);
+size_t clm_paterson_stockmeyer_power_stride_d(int n);
+size_t clm_paterson_stockmeyer_power_stride_z(int n);
+size_t clm_paterson_stockmeyer_buffer_size_d(int n, int degree, int s_requested);
+size_t clm_paterson_stockmeyer_buffer_size_z(int n, int degree, int s_requested);
+
+/* paterson_stockmeyer<T> (src/paterson_stockmeyer/interface.cppm:329-422):
+ * P = sum_{k=0}^{degree} c_k A^k, coefficients ascending in DEVICE memory,
+ * asynchronous on the context's stream. */
+clm_status_t clm_paterson_stockmeyer_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
+                                       const double *dA, int lda, double *dP, int ldp,
+                                       void *dWork, size_t work_bytes, int s_requested);
+clm_status_t clm_paterson_stockmeyer_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs,
+                                       int degree, const clm_double_complex *dA, int lda,
+                                       clm_double_complex *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested);
+
+/* horner_bufferSize<T> / horner<T> (src/horner/interface.cppm:103-194). */
+size_t clm_horner_buffer_size_d(int n);
+size_t clm_horner_buffer_size_z(int n);
+clm_status_t clm_horner_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
+                          const double *dA, int lda, double *dP, int ldp, void *dWork,
+                          size_t work_bytes);
+clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs, int degree,
+                          const clm_double_complex *dA, int lda, clm_double_complex *dP, int ldp,
+                          void *dWork, size_t work_bytes);
+
+/* ---- matrix product -----------------------------------------------------------------
+ * The NN product the reference takes from wwr::gemm (src/expm/pade.cppm:114-124,
+ * src/paterson_stockmeyer/interface.cppm:372-373): C = alpha*A*B + beta*C for
+ * an m x k A and a k x n B, on the MFMA kernels of this library, asynchronous
+ * on the context's stream. beta == 0 never reads C. */
+clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
+                        const double *dB, int ldb, double beta, double *dC, int ldc);
+clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex alpha,
+                        const clm_double_complex *dA, int lda, const clm_double_complex *dB,
+                        int ldb, clm_double_complex beta, clm_double_complex *dC, int ldc);
+
+/* Library version string. */
+const char *clm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CALAMAN_EXPM_H */

@@ -1,0 +1,311 @@
+"""CPU oracle for calaman_G's matrix-exponential path -- TEST INFRASTRUCTURE.
+
+This package is the checker, never the thing measured or shipped: only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
+import it. It wraps ``liboracle.so`` (``oracle.c``, a C restatement of the
+reference's Pade ladder, norms, ?lascl, Paterson-Stockmeyer and Horner, plus
+the build's Taylor/Paterson-Stockmeyer algorithm) with numpy-facing helpers.
+
+Matrices are passed as numpy arrays in Fortran (column-major) order; complex
+matrices as complex128, whose memory layout is the interleaved (re, im) pairs
+the C code expects.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, no GPU)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_pade_coeffs.restype = _dp
+        L.orc_pade_theta_d.restype = ctypes.c_double
+        L.orc_pade_theta_s.restype = ctypes.c_float
+        L.orc_pade_theta_s.argtypes = [ctypes.c_int]
+        L.orc_expm_plan_d.argtypes = [ctypes.c_double, _ip]
+        L.orc_expm_plan_s.argtypes = [ctypes.c_float, _ip]
+        for f in ("orc_norm1_d", "orc_norm1_z"):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+        for f in ("orc_lange_d", "orc_lange_z"):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, _dp,
+                                      ctypes.c_size_t]
+        for f in ("orc_lascl_d", "orc_lascl_z"):
+            getattr(L, f).argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_size_t,
+                                      ctypes.c_size_t, _dp, ctypes.c_size_t]
+        L.orc_gemm_d.argtypes = [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp,
+                                 ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int]
+        L.orc_gemm_z.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                 _dp, ctypes.c_int]
+        for f in ("orc_getrf_d", "orc_getrf_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip]
+        for f in ("orc_getrs_d", "orc_getrs_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp,
+                                      ctypes.c_int]
+        for f in ("orc_pade_d", "orc_pade_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                      ctypes.c_int, _ip]
+        for f in ("orc_expm_d", "orc_expm_z", "orc_expm_taylor_d", "orc_expm_taylor_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip]
+        L.orc_ps_plan.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
+        for f in ("orc_ps_d", "orc_ps_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                      ctypes.c_int, ctypes.c_int]
+        for f in ("orc_horner_d", "orc_horner_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                      ctypes.c_int]
+        for f in ("orc_taylor_theta_d", "orc_taylor_theta_s"):
+            getattr(L, f).restype = ctypes.c_double
+        L.orc_inv_factorials.restype = _dp
+        L.orc_taylor_plan_d.argtypes = [ctypes.c_double, _ip]
+        L.orc_taylor_plan_s.argtypes = [ctypes.c_double, _ip]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _is_complex(a) -> bool:
+    return np.iscomplexobj(a)
+
+
+def set_threads(t: int) -> None:
+    lib().orc_set_num_threads(int(t))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+# ---- Pade ladder -------------------------------------------------------------
+PADE_DEGREES = (3, 5, 7, 9, 13)
+
+
+def pade_coeffs(m: int):
+    p = lib().orc_pade_coeffs(m)
+    return None if not p else [p[j] for j in range(m + 1)]
+
+
+def pade_theta(m: int, single: bool = False) -> float:
+    return float(lib().orc_pade_theta_s(m) if single else lib().orc_pade_theta_d(m))
+
+
+def pade_num_gemms(m: int) -> int:
+    return int(lib().orc_pade_num_gemms(m))
+
+
+def expm_plan(norm1: float, single: bool = False):
+    out = (ctypes.c_int * 3)()
+    if single:
+        lib().orc_expm_plan_s(ctypes.c_float(norm1), out)
+    else:
+        lib().orc_expm_plan_d(norm1, out)
+    return tuple(out)
+
+
+# ---- norms -----------------------------------------------------------------
+def norm1(A) -> float:
+    c = _is_complex(A)
+    A = _fortran(A, np.complex128 if c else np.float64)
+    n = A.shape[0]
+    f = lib().orc_norm1_z if c else lib().orc_norm1_d
+    return float(f(n, _ptr(A), n))
+
+
+LANGE = {"max_abs": 0, "one": 1, "inf": 2, "frobenius": 3}
+
+
+def lange(which: str, A) -> float:
+    c = _is_complex(A)
+    A = _fortran(A, np.complex128 if c else np.float64)
+    m, n = A.shape
+    f = lib().orc_lange_z if c else lib().orc_lange_d
+    return float(f(LANGE[which], m, n, _ptr(A), max(m, 1)))
+
+
+def lascl(cfrom: float, cto: float, A):
+    """Returns (status, scaled copy)."""
+    c = _is_complex(A)
+    A = _fortran(A, np.complex128 if c else np.float64).copy(order="F")
+    m, n = A.shape
+    f = lib().orc_lascl_z if c else lib().orc_lascl_d
+    st = f(cfrom, cto, m, n, _ptr(A), max(m, 1))
+    return int(st), A
+
+
+# ---- BLAS / LAPACK pieces -------------------------------------------------------
+def gemm(A, B, alpha=1.0, beta=0.0, C=None):
+    c = _is_complex(A) or _is_complex(B) or (C is not None and _is_complex(C))
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    B = _fortran(B, dt)
+    n = A.shape[0]
+    C = np.zeros((n, n), dtype=dt, order="F") if C is None else _fortran(C, dt).copy(order="F")
+    if c:
+        al = np.array([complex(alpha).real, complex(alpha).imag])
+        be = np.array([complex(beta).real, complex(beta).imag])
+        lib().orc_gemm_z(n, _ptr(al), _ptr(A), n, _ptr(B), n, _ptr(be), _ptr(C), n)
+    else:
+        lib().orc_gemm_d(n, float(alpha), _ptr(A), n, _ptr(B), n, float(beta), _ptr(C), n)
+    return C
+
+
+def lu_solve(A, B):
+    """Solve A X = B with the oracle's getrf/getrs. Returns (X, info)."""
+    c = _is_complex(A) or _is_complex(B)
+    dt = np.complex128 if c else np.float64
+    LU = _fortran(A, dt).copy(order="F")
+    X = _fortran(B, dt).copy(order="F")
+    n = LU.shape[0]
+    nrhs = X.shape[1]
+    ipiv = (ctypes.c_int * n)()
+    if c:
+        info = lib().orc_getrf_z(n, _ptr(LU), n, ipiv)
+        lib().orc_getrs_z(n, nrhs, _ptr(LU), n, ipiv, _ptr(X), n)
+    else:
+        info = lib().orc_getrf_d(n, _ptr(LU), n, ipiv)
+        lib().orc_getrs_d(n, nrhs, _ptr(LU), n, ipiv, _ptr(X), n)
+    return X, int(info)
+
+
+# ---- Pade / expm ------------------------------------------------------------------
+def pade(m: int, A):
+    c = _is_complex(A)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    n = A.shape[0]
+    R = np.zeros((n, n), dtype=dt, order="F")
+    info = (ctypes.c_int * 2)()
+    f = lib().orc_pade_z if c else lib().orc_pade_d
+    st = f(m, n, _ptr(A), n, _ptr(R), n, info)
+    if st != 0:
+        raise ValueError(f"orc_pade status {st}")
+    return R
+
+
+def expm(A, return_plan: bool = False):
+    """The reference expm restated: Pade ladder + scaling and squaring."""
+    c = _is_complex(A)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    n = A.shape[0]
+    E = np.zeros((n, n), dtype=dt, order="F")
+    plan = (ctypes.c_int * 3)()
+    f = lib().orc_expm_z if c else lib().orc_expm_d
+    st = f(n, _ptr(A), n, _ptr(E), n, plan)
+    if st != 0:
+        raise ValueError(f"orc_expm status {st}")
+    return (E, tuple(plan)) if return_plan else E
+
+
+def expm_status(A) -> int:
+    c = _is_complex(A)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    n = A.shape[0]
+    E = np.zeros((n, n), dtype=dt, order="F")
+    f = lib().orc_expm_z if c else lib().orc_expm_d
+    return int(f(n, _ptr(A), n, _ptr(E), n, None))
+
+
+def expm_taylor(A, return_plan: bool = False):
+    """The build's algorithm on the CPU: Taylor / Paterson-Stockmeyer + squaring."""
+    c = _is_complex(A)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    n = A.shape[0]
+    E = np.zeros((n, n), dtype=dt, order="F")
+    plan = (ctypes.c_int * 4)()
+    f = lib().orc_expm_taylor_z if c else lib().orc_expm_taylor_d
+    st = f(n, _ptr(A), n, _ptr(E), n, plan)
+    if st != 0:
+        raise ValueError(f"orc_expm_taylor status {st}")
+    return (E, tuple(plan)) if return_plan else E
+
+
+# ---- polynomials ------------------------------------------------------------------
+def ps_plan(degree: int, s_requested: int = 0):
+    out = (ctypes.c_int * 5)()
+    lib().orc_ps_plan(degree, s_requested, out)
+    return dict(zip(("s", "r", "num_powers", "num_blocks", "num_gemms"), tuple(out)))
+
+
+def ps_cost(degree: int, s: int) -> int:
+    return int(lib().orc_ps_cost(degree, s))
+
+
+def paterson_stockmeyer(coeffs, A, s_requested: int = 0):
+    c = _is_complex(A) or _is_complex(coeffs)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    co = np.ascontiguousarray(np.asarray(coeffs, dtype=dt))
+    n = A.shape[0]
+    P = np.zeros((n, n), dtype=dt, order="F")
+    f = lib().orc_ps_z if c else lib().orc_ps_d
+    st = f(n, _ptr(co), len(co) - 1, _ptr(A), n, _ptr(P), n, s_requested)
+    if st != 0:
+        raise ValueError(f"orc_ps status {st}")
+    return P
+
+
+def horner(coeffs, A):
+    c = _is_complex(A) or _is_complex(coeffs)
+    dt = np.complex128 if c else np.float64
+    A = _fortran(A, dt)
+    co = np.ascontiguousarray(np.asarray(coeffs, dtype=dt))
+    n = A.shape[0]
+    P = np.zeros((n, n), dtype=dt, order="F")
+    f = lib().orc_horner_z if c else lib().orc_horner_d
+    st = f(n, _ptr(co), len(co) - 1, _ptr(A), n, _ptr(P), n)
+    if st != 0:
+        raise ValueError(f"orc_horner status {st}")
+    return P
+
+
+# ---- Taylor ladder -------------------------------------------------------------------
+TAYLOR_DEGREES = (1, 2, 3, 5, 8, 11, 15, 19, 24, 29)
+
+
+def taylor_theta(degree: int, single: bool = False) -> float:
+    f = lib().orc_taylor_theta_s if single else lib().orc_taylor_theta_d
+    return float(f(degree))
+
+
+def taylor_plan(norm1: float, single: bool = False):
+    out = (ctypes.c_int * 4)()
+    (lib().orc_taylor_plan_s if single else lib().orc_taylor_plan_d)(norm1, out)
+    return dict(zip(("degree", "s", "num_gemms", "ps_block"), tuple(out)))
+
+
+def inv_factorials():
+    p = lib().orc_inv_factorials()
+    return [p[k] for k in range(33)]

@@ -1,0 +1,206 @@
+"""Benchmark: expm/s at n = 2048 on one MI355X.
+
+Workload (the headline configuration): plain FP64 real expm of dense n = 2048
+matrices, entries N(0,1) from numpy's default_rng(seed), rescaled to
+||A||_1 in {0.5, 8, 100} (so s = 0 .. 5 squarings). One step = one expm of each
+of the three matrices through the C ABI (clm_expm_d), inputs already resident
+in HBM; value = expm/s over the timed steps. complex128 at the same sizes is
+reported in "breakdown".
+
+Also reported, on the same JSON line:
+  roofline      the dominant kernel (the 128x128 FP64 MFMA GEMM that every
+                polynomial product and squaring runs on): 2 n^3 FLOP per launch
+                over its mean launch time, timed with HIP events on the launch
+                stream, against the 78.6 TFLOP/s FP64 matrix peak;
+  cpu_baseline  the CPU oracle (the C restatement of the reference's Pade expm)
+                timed on this host's cores on one expm per norm.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 2048
+NORMS = (0.5, 8.0, 100.0)
+SEED = 0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(n, cplx):
+    import numpy as np
+    mats = []
+    for i, t in enumerate(NORMS):
+        rng = np.random.default_rng(SEED + i)
+        a = rng.standard_normal((n, n))
+        if cplx:
+            a = a + 1j * rng.standard_normal((n, n))
+        a *= t / np.abs(a).sum(0).max()
+        mats.append(a)
+    return mats
+
+
+def time_expms(mats, dev, steps, warmup):
+    """Returns (seconds per step, plans, results of the last step)."""
+    import numpy as np
+    import torch
+    import calaman_g_amd as cg
+    from calaman_g_amd.expm import expm_cm
+
+    n = mats[0].shape[0]
+    dt = torch.complex128 if np.iscomplexobj(mats[0]) else torch.float64
+    das = [torch.from_numpy(np.ascontiguousarray(a.T)).to(dev) for a in mats]
+    des = [torch.empty_like(d) for d in das]
+    work = torch.empty(cg.expm_buffer_size(n, dt), dtype=torch.uint8, device=dev)
+    info = torch.empty(2, dtype=torch.int32, device=dev)
+    plans = []
+
+    def step():
+        plans.clear()
+        for da, de in zip(das, des):
+            st, plan = expm_cm(n, da, n, de, n, work, info)
+            if st != 0:
+                raise RuntimeError(f"clm_expm failed: {st:#x}")
+            plans.append(plan)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = e0.elapsed_time(e1) * 1e-3
+    results = [d.cpu().numpy().T for d in des]
+    return max(wall, ev) / steps, [dict(p) for p in plans], results
+
+
+def gemm_roofline(dev, n=N, reps=20):
+    """Mean duration of the FP64 GEMM kernel at n, HIP events on its stream."""
+    import torch
+    from calaman_g_amd.expm import Context, gemm_cm
+
+    ctx = Context()  # enqueues on torch's current stream
+    a = torch.randn(n, n, dtype=torch.float64, device=dev)
+    b = torch.randn(n, n, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    for _ in range(3):
+        gemm_cm(ctx, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        gemm_cm(ctx, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    achieved = 2.0 * n ** 3 / t / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "dgemm_n2048_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    return {"kernel": "dgemm_mfma<128,128,64,64> n=2048", "bound": "mfma",
+            "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+            "mean_launch_ms": round(t * 1e3, 4), "flop_per_launch": 2 * n ** 3}
+
+
+def cpu_baseline(mats):
+    import oracle
+    threads = oracle.num_threads()
+    t0 = time.perf_counter()
+    outs = [oracle.expm(a) for a in mats]
+    dt = time.perf_counter() - t0
+    return {"value": round(len(mats) / dt, 4), "unit": "expm/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one n={N} FP64 expm at each ||A||_1 in {list(NORMS)} ({len(mats)} expms, "
+                      f"{dt:.1f} s): the oracle's C restatement of the reference Pade expm, "
+                      f"OpenMP {threads} threads"}, outs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(requested --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    log(f"device: {torch.cuda.get_device_name(0)}")
+    real = make_inputs(N, cplx=False)
+    sec, plans, results = time_expms(real, dev, args.steps, args.warmup)
+    expm_per_s = len(NORMS) / sec
+    log(f"fp64: {sec * 1e3:.3f} ms/step, {expm_per_s:.1f} expm/s, plans {plans}")
+
+    cplx = make_inputs(N, cplx=True)
+    zsec, zplans, _ = time_expms(cplx, dev, max(2, args.steps // 4), 1)
+    log(f"complex128: {zsec * 1e3:.3f} ms/step, {len(NORMS) / zsec:.1f} expm/s")
+
+    roof = gemm_roofline(dev)
+    log(f"roofline: {roof}")
+
+    cpu = None
+    errs = None
+    if not args.no_cpu:
+        cpu, outs = cpu_baseline(real)
+        errs = [float(np.linalg.norm(r - o) / np.linalg.norm(o)) for r, o in zip(results, outs)]
+        log(f"cpu baseline: {cpu}; rel err vs oracle {errs}")
+
+    per_norm = {}
+    for t, p in zip(NORMS, plans):
+        per_norm[str(t)] = {"taylor_degree": p["m"], "s": p["s"], "num_gemms": p["num_gemms"]}
+    line = {
+        "metric": "expm/s at n=2048: FP64 plain vs split-BF16 MFMA at <=1e-6 rel err vs FP64",
+        "value": round(expm_per_s, 3),
+        "unit": "expm/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "expm n=2048 real FP64, dense N(0,1) seeds 0-2 rescaled to "
+                               "||A||_1 in {0.5, 8, 100}; one step = one expm per norm",
+                   "n": N, "norms": list(NORMS), "precision": "fp64 plain",
+                   "method": "Taylor/Paterson-Stockmeyer + scaling-and-squaring"},
+        "plans": per_norm,
+        "rel_err_vs_oracle": errs,
+        "breakdown": {"complex128_expm_per_s": round(len(NORMS) / zsec, 3),
+                      "complex128_ms_per_step": round(zsec * 1e3, 4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

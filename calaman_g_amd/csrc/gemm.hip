@@ -65,21 +65,107 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs<T> &g, int row, in
 
 // ------------------------------------------------------------------------------------
 // FP64
+//
+// NT lanes (NT/64 waves) per workgroup; K slices of 16 are double-buffered in
+// LDS, so one barrier per slice separates "everyone has finished reading stage
+// s" from "stage s is refilled". The loop body has no branch: the last slice is
+// peeled, which keeps the accumulators resident in AGPRs across iterations.
 // ------------------------------------------------------------------------------------
-template <int BM, int BN, int WM, int WN, bool kFast>
-__global__ void __launch_bounds__(kThreads)
-    dgemm_mfma(const GemmArgs<double> g, const int tiles_m, const int tiles_n) {
-  constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BN / WN) == 4, "4 waves per workgroup");
-  constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int LDA_S = BM + 16;
-  constexpr int LDB_S = kBK + 2;
-  constexpr int A_ELEMS = BM * kBK / kThreads;
-  constexpr int B_ELEMS = BN * kBK / kThreads;
-  static_assert(A_ELEMS % 2 == 0 && B_ELEMS % 2 == 0, "double2 staging");
+template <int BM, int BN, int NT>
+struct DTile {
+  static constexpr int LDA_S = BM + 16;  // As[k][m]
+  static constexpr int LDB_S = kBK + 2;  // Bs[n][k]
+  static constexpr int A_V2 = BM * kBK / 2 / NT;  // double2 per lane per slice
+  static constexpr int B_V2 = BN * kBK / 2 / NT;
+  static constexpr int A_STAGE = kBK * LDA_S;
+  static constexpr int B_STAGE = BN * LDB_S;
+  static_assert(A_V2 >= 1 && B_V2 >= 1 && (BM * kBK / 2) % NT == 0 && (BN * kBK / 2) % NT == 0,
+                "tile staging must split evenly over the lanes");
+};
 
-  __shared__ double As[kBK * LDA_S];
-  __shared__ double Bs[BN * LDB_S];
+template <int BM, int BN, int NT, bool kFast>
+__device__ __forceinline__ void dgemm_gload(const GemmArgs<double> &g, int m0, int n0, int k0,
+                                            int tid, double2 (&ra)[DTile<BM, BN, NT>::A_V2],
+                                            double2 (&rb)[DTile<BM, BN, NT>::B_V2]) {
+  using TL = DTile<BM, BN, NT>;
+#pragma unroll
+  for (int it = 0; it < TL::A_V2; ++it) {
+    const int idx = tid + it * NT;
+    const int mm = (idx % (BM / 2)) * 2, k = idx / (BM / 2);
+    if constexpr (kFast) {
+      ra[it] = *reinterpret_cast<const double2 *>(g.A + (m0 + mm) + (long)(k0 + k) * g.lda);
+    } else {
+      const int gm = m0 + mm, gk = k0 + k;
+      const bool kin = gk < g.K;
+      ra[it].x = (kin && gm < g.M) ? g.A[gm + (long)gk * g.lda] : 0.0;
+      ra[it].y = (kin && gm + 1 < g.M) ? g.A[gm + 1 + (long)gk * g.lda] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_V2; ++it) {
+    const int idx = tid + it * NT;
+    const int kk = (idx % (kBK / 2)) * 2, nn = idx / (kBK / 2);
+    if constexpr (kFast) {
+      rb[it] = *reinterpret_cast<const double2 *>(g.B + (k0 + kk) + (long)(n0 + nn) * g.ldb);
+    } else {
+      const int gk = k0 + kk, gn = n0 + nn;
+      const bool nin = gn < g.N;
+      rb[it].x = (nin && gk < g.K) ? g.B[gk + (long)gn * g.ldb] : 0.0;
+      rb[it].y = (nin && gk + 1 < g.K) ? g.B[gk + 1 + (long)gn * g.ldb] : 0.0;
+    }
+  }
+}
+
+template <int BM, int BN, int NT>
+__device__ __forceinline__ void dgemm_sstore(double *As, double *Bs, int tid,
+                                             const double2 (&ra)[DTile<BM, BN, NT>::A_V2],
+                                             const double2 (&rb)[DTile<BM, BN, NT>::B_V2]) {
+  using TL = DTile<BM, BN, NT>;
+#pragma unroll
+  for (int it = 0; it < TL::A_V2; ++it) {
+    const int idx = tid + it * NT;
+    const int mm = (idx % (BM / 2)) * 2, k = idx / (BM / 2);
+    *reinterpret_cast<double2 *>(&As[k * TL::LDA_S + mm]) = ra[it];
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_V2; ++it) {
+    const int idx = tid + it * NT;
+    const int kk = (idx % (kBK / 2)) * 2, nn = idx / (kBK / 2);
+    *reinterpret_cast<double2 *>(&Bs[nn * TL::LDB_S + kk]) = rb[it];
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int NT>
+__device__ __forceinline__ void dgemm_compute(const double *As, const double *Bs, int lane, int wm0,
+                                              int wn0, d4 (&acc)[WM / 16][WN / 16]) {
+  using TL = DTile<BM, BN, NT>;
+  constexpr int TM = WM / 16, TN = WN / 16;
+#pragma unroll
+  for (int kk = 0; kk < kBK; kk += 4) {
+    const int kr = kk + (lane >> 4);
+    double a[TM], b[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) a[i] = As[kr * TL::LDA_S + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) b[j] = Bs[(wn0 + j * 16 + (lane & 15)) * TL::LDB_S + kr];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int NT, bool kFast>
+__global__ void __launch_bounds__(NT)
+    dgemm_mfma(const GemmArgs<double> g, const int tiles_m, const int tiles_n) {
+  using TL = DTile<BM, BN, NT>;
+  constexpr int WAVES_M = BM / WM;
+  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
+  constexpr int TM = WM / 16, TN = WN / 16;
+
+  __shared__ double As[2 * TL::A_STAGE];
+  __shared__ double Bs[2 * TL::B_STAGE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -96,110 +182,24 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double ra[A_ELEMS], rb[B_ELEMS];
-
-  auto gload = [&](const int k0) {
-    if constexpr (kFast) {
-      constexpr int ACOLS = kThreads / (BM / 2);
-      const int mm = (tid % (BM / 2)) * 2;
-      const int kq = tid / (BM / 2);
-#pragma unroll
-      for (int it = 0; it < A_ELEMS / 2; ++it) {
-        const double2 v = *reinterpret_cast<const double2 *>(
-            g.A + (m0 + mm) + (long)(k0 + kq + it * ACOLS) * g.lda);
-        ra[2 * it] = v.x;
-        ra[2 * it + 1] = v.y;
-      }
-      const int kk = (tid % (kBK / 2)) * 2;
-      const int nq = tid / (kBK / 2);
-      constexpr int BCOLS = kThreads / (kBK / 2);
-#pragma unroll
-      for (int it = 0; it < B_ELEMS / 2; ++it) {
-        const double2 v = *reinterpret_cast<const double2 *>(
-            g.B + (k0 + kk) + (long)(n0 + nq + it * BCOLS) * g.ldb);
-        rb[2 * it] = v.x;
-        rb[2 * it + 1] = v.y;
-      }
-    } else {
-#pragma unroll
-      for (int it = 0; it < A_ELEMS; ++it) {
-        const int e = tid + it * kThreads;
-        const int mm = e % BM, k = e / BM;
-        const int gm = m0 + mm, gk = k0 + k;
-        ra[it] = (gm < g.M && gk < g.K) ? g.A[gm + (long)gk * g.lda] : 0.0;
-      }
-#pragma unroll
-      for (int it = 0; it < B_ELEMS; ++it) {
-        const int e = tid + it * kThreads;
-        const int k = e % kBK, nn = e / kBK;
-        const int gk = k0 + k, gn = n0 + nn;
-        rb[it] = (gk < g.K && gn < g.N) ? g.B[gk + (long)gn * g.ldb] : 0.0;
-      }
-    }
-  };
-
-  auto sstore = [&]() {
-    if constexpr (kFast) {
-      constexpr int ACOLS = kThreads / (BM / 2);
-      const int mm = (tid % (BM / 2)) * 2;
-      const int kq = tid / (BM / 2);
-#pragma unroll
-      for (int it = 0; it < A_ELEMS / 2; ++it)
-        *reinterpret_cast<double2 *>(&As[(kq + it * ACOLS) * LDA_S + mm]) =
-            double2{ra[2 * it], ra[2 * it + 1]};
-      const int kk = (tid % (kBK / 2)) * 2;
-      const int nq = tid / (kBK / 2);
-      constexpr int BCOLS = kThreads / (kBK / 2);
-#pragma unroll
-      for (int it = 0; it < B_ELEMS / 2; ++it)
-        *reinterpret_cast<double2 *>(&Bs[(nq + it * BCOLS) * LDB_S + kk]) =
-            double2{rb[2 * it], rb[2 * it + 1]};
-    } else {
-#pragma unroll
-      for (int it = 0; it < A_ELEMS; ++it) {
-        const int e = tid + it * kThreads;
-        As[(e / BM) * LDA_S + (e % BM)] = ra[it];
-      }
-#pragma unroll
-      for (int it = 0; it < B_ELEMS; ++it) {
-        const int e = tid + it * kThreads;
-        Bs[(e / kBK) * LDB_S + (e % kBK)] = rb[it];
-      }
-    }
-  };
-
-  auto compute = [&]() {
-#pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      double a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[kr * LDA_S + wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[(wn0 + j * 16 + (lane & 15)) * LDB_S + kr];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-  };
-
+  double2 ra[TL::A_V2], rb[TL::B_V2];
   const int KT = (g.K + kBK - 1) / kBK;
   if (KT > 0) {
-    gload(0);
-    sstore();
-  }
-  __syncthreads();
-  for (int kt = 0; kt < KT; ++kt) {
-    const bool more = kt + 1 < KT;
-    if (more) gload((kt + 1) * kBK);
-    compute();
+    dgemm_gload<BM, BN, NT, kFast>(g, m0, n0, 0, tid, ra, rb);
+    dgemm_sstore<BM, BN, NT>(As, Bs, tid, ra, rb);
     __syncthreads();
-    if (more) {
-      sstore();
+    for (int kt = 0; kt < KT - 1; ++kt) {
+      const int cur = kt & 1;
+      dgemm_gload<BM, BN, NT, kFast>(g, m0, n0, (kt + 1) * kBK, tid, ra, rb);
+      dgemm_compute<BM, BN, WM, WN, NT>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE, lane, wm0,
+                                        wn0, acc);
+      dgemm_sstore<BM, BN, NT>(As + (cur ^ 1) * TL::A_STAGE, Bs + (cur ^ 1) * TL::B_STAGE, tid, ra,
+                               rb);
       __syncthreads();
     }
+    const int last = (KT - 1) & 1;
+    dgemm_compute<BM, BN, WM, WN, NT>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE, lane, wm0,
+                                      wn0, acc);
   }
 
 #pragma unroll
@@ -353,13 +353,13 @@ __global__ void __launch_bounds__(kThreads)
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
-template <int BM, int BN, int WM, int WN, typename Kfast, typename Kgen, typename T>
+template <int BM, int BN, int NT, typename Kfast, typename Kgen, typename T>
 hipError_t dispatch(hipStream_t stream, const GemmArgs<T> &g, bool fast_layout, Kfast kfast,
                     Kgen kgen) {
   const int tiles_m = (g.M + BM - 1) / BM;
   const int tiles_n = (g.N + BN - 1) / BN;
   const bool fast = fast_layout && g.M % BM == 0 && g.N % BN == 0 && g.K % kBK == 0;
-  const dim3 grid(tiles_m * tiles_n), block(kThreads);
+  const dim3 grid(tiles_m * tiles_n), block(NT);
   if (fast) hipLaunchKernelGGL(kfast, grid, block, 0, stream, g, tiles_m, tiles_n);
   else hipLaunchKernelGGL(kgen, grid, block, 0, stream, g, tiles_m, tiles_n);
   return hipGetLastError();
@@ -371,18 +371,18 @@ hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const bool fl = (g.lda % 2 == 0) && (g.ldb % 2 == 0) && aligned16(g.A) && aligned16(g.B);
   if (g.M >= 1024 && g.N >= 1024)
-    return dispatch<128, 128, 64, 64>(stream, g, fl, dgemm_mfma<128, 128, 64, 64, true>,
-                                      dgemm_mfma<128, 128, 64, 64, false>);
-  return dispatch<64, 64, 32, 32>(stream, g, fl, dgemm_mfma<64, 64, 32, 32, true>,
-                                  dgemm_mfma<64, 64, 32, 32, false>);
+    return dispatch<128, 128, 256>(stream, g, fl, dgemm_mfma<128, 128, 64, 64, 256, true>,
+                                      dgemm_mfma<128, 128, 64, 64, 256, false>);
+  return dispatch<64, 64, 256>(stream, g, fl, dgemm_mfma<64, 64, 32, 32, 256, true>,
+                                  dgemm_mfma<64, 64, 32, 32, 256, false>);
 }
 
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.M >= 1024 && g.N >= 1024)
-    return dispatch<128, 64, 64, 32>(stream, g, true, zgemm_mfma<128, 64, 64, 32, true>,
+    return dispatch<128, 64, kThreads>(stream, g, true, zgemm_mfma<128, 64, 64, 32, true>,
                                      zgemm_mfma<128, 64, 64, 32, false>);
-  return dispatch<64, 64, 32, 32>(stream, g, true, zgemm_mfma<64, 64, 32, 32, true>,
+  return dispatch<64, 64, kThreads>(stream, g, true, zgemm_mfma<64, 64, 32, 32, true>,
                                   zgemm_mfma<64, 64, 32, 32, false>);
 }
 

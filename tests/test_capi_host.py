@@ -1,0 +1,149 @@
+"""CPU: the C-ABI library loads, exports every symbol include/calaman_expm.h
+declares, and its host-only logic (plans, thresholds, workspace sizes,
+argument checks that return before any device call) matches the reference's
+rules -- no GPU needed, as in the reference's host-only suites
+(test/expm/expm_tests.cpp:389-465,
+test/paterson_stockmeyer/paterson_stockmeyer_tests.cpp:231-383).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from calaman_g_amd import _lib
+
+L = _lib.lib()
+ST = _lib
+
+
+def test_every_declared_symbol_is_exported():
+    names = _lib.header_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) <= set(_lib._SIGNATURES), "binding table out of step with the header"
+
+
+def test_version_and_status_strings():
+    assert b"gfx950" in L.clm_version()
+    assert L.clm_status_string(ST.INVALID_VALUE) == b"CLM_STATUS_INVALID_VALUE"
+    assert L.clm_status_string(12345) == b"CLM_STATUS_UNKNOWN"
+    assert ST.INVALID_VALUE & 0x7FFF0000 == ST.CLM_DOMAIN_BLAS
+    assert ST.RUNTIME_INVALID_VALUE & 0x7FFF0000 == ST.CLM_DOMAIN_RUNTIME
+
+
+@pytest.mark.parametrize("norm", [0.0, 1e-20, 1e-9, 2e-5, 0.003, 0.05, 0.2, 0.5, 0.64, 1.0,
+                                  1.26, 2.2, 3.3, 4.0, 8.0, 16.0, 100.0, 256.0, 1e4, 1e12])
+def test_expm_plan_matches_oracle(norm):
+    p = L.clm_expm_plan_d(norm)
+    t = oracle.taylor_plan(norm)
+    assert (p.m, p.s, p.num_gemms, p.ps_block) == (t["degree"], t["s"], t["num_gemms"],
+                                                   t["ps_block"])
+    assert p.method == ST.METHOD_TAYLOR_PS and p.norm1 == norm
+    # the reference's own ladder, restated on the host
+    q = L.clm_pade_plan_d(norm)
+    assert (q.m, q.s, q.num_gemms) == oracle.expm_plan(norm)
+
+
+def test_thresholds_match_oracle():
+    for m in list(range(0, 35)):
+        assert L.clm_taylor_theta_d(m) == oracle.taylor_theta(m)
+    for m in range(0, 15):
+        assert L.clm_pade_theta_d(m) == oracle.pade_theta(m)
+
+
+def test_ps_plan_functions_match_oracle():
+    for d in range(0, 40):
+        for s in (0, 1, 2, 3, 5, 50):
+            p = L.clm_paterson_stockmeyer_plan(d, s).as_dict()
+            assert p == oracle.ps_plan(d, s)
+        for s in range(1, d + 2):
+            assert L.clm_paterson_stockmeyer_cost(d, s) == oracle.ps_cost(d, s)
+
+
+def test_ps_buffer_size_is_plan_blocks_aligned():
+    """test/paterson_stockmeyer/paterson_stockmeyer_tests.cpp:297-330"""
+    for n in (1, 2, 5, 7, 16):
+        for degree in (2, 4, 8, 16):
+            plan = oracle.ps_plan(degree)
+            block = ((n * n * 8 + 255) // 256) * 256
+            assert L.clm_paterson_stockmeyer_buffer_size_d(n, degree, 0) == plan["num_blocks"] * block
+            zblock = ((n * n * 16 + 255) // 256) * 256
+            assert L.clm_paterson_stockmeyer_buffer_size_z(n, degree, 0) == plan["num_blocks"] * zblock
+    assert L.clm_paterson_stockmeyer_buffer_size_d(10, 0, 0) == 0
+    assert L.clm_paterson_stockmeyer_buffer_size_d(10, 1, 0) == 0
+    assert L.clm_paterson_stockmeyer_power_stride_d(5) == 32  # 200 B -> 256 B
+
+
+def test_expm_and_horner_buffer_sizes():
+    for n in (1, 7, 64, 2048):
+        blk = ((n * n * 8 + 255) // 256) * 256
+        assert L.clm_horner_buffer_size_d(n) == blk
+        out = ctypes.c_size_t()
+        assert L.clm_expm_buffer_size_d(n, ctypes.byref(out)) == 0
+        colsum = ((n * 8 + 255) // 256) * 256
+        # colsum + As + a 4-power bank + one scratch block: six n x n blocks
+        assert out.value == colsum + 6 * blk
+        assert L.clm_expm_buffer_size_z(n, ctypes.byref(out)) == 0
+        zblk = ((n * n * 16 + 255) // 256) * 256
+        assert out.value == colsum + 6 * zblk
+    assert L.clm_expm_buffer_size_d(0, ctypes.byref(out)) == ST.INVALID_VALUE
+
+
+def _p(x):
+    return ctypes.c_void_p(x)
+
+
+def test_expm_rejects_bad_arguments_before_device():
+    """test/expm/expm_tests.cpp:441-465 with host dummy pointers."""
+    a = np.ones(16)
+    e = np.zeros(16)
+    w = np.zeros(64)
+    pa, pe, pw = (_p(x.ctypes.data) for x in (a, e, w))
+    for f in (L.clm_expm_d, L.clm_expm_z):
+        call = lambda n, lda, lde, work: f(None, None, n, pa, lda, pe, lde, work, 512, None, None)
+        assert call(0, 4, 4, pw) == ST.INVALID_VALUE
+        assert call(4, 3, 4, pw) == ST.INVALID_VALUE
+        assert call(4, 4, 3, pw) == ST.INVALID_VALUE
+        assert call(4, 4, 4, None) == ST.INVALID_VALUE
+        assert call(46341, 46341, 46341, pw) == ST.INVALID_VALUE
+        # 512 bytes is too small for n = 4: ALLOC_FAILED, still before any device call
+        assert call(4, 4, 4, pw) == ST.ALLOC_FAILED
+
+
+def test_ps_horner_reject_with_not_initialized():
+    a = np.ones(16)
+    pa = _p(a.ctypes.data)
+    for f in (L.clm_paterson_stockmeyer_d, L.clm_paterson_stockmeyer_z):
+        assert f(None, 0, pa, 3, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # n < 1
+        assert f(None, 4, pa, -1, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # degree
+        assert f(None, 4, pa, 3, pa, 3, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # lda
+        assert f(None, 4, pa, 3, pa, 4, pa, 3, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # ldp
+        assert f(None, 4, pa, 3, pa, 4, pa, 4, pa, 10 ** 6, -1) == ST.NOT_INITIALIZED  # s
+        assert f(None, 4, None, 3, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # coeffs
+        assert f(None, 4, pa, 3, pa, 4, pa, 4, None, 0, 0) == ST.NOT_INITIALIZED  # workspace
+        assert f(None, 4, pa, 3, pa, 4, pa, 4, pa, 8, 0) == ST.NOT_INITIALIZED  # small workspace
+    for f in (L.clm_horner_d, L.clm_horner_z):
+        assert f(None, 0, pa, 3, pa, 4, pa, 4, pa, 10 ** 6) == ST.NOT_INITIALIZED
+        assert f(None, 4, pa, 3, pa, 4, pa, 4, None, 0) == ST.NOT_INITIALIZED
+        assert f(None, 4, pa, 3, None, 4, pa, 4, pa, 10 ** 6) == ST.NOT_INITIALIZED
+
+
+def test_lascl_lange_gemm_argument_checks():
+    a = np.ones(16)
+    pa = _p(a.ctypes.data)
+    for f in (L.clm_lascl_d, L.clm_lascl_z):
+        assert f(None, 0.0, 1.0, 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
+        assert f(None, float("nan"), 1.0, 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
+        assert f(None, 1.0, float("nan"), 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
+        assert f(None, 1.0, 2.0, 0, 4, pa, 4) == ST.SUCCESS  # empty: nothing enqueued
+        assert f(None, 1.0, 2.0, 4, 4, pa, 3) == ST.RUNTIME_INVALID_VALUE
+    for f in (L.clm_lange_d, L.clm_lange_z):
+        assert f(None, 7, 4, 4, pa, 4, pa, pa) == ST.RUNTIME_INVALID_VALUE
+        assert f(None, 1, 4, 4, pa, 3, pa, pa) == ST.RUNTIME_INVALID_VALUE
+        assert f(None, 1, 4, 4, pa, 4, None, pa) == ST.RUNTIME_INVALID_VALUE
+    assert L.clm_gemm_d(None, 4, 4, 4, 1.0, pa, 3, pa, 4, 0.0, pa, 4) == ST.INVALID_VALUE
+    assert L.clm_gemm_d(None, -1, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4) == ST.INVALID_VALUE
+    assert L.clm_gemm_d(None, 0, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4) == ST.SUCCESS
+    assert L.clm_matrix_norm1_d(None, 0, pa, 4, pa, ctypes.byref(ctypes.c_double())) == ST.INVALID_VALUE

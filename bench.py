@@ -16,6 +16,8 @@ Also reported, on the same JSON line:
                 timed on this host's cores on one expm per norm.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --traffic-probe   (for rocprofv3 --pmc: one 256 MiB device copy
+                                          of known size, then 5 n = 2048 GEMM launches)
 """
 from __future__ import annotations
 
@@ -117,10 +119,32 @@ def gemm_roofline(dev, n=N, reps=20):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    return {"kernel": "dgemm_mfma<128,128,64,64> n=2048", "bound": "mfma",
+    return {"kernel": "dgemm_mfma<128,128,32,32,1024,bk32> n=2048", "bound": "mfma",
             "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
             "mean_launch_ms": round(t * 1e3, 4), "flop_per_launch": 2 * n ** 3}
+
+
+def traffic_probe(dev, n=N, reps=5):
+    """A calibration copy of exactly 256 MiB each way, then the GEMM launches whose
+    HBM bytes rocprofv3's FETCH_SIZE / WRITE_SIZE counters report per dispatch."""
+    import torch
+    from calaman_g_amd.expm import Context, gemm_cm
+
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ctx = Context()
+    a = torch.randn(n, n, dtype=torch.float64, device=dev)
+    b = torch.randn(n, n, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        gemm_cm(ctx, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+    torch.cuda.synchronize()
+    log(f"traffic probe done: copy 268435456 B each way, {reps} GEMM launches n={n}")
 
 
 def cpu_baseline(mats):
@@ -142,6 +166,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the copy + GEMM launches profiled for HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -153,6 +179,9 @@ def main():
 
     dev = torch.device("cuda:0")
     log(f"device: {torch.cuda.get_device_name(0)}")
+    if args.traffic_probe:
+        traffic_probe(dev)
+        return
     real = make_inputs(N, cplx=False)
     sec, plans, results = time_expms(real, dev, args.steps, args.warmup)
     expm_per_s = len(NORMS) / sec

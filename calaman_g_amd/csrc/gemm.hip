@@ -71,23 +71,23 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs<T> &g, int row, in
 // s" from "stage s is refilled". The loop body has no branch: the last slice is
 // peeled, which keeps the accumulators resident in AGPRs across iterations.
 // ------------------------------------------------------------------------------------
-template <int BM, int BN, int NT>
+template <int BM, int BN, int NT, int BK>
 struct DTile {
   static constexpr int LDA_S = BM + 16;  // As[k][m]
-  static constexpr int LDB_S = kBK + 2;  // Bs[n][k]
-  static constexpr int A_V2 = BM * kBK / 2 / NT;  // double2 per lane per slice
-  static constexpr int B_V2 = BN * kBK / 2 / NT;
-  static constexpr int A_STAGE = kBK * LDA_S;
+  static constexpr int LDB_S = BK + 2;   // Bs[n][k]
+  static constexpr int A_V2 = BM * BK / 2 / NT;  // double2 per lane per slice
+  static constexpr int B_V2 = BN * BK / 2 / NT;
+  static constexpr int A_STAGE = BK * LDA_S;
   static constexpr int B_STAGE = BN * LDB_S;
-  static_assert(A_V2 >= 1 && B_V2 >= 1 && (BM * kBK / 2) % NT == 0 && (BN * kBK / 2) % NT == 0,
+  static_assert(A_V2 >= 1 && B_V2 >= 1 && (BM * BK / 2) % NT == 0 && (BN * BK / 2) % NT == 0,
                 "tile staging must split evenly over the lanes");
 };
 
-template <int BM, int BN, int NT, bool kFast>
+template <int BM, int BN, int NT, int BK, bool kFast>
 __device__ __forceinline__ void dgemm_gload(const GemmArgs<double> &g, int m0, int n0, int k0,
-                                            int tid, double2 (&ra)[DTile<BM, BN, NT>::A_V2],
-                                            double2 (&rb)[DTile<BM, BN, NT>::B_V2]) {
-  using TL = DTile<BM, BN, NT>;
+                                            int tid, double2 (&ra)[DTile<BM, BN, NT, BK>::A_V2],
+                                            double2 (&rb)[DTile<BM, BN, NT, BK>::B_V2]) {
+  using TL = DTile<BM, BN, NT, BK>;
 #pragma unroll
   for (int it = 0; it < TL::A_V2; ++it) {
     const int idx = tid + it * NT;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void dgemm_gload(const GemmArgs<double> &g, int m0, i
 #pragma unroll
   for (int it = 0; it < TL::B_V2; ++it) {
     const int idx = tid + it * NT;
-    const int kk = (idx % (kBK / 2)) * 2, nn = idx / (kBK / 2);
+    const int kk = (idx % (BK / 2)) * 2, nn = idx / (BK / 2);
     if constexpr (kFast) {
       rb[it] = *reinterpret_cast<const double2 *>(g.B + (k0 + kk) + (long)(n0 + nn) * g.ldb);
     } else {
@@ -116,11 +116,11 @@ __device__ __forceinline__ void dgemm_gload(const GemmArgs<double> &g, int m0, i
   }
 }
 
-template <int BM, int BN, int NT>
+template <int BM, int BN, int NT, int BK>
 __device__ __forceinline__ void dgemm_sstore(double *As, double *Bs, int tid,
-                                             const double2 (&ra)[DTile<BM, BN, NT>::A_V2],
-                                             const double2 (&rb)[DTile<BM, BN, NT>::B_V2]) {
-  using TL = DTile<BM, BN, NT>;
+                                             const double2 (&ra)[DTile<BM, BN, NT, BK>::A_V2],
+                                             const double2 (&rb)[DTile<BM, BN, NT, BK>::B_V2]) {
+  using TL = DTile<BM, BN, NT, BK>;
 #pragma unroll
   for (int it = 0; it < TL::A_V2; ++it) {
     const int idx = tid + it * NT;
@@ -130,18 +130,18 @@ __device__ __forceinline__ void dgemm_sstore(double *As, double *Bs, int tid,
 #pragma unroll
   for (int it = 0; it < TL::B_V2; ++it) {
     const int idx = tid + it * NT;
-    const int kk = (idx % (kBK / 2)) * 2, nn = idx / (kBK / 2);
+    const int kk = (idx % (BK / 2)) * 2, nn = idx / (BK / 2);
     *reinterpret_cast<double2 *>(&Bs[nn * TL::LDB_S + kk]) = rb[it];
   }
 }
 
-template <int BM, int BN, int WM, int WN, int NT>
+template <int BM, int BN, int WM, int WN, int NT, int BK>
 __device__ __forceinline__ void dgemm_compute(const double *As, const double *Bs, int lane, int wm0,
                                               int wn0, d4 (&acc)[WM / 16][WN / 16]) {
-  using TL = DTile<BM, BN, NT>;
+  using TL = DTile<BM, BN, NT, BK>;
   constexpr int TM = WM / 16, TN = WN / 16;
 #pragma unroll
-  for (int kk = 0; kk < kBK; kk += 4) {
+  for (int kk = 0; kk < BK; kk += 4) {
     const int kr = kk + (lane >> 4);
     double a[TM], b[TN];
 #pragma unroll
@@ -156,10 +156,10 @@ __device__ __forceinline__ void dgemm_compute(const double *As, const double *Bs
   }
 }
 
-template <int BM, int BN, int WM, int WN, int NT, bool kFast>
+template <int BM, int BN, int WM, int WN, int NT, bool kFast, int BK = kBK>
 __global__ void __launch_bounds__(NT)
     dgemm_mfma(const GemmArgs<double> g, const int tiles_m, const int tiles_n) {
-  using TL = DTile<BM, BN, NT>;
+  using TL = DTile<BM, BN, NT, BK>;
   constexpr int WAVES_M = BM / WM;
   static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -183,23 +183,23 @@ __global__ void __launch_bounds__(NT)
     for (int j = 0; j < TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
   double2 ra[TL::A_V2], rb[TL::B_V2];
-  const int KT = (g.K + kBK - 1) / kBK;
+  const int KT = (g.K + BK - 1) / BK;
   if (KT > 0) {
-    dgemm_gload<BM, BN, NT, kFast>(g, m0, n0, 0, tid, ra, rb);
-    dgemm_sstore<BM, BN, NT>(As, Bs, tid, ra, rb);
+    dgemm_gload<BM, BN, NT, BK, kFast>(g, m0, n0, 0, tid, ra, rb);
+    dgemm_sstore<BM, BN, NT, BK>(As, Bs, tid, ra, rb);
     __syncthreads();
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
-      dgemm_gload<BM, BN, NT, kFast>(g, m0, n0, (kt + 1) * kBK, tid, ra, rb);
-      dgemm_compute<BM, BN, WM, WN, NT>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE, lane, wm0,
-                                        wn0, acc);
-      dgemm_sstore<BM, BN, NT>(As + (cur ^ 1) * TL::A_STAGE, Bs + (cur ^ 1) * TL::B_STAGE, tid, ra,
-                               rb);
+      dgemm_gload<BM, BN, NT, BK, kFast>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
+      dgemm_compute<BM, BN, WM, WN, NT, BK>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE, lane,
+                                            wm0, wn0, acc);
+      dgemm_sstore<BM, BN, NT, BK>(As + (cur ^ 1) * TL::A_STAGE, Bs + (cur ^ 1) * TL::B_STAGE, tid,
+                                   ra, rb);
       __syncthreads();
     }
     const int last = (KT - 1) & 1;
-    dgemm_compute<BM, BN, WM, WN, NT>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE, lane, wm0,
-                                      wn0, acc);
+    dgemm_compute<BM, BN, WM, WN, NT, BK>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE, lane,
+                                          wm0, wn0, acc);
   }
 
 #pragma unroll
@@ -353,12 +353,12 @@ __global__ void __launch_bounds__(kThreads)
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
-template <int BM, int BN, int NT, typename Kfast, typename Kgen, typename T>
+template <int BM, int BN, int NT, int BK, typename Kfast, typename Kgen, typename T>
 hipError_t dispatch(hipStream_t stream, const GemmArgs<T> &g, bool fast_layout, Kfast kfast,
                     Kgen kgen) {
   const int tiles_m = (g.M + BM - 1) / BM;
   const int tiles_n = (g.N + BN - 1) / BN;
-  const bool fast = fast_layout && g.M % BM == 0 && g.N % BN == 0 && g.K % kBK == 0;
+  const bool fast = fast_layout && g.M % BM == 0 && g.N % BN == 0 && g.K % BK == 0;
   const dim3 grid(tiles_m * tiles_n), block(NT);
   if (fast) hipLaunchKernelGGL(kfast, grid, block, 0, stream, g, tiles_m, tiles_n);
   else hipLaunchKernelGGL(kgen, grid, block, 0, stream, g, tiles_m, tiles_n);
@@ -370,19 +370,24 @@ hipError_t dispatch(hipStream_t stream, const GemmArgs<T> &g, bool fast_layout, 
 hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const bool fl = (g.lda % 2 == 0) && (g.ldb % 2 == 0) && aligned16(g.A) && aligned16(g.B);
-  if (g.M >= 1024 && g.N >= 1024)
-    return dispatch<128, 128, 256>(stream, g, fl, dgemm_mfma<128, 128, 64, 64, 256, true>,
-                                      dgemm_mfma<128, 128, 64, 64, 256, false>);
-  return dispatch<64, 64, 256>(stream, g, fl, dgemm_mfma<64, 64, 32, 32, 256, true>,
-                                  dgemm_mfma<64, 64, 32, 32, 256, false>);
+  // 128 x 128 tiles once they fill the 256 CUs; 64 x 64 below that
+  const long big_tiles = static_cast<long>((g.M + 127) / 128) * ((g.N + 127) / 128);
+  // 16 waves of 32 x 32 with K slices of 32: the fastest of the swept shapes at
+  // n = 2048 and 4096 (4 waves per SIMD hide the per-slice barrier)
+  if (big_tiles >= 256)
+    return dispatch<128, 128, 1024, 32>(stream, g, fl,
+                                        dgemm_mfma<128, 128, 32, 32, 1024, true, 32>,
+                                        dgemm_mfma<128, 128, 32, 32, 1024, false, 32>);
+  return dispatch<64, 64, 256, kBK>(stream, g, fl, dgemm_mfma<64, 64, 32, 32, 256, true>,
+                                    dgemm_mfma<64, 64, 32, 32, 256, false>);
 }
 
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.M >= 1024 && g.N >= 1024)
-    return dispatch<128, 64, kThreads>(stream, g, true, zgemm_mfma<128, 64, 64, 32, true>,
+    return dispatch<128, 64, kThreads, kBK>(stream, g, true, zgemm_mfma<128, 64, 64, 32, true>,
                                      zgemm_mfma<128, 64, 64, 32, false>);
-  return dispatch<64, 64, kThreads>(stream, g, true, zgemm_mfma<64, 64, 32, 32, true>,
+  return dispatch<64, 64, kThreads, kBK>(stream, g, true, zgemm_mfma<64, 64, 32, 32, true>,
                                   zgemm_mfma<64, 64, 32, 32, false>);
 }
 

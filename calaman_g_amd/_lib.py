@@ -37,7 +37,19 @@ NORM_MAX_ABS, NORM_ONE, NORM_INF, NORM_FROBENIUS = 0, 1, 2, 3
 class ExpmPlan(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
                 ("method", ctypes.c_int), ("ps_block", ctypes.c_int), ("dtype", ctypes.c_int),
+                ("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int),
                 ("norm1", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+SPLIT_BF16X3 = 3
+SPLIT_BF16X6 = 6
+
+
+class SplitOpts(ctypes.Structure):
+    _fields_ = [("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -81,6 +93,11 @@ _SIGNATURES = {
     "clm_taylor_theta_d": (_d, [_i]),
     "clm_pade_theta_d": (_d, [_i]),
     "clm_pade_plan_d": (ExpmPlan, [_d]),
+    "clm_expm_split_plan_d": (ExpmPlan, [_d, ctypes.POINTER(SplitOpts)]),
+    "clm_taylor_theta_s": (_d, [_i]),
+    "clm_expm_split_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
+                              ctypes.POINTER(SplitOpts), ctypes.POINTER(ExpmPlan)]),
+    "clm_gemm_split_d": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
     "clm_expm_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_buffer_size_z": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "clm_internal.hpp"
 
@@ -85,6 +86,17 @@ T one() {
   return ops<T>::from_real(1.0);
 }
 
+// prec: 0 = the FP64 / complex128 MFMA product; 3 or 6 = the BF16 split product
+// (real FP64 only)
+template <typename T>
+hipError_t launch_prec(hipStream_t st, const GemmArgs<T> &g, int prec) {
+  if constexpr (std::is_same_v<T, double>) {
+    if (prec == CLM_SPLIT_BF16X3 || prec == CLM_SPLIT_BF16X6)
+      return launch_gemm_split_bf16(st, g, prec);
+  }
+  return launch_gemm(st, g);
+}
+
 // ---- Paterson-Stockmeyer plan (src/paterson_stockmeyer/interface.cppm:117-210) ----
 int ps_cost(int degree, int s) {
   if (s < 1 || degree < 1) return 0;
@@ -151,6 +163,12 @@ constexpr double kTaylorTheta[kNumTaylor] = {
     4.991228871115323e-2,  2.142358068451711e-1, 6.410835233041199e-1, 1.260381060642639,
     2.21904886936509,      3.310172839890271};
 
+// the same thresholds at u = 2^-24 (the precision variant's accuracy target)
+constexpr double kTaylorThetaS[kNumTaylor] = {
+    1.192092800768788e-7, 5.978858893805233e-4, 1.123386473528671e-2, 1.30848716459947e-1,
+    5.800524627688768e-1, 1.223479542424143,    2.21704439497472,     3.279561212635997,
+    4.649782224100757,    6.041358758192571};
+
 // the largest power bank any ladder degree needs
 int max_bank_blocks() {
   int mx = 0;
@@ -161,12 +179,12 @@ int max_bank_blocks() {
   return mx;
 }
 
-clm_expm_plan taylor_plan(double norm1) {
+clm_expm_plan taylor_plan(double norm1, const double *theta = kTaylorTheta) {
   int best = -1, best_cost = 0, best_s = 0;
   for (int i = 0; i < kNumTaylor; ++i) {
     int s = 0;
-    if (norm1 > kTaylorTheta[i]) {
-      s = static_cast<int>(std::ceil(std::log2(norm1 / kTaylorTheta[i])));
+    if (norm1 > theta[i]) {
+      s = static_cast<int>(std::ceil(std::log2(norm1 / theta[i])));
       if (s < 0) s = 0;
     }
     const int cost = ps_plan(kTaylorDegrees[i], 0).num_gemms + s;
@@ -184,6 +202,57 @@ clm_expm_plan taylor_plan(double norm1) {
   p.ps_block = ps_plan(p.m, 0).s;
   p.dtype = 0;
   p.norm1 = norm1;
+  return p;
+}
+
+// The precision variant's schedule. Degree from the 2^-24 thresholds; polynomial
+// products split 3 ways while s <= 3 (their error is amplified 2^s by the
+// squarings), 6 ways beyond; only the last squaring(s) split, 6 ways -- an error
+// made in squaring k of s is amplified 2^(s-k), so the early ones stay FP64. The
+// degree minimises the products weighted by their cost relative to an FP64
+// product (3-way split 0.2, 6-way 0.35), so cheap BF16 polynomial products are
+// traded for expensive FP64 squarings.
+struct SplitChoice {
+  int ps_split, split_squarings;
+};
+SplitChoice split_schedule(int s, const clm_split_opts *opts) {
+  SplitChoice c{(s <= 3) ? CLM_SPLIT_BF16X3 : CLM_SPLIT_BF16X6, (s >= 1) ? 1 : 0};
+  if (opts && (opts->ps_split == CLM_SPLIT_BF16X3 || opts->ps_split == CLM_SPLIT_BF16X6))
+    c.ps_split = opts->ps_split;
+  if (opts && opts->split_squarings >= 0)
+    c.split_squarings = opts->split_squarings < s ? opts->split_squarings : s;
+  return c;
+}
+
+clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
+  int best = -1, best_s = 0;
+  double best_cost = 0.0;
+  for (int i = 0; i < kNumTaylor; ++i) {
+    int s = 0;
+    if (norm1 > kTaylorThetaS[i]) {
+      s = static_cast<int>(std::ceil(std::log2(norm1 / kTaylorThetaS[i])));
+      if (s < 0) s = 0;
+    }
+    const SplitChoice c = split_schedule(s, opts);
+    const double w_ps = (c.ps_split == CLM_SPLIT_BF16X3) ? 0.2 : 0.35;
+    const double cost = ps_plan(kTaylorDegrees[i], 0).num_gemms * w_ps +
+                        (s - c.split_squarings) * 1.0 + c.split_squarings * 0.35;
+    if (best < 0 || cost < best_cost - 1e-12 || (std::fabs(cost - best_cost) <= 1e-12 && s < best_s)) {
+      best = i;
+      best_cost = cost;
+      best_s = s;
+    }
+  }
+  clm_expm_plan p{};
+  p.m = kTaylorDegrees[best];
+  p.s = best_s;
+  p.num_gemms = ps_plan(p.m, 0).num_gemms + best_s;
+  p.method = CLM_METHOD_TAYLOR_PS;
+  p.ps_block = ps_plan(p.m, 0).s;
+  p.norm1 = norm1;
+  const SplitChoice c = split_schedule(best_s, opts);
+  p.ps_split = c.ps_split;
+  p.split_squarings = c.split_squarings;
   return p;
 }
 
@@ -256,7 +325,7 @@ size_t expm_buffer_size(int n) {
 // ---- device building blocks ------------------------------------------------------------------
 template <typename T>
 clm_status_t gemm(hipStream_t st, int n, const T *A, long lda, const T *B, long ldb, T *C,
-                  long ldc) {
+                  long ldc, int prec = 0) {
   GemmArgs<T> g;
   g.M = g.N = g.K = n;
   g.A = A;
@@ -266,7 +335,7 @@ clm_status_t gemm(hipStream_t st, int n, const T *A, long lda, const T *B, long 
   g.C = C;
   g.ldc = ldc;
   g.alpha = one<T>();
-  CLM_TRY_HIP(launch_gemm(st, g));
+  CLM_TRY_HIP(launch_prec(st, g, prec));
   return CLM_STATUS_SUCCESS;
 }
 
@@ -276,12 +345,12 @@ clm_status_t gemm(hipStream_t st, int n, const T *A, long lda, const T *B, long 
 template <typename T>
 clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const T *A, long lda,
                          T *P, long ldp, T *bank, long stride, T *scratch, long ld_scratch,
-                         int s_requested) {
+                         int s_requested, int prec = 0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
   for (int k = 2; k <= p.num_powers + 1; ++k) {
     const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
     const long ld_left = (k == 2) ? lda : n;
-    CLM_TRY(gemm(st, n, left, ld_left, A, lda, bank + static_cast<long>(k - 2) * stride, n));
+    CLM_TRY(gemm(st, n, left, ld_left, A, lda, bank + static_cast<long>(k - 2) * stride, n, prec));
   }
   const T *X = (p.r < 1) ? nullptr : (p.s >= 2) ? bank + static_cast<long>(p.s - 2) * stride : A;
   const long ldx = (p.s >= 2) ? n : lda;
@@ -312,7 +381,7 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
     g.bank = bank;
     g.bank_stride = stride;
     g.ldbank = n;
-    CLM_TRY_HIP(launch_gemm(st, g));
+    CLM_TRY_HIP(launch_prec(st, g, prec));
     cur = next;
   }
   return CLM_STATUS_SUCCESS;
@@ -331,7 +400,8 @@ clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, double *col
 
 template <typename T>
 clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda, T *dE, int lde,
-                       void *work, size_t bytes, int *dInfo, clm_expm_plan *plan) {
+                       void *work, size_t bytes, int *dInfo, clm_expm_plan *plan,
+                       const clm_split_opts *split = nullptr, bool use_split = false) {
   if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
       dE == nullptr)
     return CLM_STATUS_INVALID_VALUE;
@@ -348,7 +418,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
 
-  clm_expm_plan p = taylor_plan(norm);
+  clm_expm_plan p = use_split ? split_plan(norm, split) : taylor_plan(norm);
   p.dtype = dtype_code<T>();
   if (plan) *plan = p;
 
@@ -368,12 +438,13 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   const long ld_other = (p.s % 2 == 0) ? n : lde;
 
   CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                         static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+                         static_cast<long>(ws.stride), other, ld_other, p.ps_block, p.ps_split));
 
   T *cur = target, *nxt = other;
   long ldc = ld_target, ldn = ld_other;
   for (int k = 0; k < p.s; ++k) {
-    CLM_TRY(gemm(st, n, cur, ldc, cur, ldc, nxt, ldn));
+    const int prec = (k >= p.s - p.split_squarings) ? CLM_SPLIT_BF16X6 : 0;
+    CLM_TRY(gemm(st, n, cur, ldc, cur, ldc, nxt, ldn, prec));
     std::swap(cur, nxt);
     std::swap(ldc, ldn);
   }
@@ -448,7 +519,7 @@ clm_status_t horner_impl(clm_ctx *ctx, int n, const T *dCoeffs, int degree, cons
 
 template <typename T>
 clm_status_t gemm_impl(clm_ctx *ctx, int m, int n, int k, T alpha, const T *dA, int lda,
-                       const T *dB, int ldb, T beta, T *dC, int ldc) {
+                       const T *dB, int ldb, T beta, T *dC, int ldc, int prec = 0) {
   if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) ||
       ldc < (m > 1 ? m : 1))
     return CLM_STATUS_INVALID_VALUE;
@@ -472,7 +543,7 @@ clm_status_t gemm_impl(clm_ctx *ctx, int m, int n, int k, T alpha, const T *dA, 
     g.Cin = dC;
     g.ldcin = ldc;
   }
-  CLM_TRY_HIP(launch_gemm(c->stream, g));
+  CLM_TRY_HIP(launch_prec(c->stream, g, prec));
   return CLM_STATUS_SUCCESS;
 }
 
@@ -601,6 +672,27 @@ clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream) {
 
 clm_expm_plan clm_expm_plan_d(double norm1) { return clm::taylor_plan(norm1); }
 
+clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts) {
+  return clm::split_plan(norm1, opts);
+}
+
+double clm_taylor_theta_s(int m) {
+  for (int i = 0; i < clm::kNumTaylor; ++i)
+    if (clm::kTaylorDegrees[i] == m) return clm::kTaylorThetaS[i];
+  return 0.0;
+}
+
+clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                              double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                              const clm_split_opts *opts, clm_expm_plan *plan) {
+  if (opts && ((opts->ps_split != 0 && opts->ps_split != CLM_SPLIT_BF16X3 &&
+                opts->ps_split != CLM_SPLIT_BF16X6) ||
+               opts->split_squarings < -1))
+    return CLM_STATUS_INVALID_VALUE;
+  return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
+                                lwork_bytes, dInfo, plan, opts, true);
+}
+
 double clm_taylor_theta_d(int m) {
   for (int i = 0; i < clm::kNumTaylor; ++i)
     if (clm::kTaylorDegrees[i] == m) return clm::kTaylorTheta[i];
@@ -719,6 +811,13 @@ clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs
 clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
                         const double *dB, int ldb, double beta, double *dC, int ldc) {
   return clm::gemm_impl<double>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+}
+
+clm_status_t clm_gemm_split_d(clm_ctx *ctx, int split, int m, int n, int k, double alpha,
+                              const double *dA, int lda, const double *dB, int ldb, double beta,
+                              double *dC, int ldc) {
+  if (split != CLM_SPLIT_BF16X3 && split != CLM_SPLIT_BF16X6) return CLM_STATUS_INVALID_VALUE;
+  return clm::gemm_impl<double>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, split);
 }
 
 clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex alpha,

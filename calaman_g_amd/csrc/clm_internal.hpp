@@ -104,6 +104,9 @@ struct GemmArgs {
 };
 
 hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g);
+// the same product on BF16 MFMA with operands split into 2 (split = 3 products)
+// or 3 (split = 6 products) bfloat16 terms, FP32 accumulation, FP64 in and out
+hipError_t launch_gemm_split_bf16(hipStream_t stream, const GemmArgs<double> &g, int split);
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g);
 template <typename T>
 hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);

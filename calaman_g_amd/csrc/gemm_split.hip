@@ -1,0 +1,283 @@
+// gemm_split.hip -- FP64-in / FP64-out matrix products on BF16 MFMA with split
+// operands and FP32 accumulation (the precision variant of the expm path).
+//
+// Each FP64 element x is rounded once to FP32, f = fl32(x), and split into
+// bfloat16 terms f = h + m (+ l) + tiny: h = bf16(f), m = bf16(f - h),
+// l = bf16(f - h - m). Each difference is exact in FP32 (h and m are f and
+// f - h rounded to fewer bits), so two terms carry ~16 and three ~24 bits of f.
+// The product keeps the cross terms of total order below the split:
+//   SPLIT = 3:  Ah*Bh + Ah*Bm + Am*Bh                      (3 bf16 products)
+//   SPLIT = 6:  the above + Ah*Bl + Am*Bm + Al*Bh           (6 bf16 products)
+// all accumulated by v_mfma_f32_16x16x32_bf16 in FP32. bf16 * bf16 is exact in
+// FP32, so the error is the FP32 accumulation plus the dropped terms: about
+// 2^-16 per product for SPLIT = 3 and the FP32 level (~2^-24 sqrt(k)) for 6.
+//
+// Operands are read from HBM as FP64 and split in registers on the way into
+// LDS, so no split planes ever travel through HBM. The epilogue is the FP64
+// path's (alpha, beta*Cin, fused Paterson-Stockmeyer block), in FP64.
+//
+// Layout: a 128 x 128 tile of C per workgroup of 4 wave64, each wave 64 x 64 as
+// 4 x 4 MFMA tiles of 16 x 16; K advances in slices of 32 (one MFMA k-step).
+// LDS holds every plane k-contiguous -- As[m][k], Bs[n][k], rows of 40 bf16 (80 B)
+// -- because the MFMA wants 8 consecutive k of one row in a lane; the 80-byte row
+// makes the 16 lanes of a quarter-wave hit 16 disjoint 4-bank groups. A arrives
+// m-contiguous from column-major memory and is transposed on the LDS write.
+//
+// MFMA operand layout (16x16x32 bf16): A lane l holds A[l%16][8(l/16) .. 8(l/16)+7],
+// B lane l holds B[8(l/16) .. +7][l%16], D lane l holds rows CLM_BF16_D_ROW(l, r).
+#include "clm_internal.hpp"
+
+namespace clm {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSplitBK = 32;
+constexpr int kSplitRow = 40;  // bf16 per LDS row (32 + 8 pad)
+
+// row of D element r held by lane l (16x16 f32 output)
+__device__ __forceinline__ int d_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+__device__ __forceinline__ unsigned short bf16_bits_rne(float f) {
+  const unsigned u = __float_as_uint(f);
+  const unsigned rounded = u + 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(rounded >> 16);
+}
+__device__ __forceinline__ float bf16_to_float(unsigned short b) {
+  return __uint_as_float(static_cast<unsigned>(b) << 16);
+}
+
+// split x into NPL bf16 planes (bit patterns)
+template <int NPL>
+__device__ __forceinline__ void split_bf16(double x, unsigned short (&p)[NPL]) {
+  float r = static_cast<float>(x);
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    p[t] = bf16_bits_rne(r);
+    r = r - bf16_to_float(p[t]);
+  }
+}
+
+__device__ __forceinline__ void tile_coords_s(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
+  const int total = tiles_m * tiles_n;
+  int t = bid;
+  if ((total & 7) == 0) t = (bid & 7) * (total >> 3) + (bid >> 3);
+  constexpr int G = 4;
+  const int per_group = G * tiles_n;
+  const int group = t / per_group;
+  const int first_m = group * G;
+  const int gsize = min(tiles_m - first_m, G);
+  const int r = t - group * per_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+__device__ __forceinline__ void epilogue_store_s(const GemmArgs<double> &g, int row, int col,
+                                                 double acc) {
+  double v = g.alpha * acc;
+  if (g.Cin) v = v + g.beta * g.Cin[row + (long)col * g.ldcin];
+  if (g.nterms > 0) {
+    double e = (row == col) ? g.coef[0] : 0.0;
+    if (g.nterms > 1) e = g.coef[1] * g.X1[row + (long)col * g.ldx1] + e;
+    for (int t = 2; t < g.nterms; ++t)
+      e = g.coef[t] * g.bank[(long)(t - 2) * g.bank_stride + row + (long)col * g.ldbank] + e;
+    v = v + e;
+  }
+  g.C[row + (long)col * g.ldc] = v;
+}
+
+// planes needed per operand
+template <int SPLIT>
+struct SplitPlanes {
+  static constexpr int NPL = (SPLIT == 3) ? 2 : 3;
+};
+
+template <int SPLIT, bool kFast>
+__global__ void __launch_bounds__(256)
+    dgemm_split_bf16(const GemmArgs<double> g, const int tiles_m, const int tiles_n) {
+  constexpr int BM = 128, BN = 128, WM = 64, WN = 64, NT = 256;
+  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int PLANE_A = BM * kSplitRow;  // bf16 per A plane
+  constexpr int PLANE_B = BN * kSplitRow;
+  // A: each lane stages 2 rows (m, m+1) x 8 k; B: 8 consecutive k of one column
+  constexpr int A_ITERS = (BM * kSplitBK) / (NT * 16);  // = 1
+  constexpr int B_ITERS = (BN * kSplitBK) / (NT * 8);   // = 2
+  static_assert(A_ITERS == 1 && B_ITERS == 2, "staging shape");
+
+  __shared__ __attribute__((aligned(16))) unsigned short As[NPL * PLANE_A];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[NPL * PLANE_B];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords_s(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+
+  f4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // staging coordinates
+  const int a_m = (tid % (BM / 2)) * 2;       // rows a_m, a_m + 1
+  const int a_k = (tid / (BM / 2)) * 8;       // k a_k .. a_k + 7
+  const int b_k = (tid % (kSplitBK / 8)) * 8;  // k b_k .. b_k + 7
+  const int b_n = tid / (kSplitBK / 8);        // columns b_n, b_n + 64
+
+  double ra[2][8];
+  double rb[2][8];
+
+  // global -> registers for the slice at k0
+  auto gload = [&](const int k0) {
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int gk = k0 + a_k + kk;
+      if (kFast) {
+        const double2 v = *reinterpret_cast<const double2 *>(g.A + (m0 + a_m) + (long)gk * g.lda);
+        ra[0][kk] = v.x;
+        ra[1][kk] = v.y;
+      } else {
+        const int gm = m0 + a_m;
+        const bool kin = gk < g.K;
+        ra[0][kk] = (kin && gm < g.M) ? g.A[gm + (long)gk * g.lda] : 0.0;
+        ra[1][kk] = (kin && gm + 1 < g.M) ? g.A[gm + 1 + (long)gk * g.lda] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int gn = n0 + b_n + it * 64;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk += 2) {
+        const int gk = k0 + b_k + kk;
+        if (kFast) {
+          const double2 v = *reinterpret_cast<const double2 *>(g.B + gk + (long)gn * g.ldb);
+          rb[it][kk] = v.x;
+          rb[it][kk + 1] = v.y;
+        } else {
+          const bool nin = gn < g.N;
+          rb[it][kk] = (nin && gk < g.K) ? g.B[gk + (long)gn * g.ldb] : 0.0;
+          rb[it][kk + 1] = (nin && gk + 1 < g.K) ? g.B[gk + 1 + (long)gn * g.ldb] : 0.0;
+        }
+      }
+    }
+  };
+
+  const int KT = (g.K + kSplitBK - 1) / kSplitBK;
+  if (KT > 0) gload(0);
+  for (int kt = 0; kt < KT; ++kt) {
+    __syncthreads();  // everyone is done reading the previous slice
+    // ---- split and store: 8 consecutive k of one row = one 16-byte LDS write per plane ----
+#pragma unroll
+    for (int r2 = 0; r2 < 2; ++r2) {
+      unsigned short pl[NPL][8];
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        unsigned short p[NPL];
+        split_bf16<NPL>(ra[r2][kk], p);
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) pl[t][kk] = p[t];
+      }
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        uint4 w;
+        w.x = pl[t][0] | (static_cast<unsigned>(pl[t][1]) << 16);
+        w.y = pl[t][2] | (static_cast<unsigned>(pl[t][3]) << 16);
+        w.z = pl[t][4] | (static_cast<unsigned>(pl[t][5]) << 16);
+        w.w = pl[t][6] | (static_cast<unsigned>(pl[t][7]) << 16);
+        *reinterpret_cast<uint4 *>(&As[t * PLANE_A + (a_m + r2) * kSplitRow + a_k]) = w;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      unsigned short pl[NPL][8];
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        unsigned short p[NPL];
+        split_bf16<NPL>(rb[it][kk], p);
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) pl[t][kk] = p[t];
+      }
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        uint4 w;
+        w.x = pl[t][0] | (static_cast<unsigned>(pl[t][1]) << 16);
+        w.y = pl[t][2] | (static_cast<unsigned>(pl[t][3]) << 16);
+        w.z = pl[t][4] | (static_cast<unsigned>(pl[t][5]) << 16);
+        w.w = pl[t][6] | (static_cast<unsigned>(pl[t][7]) << 16);
+        *reinterpret_cast<uint4 *>(&Bs[t * PLANE_B + (b_n + it * 64) * kSplitRow + b_k]) = w;
+      }
+    }
+    __syncthreads();
+    // the next slice's loads fly while this slice is multiplied (the last
+    // iteration re-reads its own slice instead of branching)
+    gload((kt + 1 < KT ? kt + 1 : kt) * kSplitBK);
+    // ---- MFMA: one k-step of 32 ----
+    const int koff = 8 * (lane >> 4);
+    bf16x8 a[NPL][TM], b[NPL][TN];
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        a[t][i] = *reinterpret_cast<const bf16x8 *>(
+            &As[t * PLANE_A + (wm0 + i * 16 + (lane & 15)) * kSplitRow + koff]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        b[t][j] = *reinterpret_cast<const bf16x8 *>(
+            &Bs[t * PLANE_B + (wn0 + j * 16 + (lane & 15)) * kSplitRow + koff]);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        // smallest terms first, so the FP32 accumulator adds them before the big one
+        if constexpr (SPLIT == 6) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+        }
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+      }
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + d_row(lane, r);
+        const int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (kFast || (row < g.M && col < g.N))
+          epilogue_store_s(g, row, col, static_cast<double>(acc[i][j][r]));
+      }
+}
+
+inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+template <int SPLIT>
+hipError_t launch_split(hipStream_t stream, const GemmArgs<double> &g) {
+  const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
+  const bool fast = g.M % 128 == 0 && g.N % 128 == 0 && g.K % kSplitBK == 0 && g.lda % 2 == 0 &&
+                    g.ldb % 2 == 0 && aligned16s(g.A) && aligned16s(g.B);
+  const dim3 grid(tiles_m * tiles_n), block(256);
+  if (fast) hipLaunchKernelGGL((dgemm_split_bf16<SPLIT, true>), grid, block, 0, stream, g, tiles_m, tiles_n);
+  else hipLaunchKernelGGL((dgemm_split_bf16<SPLIT, false>), grid, block, 0, stream, g, tiles_m, tiles_n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_gemm_split_bf16(hipStream_t stream, const GemmArgs<double> &g, int split) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  return split == 6 ? launch_split<6>(stream, g) : launch_split<3>(stream, g);
+}
+
+}  // namespace clm

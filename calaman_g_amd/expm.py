@@ -173,6 +173,25 @@ def expm_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int, dWor
     return int(st), plan.as_dict()
 
 
+def expm_split_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
+                  dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None, ps_split: int = 0,
+                  split_squarings: int = -1, stream: Optional[torch.cuda.Stream] = None,
+                  ctx: Optional[Context] = None) -> tuple:
+    """clm_expm_split_d: the BF16-split precision variant; returns (status, plan dict)."""
+    plan = ExpmPlan()
+    opts = _lib.SplitOpts(int(ps_split), int(split_squarings))
+    st = lib().clm_expm_split_d(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)),
+                                int(n), _ptr(dA), int(lda), _ptr(dE), int(lde), _ptr(dWork),
+                                dWork.numel() * dWork.element_size(), _ptr(dInfo),
+                                ctypes.byref(opts), ctypes.byref(plan))
+    return int(st), plan.as_dict()
+
+
+def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1) -> dict:
+    opts = _lib.SplitOpts(int(ps_split), int(split_squarings))
+    return lib().clm_expm_split_plan_d(float(norm1), ctypes.byref(opts)).as_dict()
+
+
 def matrix_norm1_cm(n: int, dA: torch.Tensor, lda: int, dColsum: torch.Tensor,
                     stream: Optional[torch.cuda.Stream] = None) -> float:
     out = ctypes.c_double(0.0)
@@ -214,7 +233,10 @@ def horner_cm(ctx: Context, n: int, dCoeffs: torch.Tensor, degree: int, dA: torc
 
 
 def gemm_cm(ctx: Context, m: int, n: int, k: int, alpha, dA: torch.Tensor, lda: int,
-            dB: torch.Tensor, ldb: int, beta, dC: torch.Tensor, ldc: int) -> int:
+            dB: torch.Tensor, ldb: int, beta, dC: torch.Tensor, ldc: int, split: int = 0) -> int:
+    if split:
+        return int(lib().clm_gemm_split_d(ctx.handle, int(split), m, n, k, float(alpha), _ptr(dA),
+                                          lda, _ptr(dB), ldb, float(beta), _ptr(dC), ldc))
     if dA.dtype == _C128:
         a = _lib.DoubleComplex(complex(alpha).real, complex(alpha).imag)
         b = _lib.DoubleComplex(complex(beta).real, complex(beta).imag)
@@ -238,6 +260,22 @@ def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
     info = torch.empty(2, dtype=torch.int32, device=a.device)
     st, plan = expm_cm(n, acm, n, ecm, n, work, info, stream)
     check(st, "expm")
+    e = ecm.t()
+    return (e, plan) if return_plan else e
+
+
+def expm_split(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
+               stream: Optional[torch.cuda.Stream] = None, return_plan: bool = False):
+    """exp(a) for a float64 device matrix on BF16-split MFMA products (~1e-6 relative)."""
+    _require_device(a, "a")
+    if a.dtype != _F64:
+        raise TypeError("expm_split takes float64")
+    n = a.shape[-1]
+    acm = to_cm(a)
+    ecm = torch.empty_like(acm)
+    work = torch.empty(expm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+    st, plan = expm_split_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings, stream)
+    check(st, "expm_split")
     e = ecm.t()
     return (e, plan) if return_plan else e
 
@@ -311,7 +349,7 @@ def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torc
 
 
 __all__ = [
-    "CalamanError", "Context", "expm", "expm_cm", "expm_plan", "expm_buffer_size", "taylor_theta",
+    "CalamanError", "Context", "expm", "expm_cm", "expm_split", "expm_split_cm", "expm_split_plan", "expm_plan", "expm_buffer_size", "taylor_theta",
     "pade_theta", "pade_plan", "matrix_norm1", "matrix_norm1_cm", "lange", "lange_cm", "lascl",
     "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",

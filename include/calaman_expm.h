@@ -95,6 +95,8 @@ typedef struct clm_expm_plan {
   int method;    /* CLM_METHOD_* */
   int ps_block;  /* Paterson-Stockmeyer block size (0 for Pade) */
   int dtype;     /* CLM_DTYPE_* (0 when only predicted from a norm) */
+  int ps_split;  /* polynomial products: 0 = FP64 MFMA, 3 / 6 = BF16 split (clm_expm_split_d) */
+  int split_squarings; /* how many of the last squarings ran BF16-split (6 products) */
   double norm1;  /* the 1-norm the plan was chosen from */
 } clm_expm_plan;
 
@@ -134,6 +136,29 @@ clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *
 clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_double_complex *dA,
                         int lda, clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes,
                         int *dInfo, clm_expm_plan *plan);
+
+/* ---- precision variant ---------------------------------------------------------------
+ * No reference counterpart (the build's precision addition). The same algorithm
+ * as clm_expm_d with its matrix products moved onto BF16 MFMA: each FP64 operand is
+ * split into bfloat16 terms and the products accumulate in FP32
+ * (CLM_SPLIT_BF16X3: 2 terms, 3 products; CLM_SPLIT_BF16X6: 3 terms, 6 products).
+ * The 1-norm, the plan and the 2^-s scaling stay FP64; the degree comes from the
+ * Taylor thresholds at u = 2^-24; the squarings run FP64 except the last
+ * `split_squarings`, whose errors the remaining squarings amplify least. Target:
+ * relative error <= 1e-6 against FP64. */
+#define CLM_SPLIT_BF16X3 3
+#define CLM_SPLIT_BF16X6 6
+typedef struct clm_split_opts {
+  int ps_split;        /* 0 = auto (X3 when s <= 3, else X6), or CLM_SPLIT_BF16X3 / X6 */
+  int split_squarings; /* -1 = auto (1 when s >= 1), else 0..s */
+} clm_split_opts;
+/* the plan clm_expm_split_d executes for a 1-norm under opts (NULL = auto) */
+clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts);
+double clm_taylor_theta_s(int m); /* theta_m at u = 2^-24 */
+/* workspace: clm_expm_buffer_size_d */
+clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                              double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                              const clm_split_opts *opts, clm_expm_plan *plan);
 
 /* ---- 1-norm --------------------------------------------------------------------
  * matrix_norm1<T> (src/expm/norm1.cppm:417-428): max_j sum_i |A(i,j)| with the
@@ -215,6 +240,10 @@ clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs
  * on the context's stream. beta == 0 never reads C. */
 clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
                         const double *dB, int ldb, double beta, double *dC, int ldc);
+/* the product on BF16 MFMA with split operands (split = CLM_SPLIT_BF16X3 / X6), FP64 in/out */
+clm_status_t clm_gemm_split_d(clm_ctx *ctx, int split, int m, int n, int k, double alpha,
+                              const double *dA, int lda, const double *dB, int ldb, double beta,
+                              double *dC, int ldc);
 clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex alpha,
                         const clm_double_complex *dA, int lda, const clm_double_complex *dB,
                         int ldb, clm_double_complex beta, clm_double_complex *dC, int ldc);

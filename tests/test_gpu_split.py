@@ -1,0 +1,117 @@
+"""GPU: the BF16-split precision variant (clm_gemm_split_d, clm_expm_split_d).
+
+No reference counterpart: the reference runs every product in the working
+precision. The oracle here is the FP64 oracle (the C restatement of the
+reference expm) and numpy's FP64 product, with the accuracy target the
+precision variant states: relative Frobenius error <= 1e-6 for expm.
+"""
+import numpy as np
+import pytest
+import torch
+
+import calaman_g_amd as cg
+import oracle
+from calaman_g_amd import _lib
+from calaman_g_amd.expm import Context, expm_split_cm, gemm_cm
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(x, y):
+    return np.linalg.norm(x - y) / np.linalg.norm(y)
+
+
+def cm(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+
+
+@pytest.mark.parametrize("split,bound", [(3, 2e-5), (6, 5e-7)])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 128), (256, 384, 512), (1024, 1024, 1024),
+                                   (37, 53, 29), (130, 127, 131)])
+def test_split_gemm_accuracy(dev, split, bound, m, n, k):
+    """Relative error of each entry against the exact product, measured in units
+    of (|A||B|)_ij: the split keeps ~16 (3 products) or ~24 (6 products) bits."""
+    rng = np.random.default_rng(m + n + k + split)
+    a, b = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+    ctx = Context()
+    dc = torch.zeros((n, m), dtype=torch.float64, device=dev)
+    assert gemm_cm(ctx, m, n, k, 1.0, cm(a, dev), m, cm(b, dev), k, 0.0, dc, m, split=split) == 0
+    got = dc.cpu().numpy().T
+    scale = np.abs(a) @ np.abs(b)
+    err = np.abs(got - a @ b) / scale
+    assert err.max() <= bound, err.max()
+
+
+def test_split_gemm_epilogue(dev):
+    m = n = k = 256
+    rng = np.random.default_rng(9)
+    a, b, c = (rng.standard_normal((m, m)) for _ in range(3))
+    ctx = Context()
+    dc = cm(c, dev)
+    assert gemm_cm(ctx, m, n, k, 2.0, cm(a, dev), m, cm(b, dev), k, -1.0, dc, m, split=6) == 0
+    got = dc.cpu().numpy().T
+    np.testing.assert_allclose(got, 2 * a @ b - c, rtol=0, atol=1e-5 * np.abs(a).max() * np.abs(b).max() * k ** 0.5)
+
+
+def test_split_gemm_rejects_bad_split(dev):
+    x = torch.zeros(16, dtype=torch.float64, device=dev)
+    assert _lib.lib().clm_gemm_split_d(None, 4, 4, 4, 4, 1.0, x.data_ptr(), 4, x.data_ptr(), 4,
+                                       0.0, x.data_ptr(), 4) == _lib.INVALID_VALUE
+
+
+def _run_split(a, dev, ps_split=0, sq=-1):
+    n = a.shape[0]
+    da = cm(a, dev)
+    de = torch.empty_like(da)
+    work = torch.empty(cg.expm_buffer_size(n), dtype=torch.uint8, device=dev)
+    info = torch.full((2,), -1, dtype=torch.int32, device=dev)
+    st, plan = expm_split_cm(n, da, n, de, n, work, info, ps_split, sq)
+    torch.cuda.synchronize()
+    assert st == 0
+    assert info.cpu().tolist() == [0, 0]
+    return de.cpu().numpy().T, plan
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+@pytest.mark.parametrize("target", [0.5, 4.0, 8.0, 100.0])
+def test_expm_split_meets_1e6(dev, n, target):
+    rng = np.random.default_rng(n + int(target))
+    a = rng.standard_normal((n, n))
+    a *= target / np.abs(a).sum(0).max()
+    got, plan = _run_split(a, dev)
+    err = rel(got, oracle.expm(a))
+    assert err <= 1e-6, (plan, err)
+    assert plan["ps_split"] in (3, 6) and plan["split_squarings"] == min(plan["s"], 1)
+    # the plan is the one the host predicts
+    assert plan == {**cg.expm_split_plan(plan["norm1"]), "dtype": 1}
+
+
+def test_expm_split_2048(dev):
+    n = 2048
+    rng = np.random.default_rng(2048)
+    a = rng.standard_normal((n, n))
+    a *= 8.0 / np.abs(a).sum(0).max()
+    got, plan = _run_split(a, dev)
+    assert rel(got, oracle.expm(a)) <= 1e-6
+
+
+def test_expm_split_schedule_knobs(dev):
+    """All-FP64 squarings and 6-way polynomial products only tighten the error."""
+    n = 256
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((n, n))
+    a *= 100.0 / np.abs(a).sum(0).max()
+    ref = oracle.expm(a)
+    e_auto, p_auto = _run_split(a, dev)
+    e_tight, p_tight = _run_split(a, dev, ps_split=6, sq=0)
+    assert p_tight["split_squarings"] == 0 and p_tight["ps_split"] == 6
+    assert rel(e_tight, ref) <= rel(e_auto, ref) * 1.5
+    assert rel(e_tight, ref) <= 1e-6
+
+
+def test_expm_split_rejects_bad_options(dev):
+    n = 8
+    da = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    work = torch.empty(cg.expm_buffer_size(n), dtype=torch.uint8, device=dev)
+    st, _ = expm_split_cm(n, da, n, torch.empty_like(da), n, work, None, ps_split=4)
+    assert st == _lib.INVALID_VALUE

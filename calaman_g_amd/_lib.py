@@ -95,6 +95,7 @@ _SIGNATURES = {
     "clm_pade_plan_d": (ExpmPlan, [_d]),
     "clm_expm_split_plan_d": (ExpmPlan, [_d, ctypes.POINTER(SplitOpts)]),
     "clm_taylor_theta_s": (_d, [_i]),
+    "clm_expm_split_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_split_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
                               ctypes.POINTER(SplitOpts), ctypes.POINTER(ExpmPlan)]),
     "clm_gemm_split_d": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),

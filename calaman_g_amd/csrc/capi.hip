@@ -387,6 +387,104 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
   return CLM_STATUS_SUCCESS;
 }
 
+// ---- the precision variant's plane workspace ----------------------------------------------
+// bfloat16 planes (up to 3 per matrix), each plane n*n padded to 256 bytes:
+//   [PR  | 3 planes]  row-major planes of the current left operand
+//   [PCA | 3 planes]  column-major planes of A/2^s (right operand of the power bank),
+//                     reused for the squarings' right operand
+//   [PCX | 3 planes]  column-major planes of X = (A/2^s)^b (right operand of the
+//                     Paterson-Stockmeyer Horner steps)
+struct SplitWs {
+  unsigned short *PR = nullptr, *PCA = nullptr, *PCX = nullptr;
+  long pstride = 0;
+  size_t carve(void *base, int n) {
+    Layout l(base);
+    pstride = static_cast<long>(Layout::align(static_cast<size_t>(n) * n * 2) / 2);
+    PR = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
+    PCA = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
+    PCX = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
+    return l.total();
+  }
+};
+
+size_t expm_split_buffer_size(int n) {
+  ExpmWs<double> ws;
+  SplitWs sp;
+  return ws.carve(nullptr, n) + sp.carve(nullptr, n);
+}
+
+clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
+                          unsigned short *pc, unsigned short *pr, long pstride) {
+  CLM_TRY_HIP(launch_split_planes(st, split, n, n, X, ldx, pc, pr, pstride));
+  return CLM_STATUS_SUCCESS;
+}
+
+// the split product C = PR-planes x PC-planes (+ epilogue in g)
+clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const unsigned short *pa,
+                         const unsigned short *pb, long pstride) {
+  CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, pa, pb, pstride));
+  return CLM_STATUS_SUCCESS;
+}
+
+// Paterson-Stockmeyer with every product on pre-split planes: each operand is
+// split once (the power bank's left factors, A and X once each, the Horner
+// accumulator once per step), the products never re-split inside their loops.
+clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int degree,
+                               const double *A, long lda, double *P, long ldp, double *bank,
+                               long stride, double *scratch, long ld_scratch, int s_requested,
+                               int split, const SplitWs &sp) {
+  const clm_ps_plan p = ps_plan(degree, s_requested);
+  const bool need_a_planes = p.num_powers > 0 || (p.r >= 1 && p.s < 2);
+  if (need_a_planes)
+    CLM_TRY(split_planes(st, split, n, A, lda, sp.PCA, p.num_powers > 0 ? sp.PR : nullptr,
+                         sp.pstride));
+  auto plain = [&](double *C, long ldc) {
+    GemmArgs<double> g;
+    g.M = g.N = g.K = n;
+    g.C = C;
+    g.ldc = ldc;
+    g.alpha = 1.0;
+    return g;
+  };
+  for (int k = 2; k <= p.num_powers + 1; ++k) {
+    if (k > 2)
+      CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(k - 3) * stride, n, nullptr,
+                           sp.PR, sp.pstride));
+    CLM_TRY(gemm_planes(st, plain(bank + static_cast<long>(k - 2) * stride, n), split, sp.PR,
+                        sp.PCA, sp.pstride));
+  }
+  const unsigned short *pcx = sp.PCA;
+  if (p.r >= 1 && p.s >= 2) {
+    CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(p.s - 2) * stride, n, sp.PCX,
+                         nullptr, sp.pstride));
+    pcx = sp.PCX;
+  }
+  double *buf[2] = {P, scratch};
+  const long ld[2] = {ldp, ld_scratch};
+  int cur = (p.r % 2 == 0) ? 0 : 1;
+  auto terms = [&](int j) {
+    const int remaining = degree - j * p.s + 1;
+    return remaining < p.s ? remaining : p.s;
+  };
+  CLM_TRY_HIP(launch_build_block(st, n, buf[cur], ld[cur], coef + static_cast<long>(p.r) * p.s,
+                                 terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
+  for (int j = p.r - 1; j >= 0; --j) {
+    const int next = 1 - cur;
+    CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, sp.PR, sp.pstride));
+    GemmArgs<double> g = plain(buf[next], ld[next]);
+    g.coef = coef + static_cast<long>(j) * p.s;
+    g.nterms = terms(j);
+    g.X1 = A;
+    g.ldx1 = lda;
+    g.bank = bank;
+    g.bank_stride = stride;
+    g.ldbank = n;
+    CLM_TRY(gemm_planes(st, g, split, sp.PR, pcx, sp.pstride));
+    cur = next;
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
 template <typename T>
 clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, double *colsum,
                         double *norm) {
@@ -405,14 +503,17 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
       dE == nullptr)
     return CLM_STATUS_INVALID_VALUE;
-  if (bytes < expm_buffer_size<T>(n)) return CLM_STATUS_ALLOC_FAILED;
+  if (bytes < (use_split ? expm_split_buffer_size(n) : expm_buffer_size<T>(n)))
+    return CLM_STATUS_ALLOC_FAILED;
   clm_ctx *c = nullptr;
   CLM_TRY(resolve_ctx(ctx, &c));
   const T *coef = taylor_coefficients<T>(c->device);
   if (!coef) return CLM_STATUS_EXECUTION_FAILED;
 
   ExpmWs<T> ws;
-  ws.carve(work, n);
+  const size_t fp64_bytes = ws.carve(work, n);
+  SplitWs sp;
+  if (use_split) sp.carve(static_cast<char *>(work) + fp64_bytes, n);
 
   double norm = 0.0;
   CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
@@ -437,14 +538,39 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   T *other = (p.s % 2 == 0) ? ws.S : dE;
   const long ld_other = (p.s % 2 == 0) ? n : lde;
 
-  CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                         static_cast<long>(ws.stride), other, ld_other, p.ps_block, p.ps_split));
+  if constexpr (std::is_same_v<T, double>) {
+    if (use_split) {
+      CLM_TRY(ps_evaluate_split(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+                                static_cast<long>(ws.stride), other, ld_other, p.ps_block,
+                                p.ps_split, sp));
+    } else {
+      CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+                             static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+    }
+  } else {
+    CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+                           static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+  }
 
   T *cur = target, *nxt = other;
   long ldc = ld_target, ldn = ld_other;
   for (int k = 0; k < p.s; ++k) {
-    const int prec = (k >= p.s - p.split_squarings) ? CLM_SPLIT_BF16X6 : 0;
-    CLM_TRY(gemm(st, n, cur, ldc, cur, ldc, nxt, ldn, prec));
+    const bool split_sq = k >= p.s - p.split_squarings;
+    if constexpr (std::is_same_v<T, double>) {
+      if (split_sq) {
+        CLM_TRY(split_planes(st, CLM_SPLIT_BF16X6, n, cur, ldc, sp.PCA, sp.PR, sp.pstride));
+        GemmArgs<double> g;
+        g.M = g.N = g.K = n;
+        g.C = nxt;
+        g.ldc = ldn;
+        g.alpha = 1.0;
+        CLM_TRY(gemm_planes(st, g, CLM_SPLIT_BF16X6, sp.PR, sp.PCA, sp.pstride));
+        std::swap(cur, nxt);
+        std::swap(ldc, ldn);
+        continue;
+      }
+    }
+    CLM_TRY(gemm(st, n, cur, ldc, cur, ldc, nxt, ldn));
     std::swap(cur, nxt);
     std::swap(ldc, ldn);
   }
@@ -680,6 +806,12 @@ double clm_taylor_theta_s(int m) {
   for (int i = 0; i < clm::kNumTaylor; ++i)
     if (clm::kTaylorDegrees[i] == m) return clm::kTaylorThetaS[i];
   return 0.0;
+}
+
+clm_status_t clm_expm_split_buffer_size_d(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_split_buffer_size(n);
+  return CLM_STATUS_SUCCESS;
 }
 
 clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,

@@ -107,6 +107,15 @@ hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g);
 // the same product on BF16 MFMA with operands split into 2 (split = 3 products)
 // or 3 (split = 6 products) bfloat16 terms, FP32 accumulation, FP64 in and out
 hipError_t launch_gemm_split_bf16(hipStream_t stream, const GemmArgs<double> &g, int split);
+// bfloat16 planes of an FP64 rows x cols matrix (2 planes for split 3, 3 for split 6):
+// pc column-major (ld rows), pr row-major (ld cols); either may be null
+hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols, const double *X,
+                               long ldx, unsigned short *pc, unsigned short *pr, long pstride);
+// the split product from pre-split planes: PA = row-major planes of A (M x K),
+// PB = column-major planes of B (K x N); g.A / g.B are not read
+hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
+                                    const unsigned short *PA, const unsigned short *PB,
+                                    long pstride);
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g);
 template <typename T>
 hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);

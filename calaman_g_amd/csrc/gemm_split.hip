@@ -260,6 +260,195 @@ __global__ void __launch_bounds__(256)
       }
 }
 
+// ------------------------------------------------------------------------------------
+// Split once, multiply many times. split_planes writes the bfloat16 planes of an
+// FP64 matrix X (rows x cols, column-major, ldx) in the layouts the MFMA operands
+// want -- k contiguous:
+//   pc: column-major planes, pc[t*pstride + i + j*rows]  (X as the right operand)
+//   pr: row-major planes,    pr[t*pstride + i*cols + j]  (X as the left operand)
+// The row-major write goes through an LDS transpose so both the FP64 read and the
+// plane writes are coalesced. dgemm_split_planes then stages the planes straight
+// into LDS with 16-byte loads: its loop is loads, LDS and MFMA only.
+// ------------------------------------------------------------------------------------
+template <int NPL>
+__global__ void __launch_bounds__(256)
+    split_planes_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
+                        unsigned short *__restrict__ pc, unsigned short *__restrict__ pr,
+                        long pstride) {
+  __shared__ unsigned short tile[NPL][64][65];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int jj = ty; jj < 64; jj += 4) {
+    const int i = i0 + tx, j = j0 + jj;
+    unsigned short p[NPL];
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) p[t] = 0;
+    if (i < rows && j < cols) {
+      split_bf16<NPL>(X[i + (long)j * ldx], p);
+      if (pc) {
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) pc[t * pstride + i + (long)j * rows] = p[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) tile[t][jj][tx] = p[t];
+  }
+  if (!pr) return;
+  __syncthreads();
+  for (int ii = ty; ii < 64; ii += 4) {
+    const int i = i0 + ii, j = j0 + tx;
+    if (i < rows && j < cols) {
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) pr[t * pstride + (long)i * cols + j] = tile[t][tx][ii];
+    }
+  }
+}
+
+constexpr int kPlaneA = 128 * kSplitRow;  // bf16 per A plane in LDS (128 rows)
+constexpr int kPlaneB = 128 * kSplitRow;
+
+template <int NPL, bool kFast>
+__device__ __forceinline__ void planes_gload(const GemmArgs<double> &g, const unsigned short *PA,
+                                             const unsigned short *PB, long pstride, int m0, int n0,
+                                             int k0, int tid, uint4 (&ra)[NPL][2],
+                                             uint4 (&rb)[NPL][2]) {
+  const long K = g.K;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t)
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int c = tid + it * 256;
+      const int r = c >> 2, kq = (c & 3) * 8;
+      const int gk = k0 + kq;
+      if (kFast) {
+        ra[t][it] = *reinterpret_cast<const uint4 *>(PA + t * pstride + (long)(m0 + r) * K + gk);
+        rb[t][it] = *reinterpret_cast<const uint4 *>(PB + t * pstride + (long)(n0 + r) * K + gk);
+      } else {
+        unsigned va[8], vb[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const bool kin = gk + e < K;
+          va[e] = (kin && m0 + r < g.M) ? PA[t * pstride + (long)(m0 + r) * K + gk + e] : 0u;
+          vb[e] = (kin && n0 + r < g.N) ? PB[t * pstride + (long)(n0 + r) * K + gk + e] : 0u;
+        }
+        ra[t][it] = uint4{va[0] | (va[1] << 16), va[2] | (va[3] << 16), va[4] | (va[5] << 16),
+                          va[6] | (va[7] << 16)};
+        rb[t][it] = uint4{vb[0] | (vb[1] << 16), vb[2] | (vb[3] << 16), vb[4] | (vb[5] << 16),
+                          vb[6] | (vb[7] << 16)};
+      }
+    }
+}
+
+template <int NPL>
+__device__ __forceinline__ void planes_sstore(unsigned short *S, int tid, const uint4 (&ra)[NPL][2],
+                                              const uint4 (&rb)[NPL][2]) {
+#pragma unroll
+  for (int t = 0; t < NPL; ++t)
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int c = tid + it * 256;
+      const int r = c >> 2, kq = (c & 3) * 8;
+      *reinterpret_cast<uint4 *>(&S[t * kPlaneA + r * kSplitRow + kq]) = ra[t][it];
+      *reinterpret_cast<uint4 *>(&S[NPL * kPlaneA + t * kPlaneB + r * kSplitRow + kq]) = rb[t][it];
+    }
+}
+
+template <int SPLIT>
+__device__ __forceinline__ void planes_compute(const unsigned short *S, int lane, int wm0, int wn0,
+                                               f4 (&acc)[4][4]) {
+  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  const int koff = 8 * (lane >> 4);
+  bf16x8 a[NPL][4], b[NPL][4];
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      a[t][i] = *reinterpret_cast<const bf16x8 *>(
+          &S[t * kPlaneA + (wm0 + i * 16 + (lane & 15)) * kSplitRow + koff]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      b[t][j] = *reinterpret_cast<const bf16x8 *>(
+          &S[NPL * kPlaneA + t * kPlaneB + (wn0 + j * 16 + (lane & 15)) * kSplitRow + koff]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if constexpr (SPLIT == 6) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+      }
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// C = alpha * A * B (+ epilogue) from the planes of A (row-major, row stride K)
+// and of B (column-major, column stride K). Same tile, wave and MFMA shape as
+// dgemm_split_bf16; LDS double-buffered, one barrier per K slice of 32.
+template <int SPLIT, bool kFast>
+__global__ void __launch_bounds__(256)
+    dgemm_split_planes(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
+                       const unsigned short *__restrict__ PB, const long pstride,
+                       const int tiles_m, const int tiles_n) {
+  constexpr int BM = 128, BN = 128, WM = 64, WN = 64;
+  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int PLANE_A = BM * kSplitRow;
+  constexpr int PLANE_B = BN * kSplitRow;
+  constexpr int STAGE = NPL * (PLANE_A + PLANE_B);
+
+  __shared__ __attribute__((aligned(16))) unsigned short L[2 * STAGE];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords_s(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int wm0 = (wave & 1) * WM;
+  const int wn0 = (wave >> 1) * WN;
+  const long K = g.K;
+
+  f4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // 16-byte chunks: per plane, 128 rows x 4 chunks of 8 k = 512, two per lane
+  uint4 ra[NPL][2], rb[NPL][2];
+
+  const int KT = (g.K + kSplitBK - 1) / kSplitBK;
+  if (KT > 0) {
+    planes_gload<NPL, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
+    planes_sstore<NPL>(L, tid, ra, rb);
+    __syncthreads();
+    for (int kt = 0; kt < KT - 1; ++kt) {
+      const int cur = kt & 1;
+      planes_gload<NPL, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * kSplitBK, tid, ra, rb);
+      planes_compute<SPLIT>(L + cur * STAGE, lane, wm0, wn0, acc);
+      planes_sstore<NPL>(L + (cur ^ 1) * STAGE, tid, ra, rb);
+      __syncthreads();
+    }
+    planes_compute<SPLIT>(L + ((KT - 1) & 1) * STAGE, lane, wm0, wn0, acc);
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + d_row(lane, r);
+        const int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (kFast || (row < g.M && col < g.N))
+          epilogue_store_s(g, row, col, static_cast<double>(acc[i][j][r]));
+      }
+}
+
 inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 template <int SPLIT>
@@ -278,6 +467,45 @@ hipError_t launch_split(hipStream_t stream, const GemmArgs<double> &g) {
 hipError_t launch_gemm_split_bf16(hipStream_t stream, const GemmArgs<double> &g, int split) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   return split == 6 ? launch_split<6>(stream, g) : launch_split<3>(stream, g);
+}
+
+hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols, const double *X,
+                               long ldx, unsigned short *pc, unsigned short *pr, long pstride) {
+  if (rows <= 0 || cols <= 0 || (!pc && !pr)) return hipSuccess;
+  const dim3 grid((rows + 63) / 64, (cols + 63) / 64), block(256);
+  if (split == 6)
+    hipLaunchKernelGGL(split_planes_kernel<3>, grid, block, 0, stream, rows, cols, X, ldx, pc, pr,
+                       pstride);
+  else
+    hipLaunchKernelGGL(split_planes_kernel<2>, grid, block, 0, stream, rows, cols, X, ldx, pc, pr,
+                       pstride);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
+                                    const unsigned short *PA, const unsigned short *PB,
+                                    long pstride) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
+  const bool fast = g.M % 128 == 0 && g.N % 128 == 0 && g.K % kSplitBK == 0 &&
+                    aligned16s(PA) && aligned16s(PB) && (pstride % 8) == 0;
+  const dim3 grid(tiles_m * tiles_n), block(256);
+  if (split == 6) {
+    if (fast)
+      hipLaunchKernelGGL((dgemm_split_planes<6, true>), grid, block, 0, stream, g, PA, PB, pstride,
+                         tiles_m, tiles_n);
+    else
+      hipLaunchKernelGGL((dgemm_split_planes<6, false>), grid, block, 0, stream, g, PA, PB,
+                         pstride, tiles_m, tiles_n);
+  } else {
+    if (fast)
+      hipLaunchKernelGGL((dgemm_split_planes<3, true>), grid, block, 0, stream, g, PA, PB, pstride,
+                         tiles_m, tiles_n);
+    else
+      hipLaunchKernelGGL((dgemm_split_planes<3, false>), grid, block, 0, stream, g, PA, PB,
+                         pstride, tiles_m, tiles_n);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace clm

@@ -187,6 +187,12 @@ def expm_split_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int
     return int(st), plan.as_dict()
 
 
+def expm_split_buffer_size(n: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(lib().clm_expm_split_buffer_size_d(int(n), ctypes.byref(out)), "expm_split_buffer_size")
+    return int(out.value)
+
+
 def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1) -> dict:
     opts = _lib.SplitOpts(int(ps_split), int(split_squarings))
     return lib().clm_expm_split_plan_d(float(norm1), ctypes.byref(opts)).as_dict()
@@ -273,7 +279,7 @@ def expm_split(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
     n = a.shape[-1]
     acm = to_cm(a)
     ecm = torch.empty_like(acm)
-    work = torch.empty(expm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+    work = torch.empty(expm_split_buffer_size(n), dtype=torch.uint8, device=a.device)
     st, plan = expm_split_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings, stream)
     check(st, "expm_split")
     e = ecm.t()
@@ -349,7 +355,7 @@ def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torc
 
 
 __all__ = [
-    "CalamanError", "Context", "expm", "expm_cm", "expm_split", "expm_split_cm", "expm_split_plan", "expm_plan", "expm_buffer_size", "taylor_theta",
+    "CalamanError", "Context", "expm", "expm_cm", "expm_split", "expm_split_cm", "expm_split_plan", "expm_split_buffer_size", "expm_plan", "expm_buffer_size", "taylor_theta",
     "pade_theta", "pade_plan", "matrix_norm1", "matrix_norm1_cm", "lange", "lange_cm", "lascl",
     "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",

@@ -155,7 +155,9 @@ typedef struct clm_split_opts {
 /* the plan clm_expm_split_d executes for a 1-norm under opts (NULL = auto) */
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts);
 double clm_taylor_theta_s(int m); /* theta_m at u = 2^-24 */
-/* workspace: clm_expm_buffer_size_d */
+/* workspace: the FP64 layout of clm_expm_buffer_size_d plus nine n x n bfloat16
+ * planes (each operand is split into planes once, then multiplied) */
+clm_status_t clm_expm_split_buffer_size_d(int n, size_t *bytes);
 clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
                               double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
                               const clm_split_opts *opts, clm_expm_plan *plan);

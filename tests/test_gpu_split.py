@@ -63,7 +63,7 @@ def _run_split(a, dev, ps_split=0, sq=-1):
     n = a.shape[0]
     da = cm(a, dev)
     de = torch.empty_like(da)
-    work = torch.empty(cg.expm_buffer_size(n), dtype=torch.uint8, device=dev)
+    work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
     info = torch.full((2,), -1, dtype=torch.int32, device=dev)
     st, plan = expm_split_cm(n, da, n, de, n, work, info, ps_split, sq)
     torch.cuda.synchronize()
@@ -112,6 +112,6 @@ def test_expm_split_schedule_knobs(dev):
 def test_expm_split_rejects_bad_options(dev):
     n = 8
     da = torch.zeros((n, n), dtype=torch.float64, device=dev)
-    work = torch.empty(cg.expm_buffer_size(n), dtype=torch.uint8, device=dev)
+    work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
     st, _ = expm_split_cm(n, da, n, torch.empty_like(da), n, work, None, ps_split=4)
     assert st == _lib.INVALID_VALUE

@@ -5,7 +5,10 @@ matrices, entries N(0,1) from numpy's default_rng(seed), rescaled to
 ||A||_1 in {0.5, 8, 100} (so s = 0 .. 5 squarings). One step = one expm of each
 of the three matrices through the C ABI (clm_expm_d), inputs already resident
 in HBM; value = expm/s over the timed steps. complex128 at the same sizes is
-reported in "breakdown".
+reported in "breakdown", and so is the BF16-split precision variant
+(clm_expm_split_d) on the same three matrices: its expm/s, its speedup over
+FP64 and its relative error against the FP64 oracle (the metric's target is
+<= 1e-6).
 
 Also reported, on the same JSON line:
   roofline      the dominant kernel (the 128x128 FP64 MFMA GEMM that every
@@ -53,25 +56,28 @@ def make_inputs(n, cplx):
     return mats
 
 
-def time_expms(mats, dev, steps, warmup):
-    """Returns (seconds per step, plans, results of the last step)."""
+def time_expms(mats, dev, steps, warmup, split=False):
+    """Returns (seconds per step, plans, results of the last step). split=True runs
+    the BF16-split precision variant (clm_expm_split_d) instead of clm_expm_d."""
     import numpy as np
     import torch
     import calaman_g_amd as cg
-    from calaman_g_amd.expm import expm_cm
+    from calaman_g_amd.expm import expm_cm, expm_split_cm
 
     n = mats[0].shape[0]
     dt = torch.complex128 if np.iscomplexobj(mats[0]) else torch.float64
     das = [torch.from_numpy(np.ascontiguousarray(a.T)).to(dev) for a in mats]
     des = [torch.empty_like(d) for d in das]
-    work = torch.empty(cg.expm_buffer_size(n, dt), dtype=torch.uint8, device=dev)
+    nbytes = cg.expm_split_buffer_size(n) if split else cg.expm_buffer_size(n, dt)
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     info = torch.empty(2, dtype=torch.int32, device=dev)
     plans = []
+    run = expm_split_cm if split else expm_cm
 
     def step():
         plans.clear()
         for da, de in zip(das, des):
-            st, plan = expm_cm(n, da, n, de, n, work, info)
+            st, plan = run(n, da, n, de, n, work, info)
             if st != 0:
                 raise RuntimeError(f"clm_expm failed: {st:#x}")
             plans.append(plan)
@@ -187,6 +193,9 @@ def main():
     expm_per_s = len(NORMS) / sec
     log(f"fp64: {sec * 1e3:.3f} ms/step, {expm_per_s:.1f} expm/s, plans {plans}")
 
+    ssec, splans, sresults = time_expms(real, dev, args.steps, args.warmup, split=True)
+    log(f"split-bf16: {ssec * 1e3:.3f} ms/step, {len(NORMS) / ssec:.1f} expm/s, plans {splans}")
+
     cplx = make_inputs(N, cplx=True)
     zsec, zplans, _ = time_expms(cplx, dev, max(2, args.steps // 4), 1)
     log(f"complex128: {zsec * 1e3:.3f} ms/step, {len(NORMS) / zsec:.1f} expm/s")
@@ -196,10 +205,12 @@ def main():
 
     cpu = None
     errs = None
+    serrs = None
     if not args.no_cpu:
         cpu, outs = cpu_baseline(real)
         errs = [float(np.linalg.norm(r - o) / np.linalg.norm(o)) for r, o in zip(results, outs)]
-        log(f"cpu baseline: {cpu}; rel err vs oracle {errs}")
+        serrs = [float(np.linalg.norm(r - o) / np.linalg.norm(o)) for r, o in zip(sresults, outs)]
+        log(f"cpu baseline: {cpu}; rel err vs oracle fp64 {errs} split {serrs}")
 
     per_norm = {}
     for t, p in zip(NORMS, plans):
@@ -223,8 +234,18 @@ def main():
                    "method": "Taylor/Paterson-Stockmeyer + scaling-and-squaring"},
         "plans": per_norm,
         "rel_err_vs_oracle": errs,
-        "breakdown": {"complex128_expm_per_s": round(len(NORMS) / zsec, 3),
-                      "complex128_ms_per_step": round(zsec * 1e3, 4)},
+        "breakdown": {
+            "split_bf16_expm_per_s": round(len(NORMS) / ssec, 3),
+            "split_bf16_ms_per_step": round(ssec * 1e3, 4),
+            "split_bf16_speedup_vs_fp64": round(sec / ssec, 3),
+            "split_bf16_rel_err_vs_fp64_oracle": serrs,
+            "split_bf16_max_rel_err": (max(serrs) if serrs else None),
+            "split_bf16_plans": {str(t): {"taylor_degree": p["m"], "s": p["s"],
+                                          "ps_split": p["ps_split"],
+                                          "split_squarings": p["split_squarings"]}
+                                 for t, p in zip(NORMS, splans)},
+            "complex128_expm_per_s": round(len(NORMS) / zsec, 3),
+            "complex128_ms_per_step": round(zsec * 1e3, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

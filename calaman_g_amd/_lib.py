@@ -29,7 +29,9 @@ RUNTIME_ERROR = CLM_DOMAIN_RUNTIME | 2
 METHOD_TAYLOR_PS = 0
 METHOD_PADE = 1
 DTYPE_F64 = 1
+DTYPE_F32 = 2
 DTYPE_C128 = 3
+DTYPE_C64 = 4
 
 NORM_MAX_ABS, NORM_ONE, NORM_INF, NORM_FROBENIUS = 0, 1, 2, 3
 
@@ -67,6 +69,10 @@ class DoubleComplex(ctypes.Structure):
     _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
 
 
+class FloatComplex(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
+
+
 class CalamanError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -80,6 +86,7 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _sz = ctypes.c_size_t
 _d = ctypes.c_double
+_f = ctypes.c_float
 
 # name -> (restype, argtypes)
 _SIGNATURES = {
@@ -125,6 +132,32 @@ _SIGNATURES = {
     "clm_gemm_d": (_i, [_vp, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
     "clm_gemm_z": (_i, [_vp, _i, _i, _i, DoubleComplex, _vp, _i, _vp, _i, DoubleComplex, _vp,
                         _i]),
+    # single precision
+    "clm_expm_plan_s": (ExpmPlan, [_d]),
+    "clm_pade_theta_s": (_d, [_i]),
+    "clm_pade_plan_s": (ExpmPlan, [_f]),
+    "clm_expm_buffer_size_s": (_i, [_i, ctypes.POINTER(_sz)]),
+    "clm_expm_buffer_size_c": (_i, [_i, ctypes.POINTER(_sz)]),
+    "clm_expm_s": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
+    "clm_expm_c": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
+    "clm_matrix_norm1_s": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_f)]),
+    "clm_matrix_norm1_c": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_f)]),
+    "clm_lange_s": (_i, [_vp, _i, _sz, _sz, _vp, _sz, _vp, _vp]),
+    "clm_lange_c": (_i, [_vp, _i, _sz, _sz, _vp, _sz, _vp, _vp]),
+    "clm_lascl_s": (_i, [_vp, _f, _f, _sz, _sz, _vp, _sz]),
+    "clm_lascl_c": (_i, [_vp, _f, _f, _sz, _sz, _vp, _sz]),
+    "clm_paterson_stockmeyer_power_stride_s": (_sz, [_i]),
+    "clm_paterson_stockmeyer_power_stride_c": (_sz, [_i]),
+    "clm_paterson_stockmeyer_buffer_size_s": (_sz, [_i, _i, _i]),
+    "clm_paterson_stockmeyer_buffer_size_c": (_sz, [_i, _i, _i]),
+    "clm_paterson_stockmeyer_s": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _i]),
+    "clm_paterson_stockmeyer_c": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _i]),
+    "clm_horner_buffer_size_s": (_sz, [_i]),
+    "clm_horner_buffer_size_c": (_sz, [_i]),
+    "clm_horner_s": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "clm_horner_c": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "clm_gemm_s": (_i, [_vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
+    "clm_gemm_c": (_i, [_vp, _i, _i, _i, FloatComplex, _vp, _i, _vp, _i, FloatComplex, _vp, _i]),
 }
 
 

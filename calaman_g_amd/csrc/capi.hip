@@ -15,6 +15,7 @@
 // Argument checks come before any HIP call, so rejected calls never touch a device.
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <type_traits>
 
@@ -60,25 +61,16 @@ clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
 }
 
 template <typename T>
-const T *taylor_coefficients(int device);
-template <>
-const double *taylor_coefficients<double>(int device) {
-  return inv_factorials_d(device);
-}
-template <>
-const zd *taylor_coefficients<zd>(int device) {
-  return inv_factorials_z(device);
+constexpr int dtype_code() {
+  if constexpr (std::is_same_v<T, double>) return CLM_DTYPE_F64;
+  else if constexpr (std::is_same_v<T, float>) return CLM_DTYPE_F32;
+  else if constexpr (std::is_same_v<T, zd>) return CLM_DTYPE_C128;
+  else return CLM_DTYPE_C64;
 }
 
 template <typename T>
-constexpr int dtype_code();
-template <>
-constexpr int dtype_code<double>() {
-  return CLM_DTYPE_F64;
-}
-template <>
-constexpr int dtype_code<zd>() {
-  return CLM_DTYPE_C128;
+constexpr bool is_single() {
+  return std::is_same_v<real_t<T>, float>;
 }
 
 template <typename T>
@@ -256,7 +248,18 @@ clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
   return p;
 }
 
-// ---- the reference's Pade ladder (src/expm/plan.cppm:246-326, expm_bridge.h:128-130) ----
+// ---- the reference's Pade ladder (src/expm/plan.cppm:51-130, expm_bridge.h:128-130) ----
+double pade_theta_single(int m) {
+  switch (m) {
+  case 3: return static_cast<double>(4.258730034897931e-1f);
+  case 5: return static_cast<double>(1.880152698533769f);
+  case 7: return static_cast<double>(3.925724846433284f);
+  case 9: return static_cast<double>(6.249156334514102f);
+  case 13: return static_cast<double>(1.124873763647540e1f);
+  default: return 0.0;
+  }
+}
+
 double pade_theta(int m) {
   switch (m) {
   case 3: return 1.495585217958292e-2;
@@ -290,6 +293,30 @@ clm_expm_plan pade_plan(double norm1) {
   return p;
 }
 
+// expm_plan<float>: the comparisons and the log2 in float, as the reference's
+// template does for R = float
+clm_expm_plan pade_plan_single(float norm1) {
+  clm_expm_plan p{};
+  p.method = CLM_METHOD_PADE;
+  p.norm1 = norm1;
+  const int degrees[5] = {3, 5, 7, 9, 13};
+  for (int m : degrees) {
+    if (norm1 <= static_cast<float>(pade_theta_single(m))) {
+      p.m = m;
+      p.s = 0;
+      p.num_gemms = pade_num_gemms(m);
+      return p;
+    }
+  }
+  const float theta = static_cast<float>(pade_theta_single(13));
+  int s = static_cast<int>(std::ceil(std::log2(norm1 / theta)));
+  if (s < 0) s = 0;
+  p.m = 13;
+  p.s = s;
+  p.num_gemms = pade_num_gemms(13) + s;
+  return p;
+}
+
 // ---- expm workspace --------------------------------------------------------------------------
 //   [colsum | n doubles ]  per-column |.| sums; colsum[0] ends as the 1-norm
 //   [As     | n*n T     ]  2^-s A
@@ -299,7 +326,7 @@ clm_expm_plan pade_plan(double norm1) {
 // (src/expm/README.md:206-211); nothing for an LU.
 template <typename T>
 struct ExpmWs {
-  double *colsum = nullptr;
+  real_t<T> *colsum = nullptr;
   T *As = nullptr;
   T *bank = nullptr;
   T *S = nullptr;
@@ -308,7 +335,7 @@ struct ExpmWs {
     Layout l(base);
     const size_t nn = static_cast<size_t>(n) * n;
     stride = power_stride<T>(n);
-    colsum = l.take<double>(static_cast<size_t>(n));
+    colsum = l.take<real_t<T>>(static_cast<size_t>(n));
     As = l.take<T>(nn);
     bank = l.take<T>(static_cast<size_t>(max_bank_blocks()) * stride);
     S = l.take<T>(nn);
@@ -486,11 +513,11 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
 }
 
 template <typename T>
-clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, double *colsum,
-                        double *norm) {
+clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, real_t<T> *colsum,
+                        real_t<T> *norm) {
   CLM_TRY_HIP(launch_norm1(st, n, A, lda, colsum));
-  double host = 0.0;
-  CLM_TRY_HIP(hipMemcpyAsync(&host, colsum, sizeof(double), hipMemcpyDeviceToHost, st));
+  real_t<T> host = 0;
+  CLM_TRY_HIP(hipMemcpyAsync(&host, colsum, sizeof(host), hipMemcpyDeviceToHost, st));
   CLM_TRY_HIP(hipStreamSynchronize(st));
   *norm = host;
   return CLM_STATUS_SUCCESS;
@@ -507,7 +534,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
     return CLM_STATUS_ALLOC_FAILED;
   clm_ctx *c = nullptr;
   CLM_TRY(resolve_ctx(ctx, &c));
-  const T *coef = taylor_coefficients<T>(c->device);
+  const T *coef = inv_factorials<T>(c->device);
   if (!coef) return CLM_STATUS_EXECUTION_FAILED;
 
   ExpmWs<T> ws;
@@ -515,11 +542,14 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   SplitWs sp;
   if (use_split) sp.carve(static_cast<char *>(work) + fp64_bytes, n);
 
-  double norm = 0.0;
+  real_t<T> norm = 0;
   CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
 
-  clm_expm_plan p = use_split ? split_plan(norm, split) : taylor_plan(norm);
+  // the degree ladder at the element type's unit roundoff (2^-53 or 2^-24)
+  clm_expm_plan p = use_split      ? split_plan(norm, split)
+                    : is_single<T>() ? taylor_plan(norm, kTaylorThetaS)
+                                     : taylor_plan(norm);
   p.dtype = dtype_code<T>();
   if (plan) *plan = p;
 
@@ -674,8 +704,8 @@ clm_status_t gemm_impl(clm_ctx *ctx, int m, int n, int k, T alpha, const T *dA, 
 }
 
 template <typename T>
-clm_status_t norm1_impl(clm_stream_t stream, int n, const T *dA, int lda, double *dColsum,
-                        double *norm) {
+clm_status_t norm1_impl(clm_stream_t stream, int n, const T *dA, int lda, real_t<T> *dColsum,
+                        real_t<T> *norm) {
   if (n < 1 || n > kMaxDim || lda < n || dA == nullptr || dColsum == nullptr || norm == nullptr)
     return CLM_STATUS_INVALID_VALUE;
   return read_norm1(static_cast<hipStream_t>(stream), n, dA, lda, dColsum, norm);
@@ -683,7 +713,7 @@ clm_status_t norm1_impl(clm_stream_t stream, int n, const T *dA, int lda, double
 
 template <typename T>
 clm_status_t lange_impl(clm_stream_t stream, int which, size_t m, size_t n, const T *dA,
-                        size_t lda, double *dResult, double *dScratch) {
+                        size_t lda, real_t<T> *dResult, real_t<T> *dScratch) {
   if (which < CLM_NORM_MAX_ABS || which > CLM_NORM_FROBENIUS || dResult == nullptr ||
       lda < (m > 1 ? m : 1))
     return CLM_STATUS_RUNTIME_INVALID_VALUE;
@@ -696,31 +726,33 @@ clm_status_t lange_impl(clm_stream_t stream, int which, size_t m, size_t n, cons
 }
 
 // ?lascl's guarded multiplier loop, verbatim in structure from netlib via the
-// reference (src/lapack/lascl/interface.cppm:101-140); each factor is one pass.
+// reference (src/lapack/lascl/interface.cppm:101-140), in the element type's
+// real precision R; each factor is one pass.
 template <typename T>
-clm_status_t lascl_impl(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n, T *dA,
-                        size_t lda) {
-  if (cfrom == 0.0 || std::isnan(cfrom) || std::isnan(cto)) return CLM_STATUS_RUNTIME_INVALID_VALUE;
+clm_status_t lascl_impl(clm_stream_t stream, real_t<T> cfrom, real_t<T> cto, size_t m, size_t n,
+                        T *dA, size_t lda) {
+  using R = real_t<T>;
+  if (cfrom == R(0) || std::isnan(cfrom) || std::isnan(cto)) return CLM_STATUS_RUNTIME_INVALID_VALUE;
   if (m == 0 || n == 0) return CLM_STATUS_SUCCESS;
   if (dA == nullptr || lda < m) return CLM_STATUS_RUNTIME_INVALID_VALUE;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const double smlnum = 2.2250738585072014e-308;  // DBL_MIN, DLAMCH('S')
-  const double bignum = 1.0 / smlnum;
-  double cfromc = cfrom, ctoc = cto;
+  const R smlnum = std::numeric_limits<R>::min();  // ?LAMCH('S')
+  const R bignum = R(1) / smlnum;
+  R cfromc = cfrom, ctoc = cto;
   bool done = false;
   do {
-    const double cfrom1 = cfromc * smlnum;
-    double mul;
+    const R cfrom1 = cfromc * smlnum;
+    R mul;
     if (cfrom1 == cfromc) {
       mul = ctoc / cfromc;
       done = true;
     } else {
-      const double cto1 = ctoc / bignum;
+      const R cto1 = ctoc / bignum;
       if (cto1 == ctoc) {
         mul = ctoc;
         done = true;
-        cfromc = 1.0;
-      } else if (std::fabs(cfrom1) > std::fabs(ctoc) && ctoc != 0.0) {
+        cfromc = R(1);
+      } else if (std::fabs(cfrom1) > std::fabs(ctoc) && ctoc != R(0)) {
         mul = smlnum;
         done = false;
         cfromc = cfrom1;
@@ -746,9 +778,11 @@ clm_status_t lascl_impl(clm_stream_t stream, double cfrom, double cto, size_t m,
 }  // namespace
 }  // namespace clm
 
+using clm::zc;
 using clm::zd;
 
 static_assert(sizeof(clm_double_complex) == sizeof(zd), "complex layout");
+static_assert(sizeof(clm_float_complex) == sizeof(zc), "complex layout");
 
 extern "C" {
 
@@ -958,6 +992,126 @@ clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex al
   return clm::gemm_impl<zd>(ctx, m, n, k, zd{alpha.re, alpha.im}, reinterpret_cast<const zd *>(dA),
                             lda, reinterpret_cast<const zd *>(dB), ldb, zd{beta.re, beta.im},
                             reinterpret_cast<zd *>(dC), ldc);
+}
+
+// ---- single precision: float and complex<float> ---------------------------------------
+clm_expm_plan clm_expm_plan_s(double norm1) {
+  return clm::taylor_plan(norm1, clm::kTaylorThetaS);
+}
+double clm_pade_theta_s(int m) { return clm::pade_theta_single(m); }
+clm_expm_plan clm_pade_plan_s(float norm1) { return clm::pade_plan_single(norm1); }
+
+clm_status_t clm_expm_buffer_size_s(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_buffer_size<float>(n);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_buffer_size_c(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_buffer_size<zc>(n);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_s(clm_ctx *ctx, clm_stream_t stream, int n, const float *dA, int lda,
+                        float *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                        clm_expm_plan *plan) {
+  return clm::expm_impl<float>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
+                               lwork_bytes, dInfo, plan);
+}
+
+clm_status_t clm_expm_c(clm_ctx *ctx, clm_stream_t stream, int n, const clm_float_complex *dA,
+                        int lda, clm_float_complex *dE, int lde, void *dWork, size_t lwork_bytes,
+                        int *dInfo, clm_expm_plan *plan) {
+  return clm::expm_impl<zc>(ctx, static_cast<hipStream_t>(stream), n,
+                            reinterpret_cast<const zc *>(dA), lda, reinterpret_cast<zc *>(dE), lde,
+                            dWork, lwork_bytes, dInfo, plan);
+}
+
+clm_status_t clm_matrix_norm1_s(clm_stream_t stream, int n, const float *dA, int lda,
+                                float *dColsum, float *norm) {
+  return clm::norm1_impl<float>(stream, n, dA, lda, dColsum, norm);
+}
+
+clm_status_t clm_matrix_norm1_c(clm_stream_t stream, int n, const clm_float_complex *dA, int lda,
+                                float *dColsum, float *norm) {
+  return clm::norm1_impl<zc>(stream, n, reinterpret_cast<const zc *>(dA), lda, dColsum, norm);
+}
+
+clm_status_t clm_lange_s(clm_stream_t stream, int which, size_t m, size_t n, const float *dA,
+                         size_t lda, float *dResult, float *dScratch) {
+  return clm::lange_impl<float>(stream, which, m, n, dA, lda, dResult, dScratch);
+}
+
+clm_status_t clm_lange_c(clm_stream_t stream, int which, size_t m, size_t n,
+                         const clm_float_complex *dA, size_t lda, float *dResult,
+                         float *dScratch) {
+  return clm::lange_impl<zc>(stream, which, m, n, reinterpret_cast<const zc *>(dA), lda, dResult,
+                             dScratch);
+}
+
+clm_status_t clm_lascl_s(clm_stream_t stream, float cfrom, float cto, size_t m, size_t n,
+                         float *dA, size_t lda) {
+  return clm::lascl_impl<float>(stream, cfrom, cto, m, n, dA, lda);
+}
+
+clm_status_t clm_lascl_c(clm_stream_t stream, float cfrom, float cto, size_t m, size_t n,
+                         clm_float_complex *dA, size_t lda) {
+  return clm::lascl_impl<zc>(stream, cfrom, cto, m, n, reinterpret_cast<zc *>(dA), lda);
+}
+
+size_t clm_paterson_stockmeyer_power_stride_s(int n) { return clm::power_stride<float>(n); }
+size_t clm_paterson_stockmeyer_power_stride_c(int n) { return clm::power_stride<zc>(n); }
+size_t clm_paterson_stockmeyer_buffer_size_s(int n, int degree, int s_requested) {
+  return clm::ps_buffer_size<float>(n, degree, s_requested);
+}
+size_t clm_paterson_stockmeyer_buffer_size_c(int n, int degree, int s_requested) {
+  return clm::ps_buffer_size<zc>(n, degree, s_requested);
+}
+
+clm_status_t clm_paterson_stockmeyer_s(clm_ctx *ctx, int n, const float *dCoeffs, int degree,
+                                       const float *dA, int lda, float *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested) {
+  return clm::ps_impl<float>(ctx, n, dCoeffs, degree, dA, lda, dP, ldp, dWork, work_bytes,
+                             s_requested);
+}
+
+clm_status_t clm_paterson_stockmeyer_c(clm_ctx *ctx, int n, const clm_float_complex *dCoeffs,
+                                       int degree, const clm_float_complex *dA, int lda,
+                                       clm_float_complex *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested) {
+  return clm::ps_impl<zc>(ctx, n, reinterpret_cast<const zc *>(dCoeffs), degree,
+                          reinterpret_cast<const zc *>(dA), lda, reinterpret_cast<zc *>(dP), ldp,
+                          dWork, work_bytes, s_requested);
+}
+
+size_t clm_horner_buffer_size_s(int n) { return clm::horner_buffer_size<float>(n); }
+size_t clm_horner_buffer_size_c(int n) { return clm::horner_buffer_size<zc>(n); }
+
+clm_status_t clm_horner_s(clm_ctx *ctx, int n, const float *dCoeffs, int degree, const float *dA,
+                          int lda, float *dP, int ldp, void *dWork, size_t work_bytes) {
+  return clm::horner_impl<float>(ctx, n, dCoeffs, degree, dA, lda, dP, ldp, dWork, work_bytes);
+}
+
+clm_status_t clm_horner_c(clm_ctx *ctx, int n, const clm_float_complex *dCoeffs, int degree,
+                          const clm_float_complex *dA, int lda, clm_float_complex *dP, int ldp,
+                          void *dWork, size_t work_bytes) {
+  return clm::horner_impl<zc>(ctx, n, reinterpret_cast<const zc *>(dCoeffs), degree,
+                              reinterpret_cast<const zc *>(dA), lda, reinterpret_cast<zc *>(dP),
+                              ldp, dWork, work_bytes);
+}
+
+clm_status_t clm_gemm_s(clm_ctx *ctx, int m, int n, int k, float alpha, const float *dA, int lda,
+                        const float *dB, int ldb, float beta, float *dC, int ldc) {
+  return clm::gemm_impl<float>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+}
+
+clm_status_t clm_gemm_c(clm_ctx *ctx, int m, int n, int k, clm_float_complex alpha,
+                        const clm_float_complex *dA, int lda, const clm_float_complex *dB,
+                        int ldb, clm_float_complex beta, clm_float_complex *dC, int ldc) {
+  return clm::gemm_impl<zc>(ctx, m, n, k, zc{alpha.re, alpha.im}, reinterpret_cast<const zc *>(dA),
+                            lda, reinterpret_cast<const zc *>(dB), ldb, zc{beta.re, beta.im},
+                            reinterpret_cast<zc *>(dC), ldc);
 }
 
 }  // extern "C"

@@ -22,6 +22,11 @@ struct alignas(16) zd {
 
 __host__ __device__ inline zd make_zd(double r, double i) { return zd{r, i}; }
 
+// Interleaved float complex (complex64), one 8-byte access.
+struct alignas(8) zc {
+  float re, im;
+};
+
 // ---- element arithmetic (real and complex share the kernels' templates) ----
 template <typename T>
 struct ops;
@@ -40,6 +45,35 @@ struct ops<double> {
 };
 
 template <>
+struct ops<float> {
+  using real = float;
+  __host__ __device__ static float zero() { return 0.f; }
+  __host__ __device__ static float from_real(double r) { return static_cast<float>(r); }
+  __host__ __device__ static float add(float a, float b) { return a + b; }
+  __host__ __device__ static float mul(float a, float b) { return a * b; }
+  __host__ __device__ static float mad(float c, float x, float acc) { return c * x + acc; }
+  __host__ __device__ static float scale(float x, double r) { return x * static_cast<float>(r); }
+  __host__ __device__ static bool is_zero(float a) { return a == 0.f; }
+};
+
+template <>
+struct ops<zc> {
+  using real = float;
+  __host__ __device__ static zc zero() { return zc{0.f, 0.f}; }
+  __host__ __device__ static zc from_real(double r) { return zc{static_cast<float>(r), 0.f}; }
+  __host__ __device__ static zc add(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
+  __host__ __device__ static zc mul(zc a, zc b) {
+    return zc{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  }
+  __host__ __device__ static zc mad(zc c, zc x, zc acc) { return add(mul(c, x), acc); }
+  __host__ __device__ static zc scale(zc x, double r) {
+    const float f = static_cast<float>(r);
+    return zc{x.re * f, x.im * f};
+  }
+  __host__ __device__ static bool is_zero(zc a) { return a.re == 0.f && a.im == 0.f; }
+};
+
+template <>
 struct ops<zd> {
   using real = double;
   __host__ __device__ static zd zero() { return zd{0.0, 0.0}; }
@@ -52,6 +86,9 @@ struct ops<zd> {
   __host__ __device__ static zd scale(zd x, double r) { return zd{x.re * r, x.im * r}; }
   __host__ __device__ static bool is_zero(zd a) { return a.re == 0.0 && a.im == 0.0; }
 };
+
+template <typename T>
+using real_t = typename ops<T>::real;
 
 // ---- workspace carve -----------------------------------------------------------
 // One caller-owned buffer carved into 256-byte aligned regions; a null base
@@ -123,11 +160,11 @@ hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);
 // ---- element-wise and reduction kernels ------------------------------------------------
 // 1-norm in two launches (colsums, NaN-propagating max into dColsum[0]).
 template <typename T>
-hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, double *colsum);
+hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum);
 // lange: which in CLM_NORM_*; result in *d_result.
 template <typename T>
 hipError_t launch_lange(hipStream_t stream, int which, size_t m, size_t n, const T *A, size_t lda,
-                        double *d_result, double *d_scratch);
+                        real_t<T> *d_result, real_t<T> *d_scratch);
 // out = f * in (n x n), f a real factor
 template <typename T>
 hipError_t launch_scale_copy(hipStream_t stream, int n, double f, const T *in, long ldi, T *out,
@@ -152,5 +189,9 @@ hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v);
 // Device tables of 1/k!, k = 0..32 (correctly rounded), for the Taylor coefficients.
 const double *inv_factorials_d(int device);
 const zd *inv_factorials_z(int device);
+const float *inv_factorials_s(int device);
+const zc *inv_factorials_c(int device);
+template <typename T>
+const T *inv_factorials(int device);
 
 }  // namespace clm

@@ -28,10 +28,15 @@ constexpr int kEW = 256;  // element-wise block
 
 __device__ __forceinline__ double modulus(double x) { return fabs(x); }
 __device__ __forceinline__ double modulus(zd x) { return hypot(x.re, x.im); }
+__device__ __forceinline__ float modulus(float x) { return fabsf(x); }
+__device__ __forceinline__ float modulus(zc x) { return hypotf(x.re, x.im); }
 __device__ __forceinline__ double square(double x) { return x * x; }
 __device__ __forceinline__ double square(zd x) { return x.re * x.re + x.im * x.im; }
+__device__ __forceinline__ float square(float x) { return x * x; }
+__device__ __forceinline__ float square(zc x) { return x.re * x.re + x.im * x.im; }
 
-__device__ __forceinline__ double nan_max(double a, double b) {
+template <typename R>
+__device__ __forceinline__ R nan_max(R a, R b) {
   if (a != a) return a;
   if (b != b) return b;
   return a < b ? b : a;
@@ -40,16 +45,18 @@ __device__ __forceinline__ double nan_max(double a, double b) {
 // pre: 0 |x|, 1 |x|^2; op: 0 plus, 1 nan_max. One block per column.
 template <typename T, int PRE, int OP>
 __global__ void __launch_bounds__(kRB)
-    reduce_columns_kernel(const T *__restrict__ A, double *__restrict__ out, size_t m, size_t lda) {
-  __shared__ double sdata[kRB];
+    reduce_columns_kernel(const T *__restrict__ A, real_t<T> *__restrict__ out, size_t m,
+                          size_t lda) {
+  using R = real_t<T>;
+  __shared__ R sdata[kRB];
   const size_t col = blockIdx.x;
   const unsigned tid = threadIdx.x;
   const T *column = A + col * lda;
   const unsigned nactive = (m < kRB) ? static_cast<unsigned>(m) : kRB;
   if (tid < nactive) {
-    double acc = PRE == 0 ? modulus(column[tid]) : square(column[tid]);
+    R acc = PRE == 0 ? modulus(column[tid]) : square(column[tid]);
     for (size_t i = tid + kRB; i < m; i += kRB) {
-      const double x = PRE == 0 ? modulus(column[i]) : square(column[i]);
+      const R x = PRE == 0 ? modulus(column[i]) : square(column[i]);
       acc = OP == 0 ? acc + x : nan_max(acc, x);
     }
     sdata[tid] = acc;
@@ -64,19 +71,20 @@ __global__ void __launch_bounds__(kRB)
 }
 
 // max_reduce (src/expm/expm.cu:212-245): best seeded at 0, a NaN wins and stays.
-__global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, double *values) {
-  __shared__ double sdata[kRB];
+template <typename R>
+__global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, R *values) {
+  __shared__ R sdata[kRB];
   const int tid = threadIdx.x;
-  double best = 0.0;
+  R best = 0;
   for (int i = tid; i < n; i += kRB) {
-    const double v = values[i];
+    const R v = values[i];
     if (v > best || isnan(v)) best = v;
   }
   sdata[tid] = best;
   __syncthreads();
   for (int s = kRB / 2; s > 0; s >>= 1) {
     if (tid < s) {
-      const double other = sdata[tid + s];
+      const R other = sdata[tid + s];
       if (other > sdata[tid] || isnan(other)) sdata[tid] = other;
     }
     __syncthreads();
@@ -84,15 +92,15 @@ __global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, double *values) 
   if (tid == 0) values[0] = sdata[0];
 }
 
-// single-column identity-free fold of n doubles into out[0] (lange stage 2)
-template <int OP>
-__global__ void __launch_bounds__(kRB) fold_vector_kernel(const double *v, size_t n, double *out,
+// single-column identity-free fold of n values into out[0] (lange stage 2)
+template <typename R, int OP>
+__global__ void __launch_bounds__(kRB) fold_vector_kernel(const R *v, size_t n, R *out,
                                                           int take_sqrt) {
-  __shared__ double sdata[kRB];
+  __shared__ R sdata[kRB];
   const unsigned tid = threadIdx.x;
   const unsigned nactive = (n < kRB) ? static_cast<unsigned>(n) : kRB;
   if (tid < nactive) {
-    double acc = v[tid];
+    R acc = v[tid];
     for (size_t i = tid + kRB; i < n; i += kRB) acc = OP == 0 ? acc + v[i] : nan_max(acc, v[i]);
     sdata[tid] = acc;
   }
@@ -108,17 +116,18 @@ __global__ void __launch_bounds__(kRB) fold_vector_kernel(const double *v, size_
 // lange 'I': one lane per row, j ascending, seed 0 (src/lapack/lange/lange.cu:108-119)
 template <typename T>
 __global__ void __launch_bounds__(kEW)
-    row_abssum_kernel(const T *__restrict__ A, double *__restrict__ rows, size_t m, size_t n,
+    row_abssum_kernel(const T *__restrict__ A, real_t<T> *__restrict__ rows, size_t m, size_t n,
                       size_t lda) {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
-    double acc = 0.0;
+    real_t<T> acc = 0;
     for (size_t j = 0; j < n; ++j) acc += modulus(A[i + j * lda]);
     rows[i] = acc;
   }
 }
 
-__global__ void zero_scalar_kernel(double *d) { d[0] = 0.0; }
+template <typename R>
+__global__ void zero_scalar_kernel(R *d) { d[0] = 0; }
 
 template <typename T>
 __global__ void __launch_bounds__(kEW)
@@ -180,46 +189,47 @@ inline dim3 col_grid(size_t rows, size_t cols) {
 }  // namespace
 
 template <typename T>
-hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, double *colsum) {
+hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum) {
   if (n < 1) return hipSuccess;
   hipLaunchKernelGGL((reduce_columns_kernel<T, 0, 0>), dim3(n), dim3(kRB), 0, stream, A, colsum,
                      static_cast<size_t>(n), static_cast<size_t>(lda));
-  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(kRB), 0, stream, n, colsum);
+  hipLaunchKernelGGL(max_reduce_kernel<real_t<T>>, dim3(1), dim3(kRB), 0, stream, n, colsum);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_lange(hipStream_t stream, int which, size_t m, size_t n, const T *A, size_t lda,
-                        double *d_result, double *d_scratch) {
+                        real_t<T> *d_result, real_t<T> *d_scratch) {
+  using R = real_t<T>;
   if (m == 0 || n == 0) {
-    hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, stream, d_result);
+    hipLaunchKernelGGL(zero_scalar_kernel<R>, dim3(1), dim3(1), 0, stream, d_result);
     return hipGetLastError();
   }
   switch (which) {
   case CLM_NORM_MAX_ABS:
     hipLaunchKernelGGL((reduce_columns_kernel<T, 0, 1>), dim3(n), dim3(kRB), 0, stream, A,
                        d_scratch, m, lda);
-    hipLaunchKernelGGL(fold_vector_kernel<1>, dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
+    hipLaunchKernelGGL((fold_vector_kernel<R, 1>), dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
                        0);
     break;
   case CLM_NORM_ONE:
     hipLaunchKernelGGL((reduce_columns_kernel<T, 0, 0>), dim3(n), dim3(kRB), 0, stream, A,
                        d_scratch, m, lda);
-    hipLaunchKernelGGL(fold_vector_kernel<1>, dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
+    hipLaunchKernelGGL((fold_vector_kernel<R, 1>), dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
                        0);
     break;
   case CLM_NORM_INF: {
     const unsigned blocks = static_cast<unsigned>((m + kEW - 1) / kEW);
     hipLaunchKernelGGL(row_abssum_kernel<T>, dim3(blocks), dim3(kEW), 0, stream, A, d_scratch, m,
                        n, lda);
-    hipLaunchKernelGGL(fold_vector_kernel<1>, dim3(1), dim3(kRB), 0, stream, d_scratch, m, d_result,
+    hipLaunchKernelGGL((fold_vector_kernel<R, 1>), dim3(1), dim3(kRB), 0, stream, d_scratch, m, d_result,
                        0);
     break;
   }
   default:  // frobenius
     hipLaunchKernelGGL((reduce_columns_kernel<T, 1, 0>), dim3(n), dim3(kRB), 0, stream, A,
                        d_scratch, m, lda);
-    hipLaunchKernelGGL(fold_vector_kernel<0>, dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
+    hipLaunchKernelGGL((fold_vector_kernel<R, 0>), dim3(1), dim3(kRB), 0, stream, d_scratch, n, d_result,
                        1);
     break;
   }
@@ -274,9 +284,9 @@ hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v) {
 }
 
 #define CLM_INSTANTIATE(T)                                                                         \
-  template hipError_t launch_norm1<T>(hipStream_t, int, const T *, long, double *);                \
+  template hipError_t launch_norm1<T>(hipStream_t, int, const T *, long, real_t<T> *);             \
   template hipError_t launch_lange<T>(hipStream_t, int, size_t, size_t, const T *, size_t,          \
-                                      double *, double *);                                         \
+                                      real_t<T> *, real_t<T> *);                                   \
   template hipError_t launch_scale_copy<T>(hipStream_t, int, double, const T *, long, T *, long);  \
   template hipError_t launch_lascl<T>(hipStream_t, size_t, size_t, double, T *, size_t);           \
   template hipError_t launch_build_block<T>(hipStream_t, int, T *, long, const T *, int, const T *, \
@@ -286,6 +296,8 @@ hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v) {
 
 CLM_INSTANTIATE(double)
 CLM_INSTANTIATE(zd)
+CLM_INSTANTIATE(float)
+CLM_INSTANTIATE(zc)
 #undef CLM_INSTANTIATE
 
 // ---- 1/k! tables ------------------------------------------------------------------------
@@ -305,20 +317,40 @@ CLM_INSTANTIATE(zd)
       X(0x1.3932c5047d60ep-108), X(0x1.434d2e783f5bcp-113), X(0x1.434d2e783f5bcp-118)
 #define CLM_AS_REAL(v) v
 #define CLM_AS_COMPLEX(v) zd{v, 0.0}
+// the same values correctly rounded to FP32 (float(Fraction(1, k!)))
+#define CLM_INV_FACT_F(X)                                                                  \
+  X(0x1.000000p+0f), X(0x1.000000p+0f), X(0x1.000000p-1f), X(0x1.555556p-3f),                   \
+      X(0x1.555556p-5f), X(0x1.111112p-7f), X(0x1.6c16c2p-10f), X(0x1.a01a02p-13f),             \
+      X(0x1.a01a02p-16f), X(0x1.71de3ap-19f), X(0x1.27e4fcp-22f), X(0x1.ae6456p-26f),           \
+      X(0x1.1eed8ep-29f), X(0x1.612462p-33f), X(0x1.93974ap-37f), X(0x1.ae7f3ep-41f),           \
+      X(0x1.ae7f3ep-45f), X(0x1.952c78p-49f), X(0x1.682786p-53f), X(0x1.2f49b4p-57f),           \
+      X(0x1.e542bap-62f), X(0x1.71b8f0p-66f), X(0x1.0ce396p-70f), X(0x1.761b42p-75f),           \
+      X(0x1.f2cf02p-80f), X(0x1.3f3ccep-84f), X(0x1.88e860p-89f), X(0x1.d1ab1cp-94f),           \
+      X(0x1.0a18a2p-98f), X(0x1.259f98p-103f), X(0x1.3932c6p-108f), X(0x1.434d2ep-113f),        \
+      X(0x1.434d2ep-118f)
+#define CLM_AS_COMPLEX_F(v) zc{v, 0.f}
 // extern linkage: hipGetSymbolAddress resolves a __device__ variable by its
 // exported name, which a namespace-scope const (internal linkage) does not have.
 extern __device__ const double kInvFactD[33];
 extern __device__ const zd kInvFactZ[33];
+extern __device__ const float kInvFactS[33];
+extern __device__ const zc kInvFactC[33];
 __device__ const double kInvFactD[33] = {CLM_INV_FACT(CLM_AS_REAL)};
 __device__ const zd kInvFactZ[33] = {CLM_INV_FACT(CLM_AS_COMPLEX)};
+__device__ const float kInvFactS[33] = {CLM_INV_FACT_F(CLM_AS_REAL)};
+__device__ const zc kInvFactC[33] = {CLM_INV_FACT_F(CLM_AS_COMPLEX_F)};
 #undef CLM_AS_REAL
 #undef CLM_AS_COMPLEX
+#undef CLM_AS_COMPLEX_F
 #undef CLM_INV_FACT
+#undef CLM_INV_FACT_F
 
 namespace {
 struct DeviceTables {
   const double *d = nullptr;
   const zd *z = nullptr;
+  const float *s = nullptr;
+  const zc *c = nullptr;
 };
 DeviceTables g_tables[64];
 std::mutex g_tables_mutex;
@@ -329,11 +361,15 @@ const DeviceTables &tables(int device) {
   std::lock_guard<std::mutex> lock(g_tables_mutex);
   DeviceTables &t = g_tables[device & 63];
   if (!t.d) {
-    void *pd = nullptr, *pz = nullptr;
+    void *pd = nullptr, *pz = nullptr, *ps = nullptr, *pc = nullptr;
     if (hipGetSymbolAddress(&pd, HIP_SYMBOL(kInvFactD)) != hipSuccess) return t;
     if (hipGetSymbolAddress(&pz, HIP_SYMBOL(kInvFactZ)) != hipSuccess) return t;
+    if (hipGetSymbolAddress(&ps, HIP_SYMBOL(kInvFactS)) != hipSuccess) return t;
+    if (hipGetSymbolAddress(&pc, HIP_SYMBOL(kInvFactC)) != hipSuccess) return t;
     t.z = static_cast<const zd *>(pz);
-    t.d = static_cast<const double *>(pd);
+    t.s = static_cast<const float *>(ps);
+    t.c = static_cast<const zc *>(pc);
+    t.d = static_cast<const double *>(pd);  // set last: it marks the entry complete
   }
   return t;
 }
@@ -341,5 +377,24 @@ const DeviceTables &tables(int device) {
 
 const double *inv_factorials_d(int device) { return tables(device).d; }
 const zd *inv_factorials_z(int device) { return tables(device).z; }
+const float *inv_factorials_s(int device) { return tables(device).s; }
+const zc *inv_factorials_c(int device) { return tables(device).c; }
+
+template <>
+const double *inv_factorials<double>(int device) {
+  return inv_factorials_d(device);
+}
+template <>
+const zd *inv_factorials<zd>(int device) {
+  return inv_factorials_z(device);
+}
+template <>
+const float *inv_factorials<float>(int device) {
+  return inv_factorials_s(device);
+}
+template <>
+const zc *inv_factorials<zc>(int device) {
+  return inv_factorials_c(device);
+}
 
 }  // namespace clm

@@ -31,14 +31,25 @@ from ._lib import CalamanError, ExpmPlan, check, lib
 
 _F64 = torch.float64
 _C128 = torch.complex128
+_F32 = torch.float32
+_C64 = torch.complex64
+_SUFFIX = {_F32: "s", _F64: "d", _C64: "c", _C128: "z"}
 
 
 def _suffix(dtype: torch.dtype) -> str:
-    if dtype == _F64:
-        return "d"
-    if dtype == _C128:
-        return "z"
-    raise TypeError(f"unsupported dtype {dtype}; float64 and complex128 are supported")
+    try:
+        return _SUFFIX[dtype]
+    except KeyError:
+        raise TypeError(f"unsupported dtype {dtype}; float32, float64, complex64 and complex128 "
+                        "are supported") from None
+
+
+def _real_dtype(dtype: torch.dtype) -> torch.dtype:
+    return _F32 if dtype in (_F32, _C64) else _F64
+
+
+def _is_single(dtype: torch.dtype) -> bool:
+    return dtype in (_F32, _C64)
 
 
 def _stream_ptr(stream: Optional[torch.cuda.Stream]) -> int:
@@ -69,22 +80,28 @@ def from_cm(buf: torch.Tensor, n_rows: int, n_cols: int, ld: Optional[int] = Non
 
 
 # ---- plans and sizes ------------------------------------------------------------------
-def expm_plan(norm1: float) -> dict:
-    """The plan expm() executes for a given 1-norm (the role of expm_plan<T>)."""
-    return lib().clm_expm_plan_d(float(norm1)).as_dict()
+def expm_plan(norm1: float, dtype: torch.dtype = _F64) -> dict:
+    """The plan expm() executes for a given 1-norm (the role of expm_plan<T>);
+    float32/complex64 use the thresholds at u = 2^-24."""
+    f = lib().clm_expm_plan_s if _is_single(dtype) else lib().clm_expm_plan_d
+    return f(float(norm1)).as_dict()
 
 
-def taylor_theta(m: int) -> float:
-    return float(lib().clm_taylor_theta_d(int(m)))
+def taylor_theta(m: int, single: bool = False) -> float:
+    f = lib().clm_taylor_theta_s if single else lib().clm_taylor_theta_d
+    return float(f(int(m)))
 
 
-def pade_theta(m: int) -> float:
-    """pade_theta<double> (src/expm/plan.cppm:246-281)."""
-    return float(lib().clm_pade_theta_d(int(m)))
+def pade_theta(m: int, single: bool = False) -> float:
+    """pade_theta<double> / pade_theta<float> (src/expm/plan.cppm:51-85)."""
+    f = lib().clm_pade_theta_s if single else lib().clm_pade_theta_d
+    return float(f(int(m)))
 
 
-def pade_plan(norm1: float) -> dict:
-    """The reference's own expm_plan<double> (src/expm/plan.cppm:308-326)."""
+def pade_plan(norm1: float, single: bool = False) -> dict:
+    """The reference's own expm_plan<double> / <float> (src/expm/plan.cppm:112-130)."""
+    if single:
+        return lib().clm_pade_plan_s(float(norm1)).as_dict()
     return lib().clm_pade_plan_d(float(norm1)).as_dict()
 
 
@@ -164,7 +181,7 @@ def _ctx_for(stream: Optional[torch.cuda.Stream]) -> Context:
 def expm_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int, dWork: torch.Tensor,
             dInfo: Optional[torch.Tensor] = None, stream: Optional[torch.cuda.Stream] = None,
             ctx: Optional[Context] = None) -> tuple:
-    """clm_expm_{d,z}; returns (status, plan dict)."""
+    """clm_expm_{s,d,c,z}; returns (status, plan dict)."""
     plan = ExpmPlan()
     f = getattr(lib(), f"clm_expm_{_suffix(dA.dtype)}")
     st = f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA),
@@ -200,7 +217,7 @@ def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1) 
 
 def matrix_norm1_cm(n: int, dA: torch.Tensor, lda: int, dColsum: torch.Tensor,
                     stream: Optional[torch.cuda.Stream] = None) -> float:
-    out = ctypes.c_double(0.0)
+    out = ctypes.c_float(0.0) if _is_single(dA.dtype) else ctypes.c_double(0.0)
     f = getattr(lib(), f"clm_matrix_norm1_{_suffix(dA.dtype)}")
     check(f(ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA), int(lda), _ptr(dColsum),
             ctypes.byref(out)), "matrix_norm1")
@@ -243,19 +260,21 @@ def gemm_cm(ctx: Context, m: int, n: int, k: int, alpha, dA: torch.Tensor, lda: 
     if split:
         return int(lib().clm_gemm_split_d(ctx.handle, int(split), m, n, k, float(alpha), _ptr(dA),
                                           lda, _ptr(dB), ldb, float(beta), _ptr(dC), ldc))
-    if dA.dtype == _C128:
-        a = _lib.DoubleComplex(complex(alpha).real, complex(alpha).imag)
-        b = _lib.DoubleComplex(complex(beta).real, complex(beta).imag)
-        return int(lib().clm_gemm_z(ctx.handle, m, n, k, a, _ptr(dA), lda, _ptr(dB), ldb, b,
-                                    _ptr(dC), ldc))
-    return int(lib().clm_gemm_d(ctx.handle, m, n, k, float(alpha), _ptr(dA), lda, _ptr(dB), ldb,
-                                float(beta), _ptr(dC), ldc))
+    sfx = _suffix(dA.dtype)
+    if dA.dtype in (_C128, _C64):
+        ct = _lib.DoubleComplex if dA.dtype == _C128 else _lib.FloatComplex
+        a = ct(complex(alpha).real, complex(alpha).imag)
+        b = ct(complex(beta).real, complex(beta).imag)
+    else:
+        a, b = float(alpha), float(beta)
+    return int(getattr(lib(), f"clm_gemm_{sfx}")(ctx.handle, m, n, k, a, _ptr(dA), lda, _ptr(dB),
+                                                  ldb, b, _ptr(dC), ldc))
 
 
 # ---- convenience layer (row-major torch matrices) ----------------------------------------------
 def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
          return_plan: bool = False):
-    """exp(a) for a square float64/complex128 device matrix."""
+    """exp(a) for a square float32/float64/complex64/complex128 device matrix."""
     _require_device(a, "a")
     n = a.shape[-1]
     if a.dim() != 2 or a.shape[0] != n:
@@ -290,7 +309,7 @@ def matrix_norm1(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) ->
     _require_device(a, "a")
     n = a.shape[0]
     acm = to_cm(a)
-    colsum = torch.empty(n, dtype=_F64, device=a.device)
+    colsum = torch.empty(n, dtype=_real_dtype(a.dtype), device=a.device)
     return matrix_norm1_cm(n, acm, n, colsum, stream)
 
 
@@ -299,8 +318,8 @@ def lange(which: str, a: torch.Tensor, stream: Optional[torch.cuda.Stream] = Non
     code = {"max_abs": 0, "one": 1, "inf": 2, "frobenius": 3}[which]
     m, n = a.shape
     acm = to_cm(a)
-    out = torch.empty(1, dtype=_F64, device=a.device)
-    scratch = torch.empty(max(m, n, 1), dtype=_F64, device=a.device)
+    out = torch.empty(1, dtype=_real_dtype(a.dtype), device=a.device)
+    scratch = torch.empty(max(m, n, 1), dtype=_real_dtype(a.dtype), device=a.device)
     check(lange_cm(code, m, n, acm, max(m, 1), out, scratch, stream), "lange")
     return out[0]
 

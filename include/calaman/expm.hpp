@@ -9,9 +9,9 @@
 // src/paterson_stockmeyer/interface.cppm:329-332, src/horner/interface.cppm:156-159,
 // src/lapack/lange/interface.cppm:82-85, src/lapack/lascl/interface.cppm:79-82).
 // The two vendor handles become clm_ctx* (nullptr = the library's default
-// context for the current device). Element types: double and
-// std::complex<double> (hipDoubleComplex converts implicitly through the same
-// layout).
+// context for the current device). Element types: float, double,
+// std::complex<float>, std::complex<double>, hipFloatComplex and
+// hipDoubleComplex (the complex types share one interleaved layout).
 #pragma once
 
 #include <hip/hip_complex.h>
@@ -41,7 +41,7 @@ struct Status {
 
 using Handle = clm_ctx *;
 
-// ExpmPlan (src/expm/plan.cppm:290-294); m is the Taylor degree here.
+// ExpmPlan (src/expm/plan.cppm:94-98); m is the Taylor degree here.
 struct ExpmPlan {
   int m = 0;
   int s = 0;
@@ -56,23 +56,54 @@ struct PatersonStockmeyerPlan {
 };
 
 namespace detail {
+// Per element type: the C entry points of its precision and its real type.
+#define CALAMAN_ABI(T, E, R, SFX, CPLX)                                                           \
+  template <>                                                                                    \
+  struct abi<T> {                                                                                \
+    using elem = E;                                                                              \
+    using real = R;                                                                              \
+    static constexpr bool complex = CPLX;                                                        \
+    static clm_status_t buffer_size(int n, std::size_t *b) { return clm_expm_buffer_size_##SFX(n, b); } \
+    static clm_status_t expm(clm_ctx *h, hipStream_t st, int n, const E *a, int lda, E *e,       \
+                             int lde, void *w, std::size_t wb, int *info, clm_expm_plan *p) {    \
+      return clm_expm_##SFX(h, st, n, a, lda, e, lde, w, wb, info, p);                           \
+    }                                                                                            \
+    static clm_status_t norm1(hipStream_t st, int n, const E *a, int lda, R *cs, R *out) {      \
+      return clm_matrix_norm1_##SFX(st, n, a, lda, cs, out);                                     \
+    }                                                                                            \
+    static clm_status_t lange(hipStream_t st, int w, std::size_t m, std::size_t n, const E *a,  \
+                              std::size_t lda, R *res, R *scr) {                                 \
+      return clm_lange_##SFX(st, w, m, n, a, lda, res, scr);                                     \
+    }                                                                                            \
+    static clm_status_t lascl(hipStream_t st, R cf, R ct, std::size_t m, std::size_t n, E *a,   \
+                              std::size_t lda) {                                                 \
+      return clm_lascl_##SFX(st, cf, ct, m, n, a, lda);                                          \
+    }                                                                                            \
+    static std::size_t ps_stride(int n) { return clm_paterson_stockmeyer_power_stride_##SFX(n); } \
+    static std::size_t ps_size(int n, int d, int s) {                                            \
+      return clm_paterson_stockmeyer_buffer_size_##SFX(n, d, s);                                 \
+    }                                                                                            \
+    static clm_status_t ps(clm_ctx *h, int n, const E *c, int d, const E *a, int lda, E *p,     \
+                           int ldp, void *w, std::size_t wb, int s) {                            \
+      return clm_paterson_stockmeyer_##SFX(h, n, c, d, a, lda, p, ldp, w, wb, s);                \
+    }                                                                                            \
+    static std::size_t horner_size(int n) { return clm_horner_buffer_size_##SFX(n); }           \
+    static clm_status_t horner(clm_ctx *h, int n, const E *c, int d, const E *a, int lda, E *p, \
+                               int ldp, void *w, std::size_t wb) {                               \
+      return clm_horner_##SFX(h, n, c, d, a, lda, p, ldp, w, wb);                                \
+    }                                                                                            \
+  };
+
 template <typename T>
 struct abi;
-template <>
-struct abi<double> {
-  using elem = double;
-  static constexpr bool complex = false;
-};
-template <>
-struct abi<std::complex<double>> {
-  using elem = clm_double_complex;
-  static constexpr bool complex = true;
-};
-template <>
-struct abi<hipDoubleComplex> {
-  using elem = clm_double_complex;
-  static constexpr bool complex = true;
-};
+CALAMAN_ABI(float, float, float, s, false)
+CALAMAN_ABI(double, double, double, d, false)
+CALAMAN_ABI(std::complex<float>, clm_float_complex, float, c, true)
+CALAMAN_ABI(std::complex<double>, clm_double_complex, double, z, true)
+CALAMAN_ABI(hipFloatComplex, clm_float_complex, float, c, true)
+CALAMAN_ABI(hipDoubleComplex, clm_double_complex, double, z, true)
+#undef CALAMAN_ABI
+
 template <typename T>
 using elem_t = typename abi<T>::elem;
 template <typename T>
@@ -83,33 +114,39 @@ template <typename T>
 inline elem_t<T> *c(T *p) {
   return reinterpret_cast<elem_t<T> *>(p);
 }
+template <typename T>
+constexpr bool single = std::is_same_v<typename abi<T>::real, float>;
 static_assert(sizeof(std::complex<double>) == sizeof(clm_double_complex), "complex layout");
+static_assert(sizeof(std::complex<float>) == sizeof(clm_float_complex), "complex layout");
 }  // namespace detail
+
+// ComplexToRealType<T> (the reference's real type of an element type)
+template <typename T>
+using RealType = typename detail::abi<T>::real;
 
 // ---- plan / thresholds -------------------------------------------------------------
 template <typename T>
 inline ExpmPlan expm_plan(double norm1) {
-  const clm_expm_plan p = clm_expm_plan_d(norm1);
+  const clm_expm_plan p = detail::single<T> ? clm_expm_plan_s(norm1) : clm_expm_plan_d(norm1);
   return ExpmPlan{p.m, p.s, p.num_gemms};
 }
 
 // backward-error threshold of the degree-m Taylor rung (the role of pade_theta<T>)
 template <typename T>
 inline double taylor_theta(int m) {
-  return clm_taylor_theta_d(m);
+  return detail::single<T> ? clm_taylor_theta_s(m) : clm_taylor_theta_d(m);
 }
 
 // the reference's own Pade thresholds, for callers that predict its cost
 template <typename T>
 inline double pade_theta(int m) {
-  return clm_pade_theta_d(m);
+  return detail::single<T> ? clm_pade_theta_s(m) : clm_pade_theta_d(m);
 }
 
 // ---- expm ----------------------------------------------------------------------------------
 template <typename T>
 inline Status expm_bufferSize(Handle /*solver*/, int n, std::size_t *lwork_bytes) {
-  return detail::abi<T>::complex ? clm_expm_buffer_size_z(n, lwork_bytes)
-                                 : clm_expm_buffer_size_d(n, lwork_bytes);
+  return detail::abi<T>::buffer_size(n, lwork_bytes);
 }
 
 template <typename T>
@@ -117,52 +154,41 @@ inline Status expm(Handle blas, Handle /*solver*/, hipStream_t stream, int n, co
                    int lda, T *d_expA, int lde, void *d_work, std::size_t lwork_bytes, int *d_info,
                    ExpmPlan *plan = nullptr) {
   clm_expm_plan p{};
-  clm_status_t st;
-  if constexpr (detail::abi<T>::complex)
-    st = clm_expm_z(blas, stream, n, detail::c(d_A), lda, detail::c(d_expA), lde, d_work,
-                    lwork_bytes, d_info, &p);
-  else
-    st = clm_expm_d(blas, stream, n, d_A, lda, d_expA, lde, d_work, lwork_bytes, d_info, &p);
+  const clm_status_t st = detail::abi<T>::expm(blas, stream, n, detail::c(d_A), lda,
+                                               detail::c(d_expA), lde, d_work, lwork_bytes,
+                                               d_info, &p);
   if (plan && st == CLM_STATUS_SUCCESS) *plan = ExpmPlan{p.m, p.s, p.num_gemms};
   return st;
 }
 
-// matrix_norm1<T> (src/expm/norm1.cppm:417-428): synchronizes, returns on the host.
+// matrix_norm1<T> (src/expm/norm1.cppm:45-56): synchronizes, returns on the host.
 template <typename T>
-inline double matrix_norm1(hipStream_t stream, int n, const T *d_A, int lda, double *d_colsum) {
-  double out = 0.0;
-  if constexpr (detail::abi<T>::complex)
-    clm_matrix_norm1_z(stream, n, detail::c(d_A), lda, d_colsum, &out);
-  else
-    clm_matrix_norm1_d(stream, n, d_A, lda, d_colsum, &out);
+inline RealType<T> matrix_norm1(hipStream_t stream, int n, const T *d_A, int lda,
+                                RealType<T> *d_colsum) {
+  RealType<T> out = 0;
+  static_cast<void>(detail::abi<T>::norm1(stream, n, detail::c(d_A), lda, d_colsum, &out));
   return out;
 }
 
 // lange<T>: like the reference, allocates its per-column scratch on the stream.
 template <typename T>
 inline Status lange(hipStream_t stream, MatrixNorm which, std::size_t m, std::size_t n,
-                    const T *d_A, std::size_t lda, double *d_result) {
-  double *scratch = nullptr;
+                    const T *d_A, std::size_t lda, RealType<T> *d_result) {
+  RealType<T> *scratch = nullptr;
   const std::size_t len = (m > n ? m : n) + 1;
-  if (hipMallocAsync(reinterpret_cast<void **>(&scratch), len * sizeof(double), stream) !=
+  if (hipMallocAsync(reinterpret_cast<void **>(&scratch), len * sizeof(RealType<T>), stream) !=
       hipSuccess)
     return CLM_STATUS_RUNTIME_ERROR;
-  clm_status_t st;
-  if constexpr (detail::abi<T>::complex)
-    st = clm_lange_z(stream, static_cast<int>(which), m, n, detail::c(d_A), lda, d_result, scratch);
-  else
-    st = clm_lange_d(stream, static_cast<int>(which), m, n, d_A, lda, d_result, scratch);
+  const clm_status_t st = detail::abi<T>::lange(stream, static_cast<int>(which), m, n,
+                                                detail::c(d_A), lda, d_result, scratch);
   static_cast<void>(hipFreeAsync(scratch, stream));
   return st;
 }
 
 template <typename T>
-inline Status lascl(hipStream_t stream, double cfrom, double cto, std::size_t m, std::size_t n,
-                    T *d_a, std::size_t lda) {
-  if constexpr (detail::abi<T>::complex)
-    return clm_lascl_z(stream, cfrom, cto, m, n, detail::c(d_a), lda);
-  else
-    return clm_lascl_d(stream, cfrom, cto, m, n, d_a, lda);
+inline Status lascl(hipStream_t stream, RealType<T> cfrom, RealType<T> cto, std::size_t m,
+                    std::size_t n, T *d_a, std::size_t lda) {
+  return detail::abi<T>::lascl(stream, cfrom, cto, m, n, detail::c(d_a), lda);
 }
 
 // ---- Paterson-Stockmeyer / Horner ------------------------------------------------------------------
@@ -176,38 +202,29 @@ inline PatersonStockmeyerPlan paterson_stockmeyer_plan(int degree, int s_request
 }
 template <typename T>
 inline std::size_t paterson_stockmeyer_power_stride(int n) {
-  return detail::abi<T>::complex ? clm_paterson_stockmeyer_power_stride_z(n)
-                                 : clm_paterson_stockmeyer_power_stride_d(n);
+  return detail::abi<T>::ps_stride(n);
 }
 template <typename T>
 inline std::size_t paterson_stockmeyer_bufferSize(int n, int degree, int s_requested = 0) {
-  return detail::abi<T>::complex ? clm_paterson_stockmeyer_buffer_size_z(n, degree, s_requested)
-                                 : clm_paterson_stockmeyer_buffer_size_d(n, degree, s_requested);
+  return detail::abi<T>::ps_size(n, degree, s_requested);
 }
 template <typename T>
 inline Status paterson_stockmeyer(Handle handle, int n, const T *d_coeffs, int degree,
                                   const T *d_A, int lda, T *d_P, int ldp, void *d_work,
                                   std::size_t work_bytes, int s_requested = 0) {
-  if constexpr (detail::abi<T>::complex)
-    return clm_paterson_stockmeyer_z(handle, n, detail::c(d_coeffs), degree, detail::c(d_A), lda,
-                                     detail::c(d_P), ldp, d_work, work_bytes, s_requested);
-  else
-    return clm_paterson_stockmeyer_d(handle, n, d_coeffs, degree, d_A, lda, d_P, ldp, d_work,
-                                     work_bytes, s_requested);
+  return detail::abi<T>::ps(handle, n, detail::c(d_coeffs), degree, detail::c(d_A), lda,
+                            detail::c(d_P), ldp, d_work, work_bytes, s_requested);
 }
 
 template <typename T>
 inline std::size_t horner_bufferSize(int n) {
-  return detail::abi<T>::complex ? clm_horner_buffer_size_z(n) : clm_horner_buffer_size_d(n);
+  return detail::abi<T>::horner_size(n);
 }
 template <typename T>
 inline Status horner(Handle handle, int n, const T *d_coeffs, int degree, const T *d_A, int lda,
                      T *d_P, int ldp, void *d_work, std::size_t work_bytes) {
-  if constexpr (detail::abi<T>::complex)
-    return clm_horner_z(handle, n, detail::c(d_coeffs), degree, detail::c(d_A), lda,
-                        detail::c(d_P), ldp, d_work, work_bytes);
-  else
-    return clm_horner_d(handle, n, d_coeffs, degree, d_A, lda, d_P, ldp, d_work, work_bytes);
+  return detail::abi<T>::horner(handle, n, detail::c(d_coeffs), degree, detail::c(d_A), lda,
+                                detail::c(d_P), ldp, d_work, work_bytes);
 }
 
 }  // namespace calaman

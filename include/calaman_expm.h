@@ -5,8 +5,8 @@
  * names the reference interface it replaces (paths relative to the calaman_G
  * source root). The reference's C++ templates over {float, double,
  * complex<float>, complex<double>} become type-suffixed C functions here
- * (d = double, z = complex<double>); the C++ header calaman/expm.hpp restores
- * the template spelling on top of these.
+ * (s = float, d = double, c = complex<float>, z = complex<double>); the C++
+ * header calaman/expm.hpp restores the template spelling on top of these.
  *
  * Conventions kept from the reference:
  *   - matrices are column-major with a leading dimension >= n; only the n x n
@@ -41,6 +41,9 @@ extern "C" {
 typedef struct clm_double_complex {
   double re, im;
 } clm_double_complex;
+typedef struct clm_float_complex {
+  float re, im;
+} clm_float_complex;
 
 /* A HIP stream (hipStream_t) as an opaque pointer; NULL is the null stream. */
 typedef void *clm_stream_t;
@@ -87,7 +90,9 @@ clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream);
 #define CLM_METHOD_TAYLOR_PS 0 /* Taylor series by Paterson-Stockmeyer (this library) */
 #define CLM_METHOD_PADE 1      /* diagonal Pade + LU (the reference's route; plan only) */
 #define CLM_DTYPE_F64 1
+#define CLM_DTYPE_F32 2
 #define CLM_DTYPE_C128 3
+#define CLM_DTYPE_C64 4
 typedef struct clm_expm_plan {
   int m;         /* degree: Taylor degree, or Pade degree for CLM_METHOD_PADE */
   int s;         /* squarings: the matrix is evaluated at A / 2^s */
@@ -101,23 +106,31 @@ typedef struct clm_expm_plan {
 } clm_expm_plan;
 
 /* The plan clm_expm_d/z execute for a given 1-norm (the role of expm_plan<T>,
- * src/expm/plan.cppm:308-326): over the Taylor degrees
+ * src/expm/plan.cppm:112-130): over the Taylor degrees
  * {1,2,3,5,8,11,15,19,24,29}, minimise Paterson-Stockmeyer products + s with
  * s = max(0, ceil(log2(norm1 / theta_m))), ties to the smaller s. */
 clm_expm_plan clm_expm_plan_d(double norm1);
 /* Backward-error threshold theta_m of the degree-m Taylor approximant at
  * u = 2^-53 (0 for a degree off the ladder); the counterpart of pade_theta<T>
- * (src/expm/plan.cppm:246-281). */
+ * (src/expm/plan.cppm:51-85). */
 double clm_taylor_theta_d(int m);
 /* The reference's own Pade ladder, restated for callers that predict its cost:
- * pade_theta<double> and expm_plan<double> (src/expm/plan.cppm:246-326). */
+ * pade_theta<double> and expm_plan<double> (src/expm/plan.cppm:51-130). */
 double clm_pade_theta_d(int m);
 clm_expm_plan clm_pade_plan_d(double norm1);
+/* Single precision (clm_expm_s/c): the same Taylor ladder with the thresholds at
+ * u = 2^-24 (clm_taylor_theta_s); the reference's float Pade ladder
+ * (pade_theta<float>, expm_plan<float>, src/expm/plan.cppm:51-68,112-130). */
+clm_expm_plan clm_expm_plan_s(double norm1);
+double clm_pade_theta_s(int m);
+clm_expm_plan clm_pade_plan_s(float norm1);
 
 /* ---- expm --------------------------------------------------------------------
- * expm_bufferSize<T> (src/expm/buffer_size.cppm:242-262): bytes of device
+ * expm_bufferSize<T> (src/expm/buffer_size.cppm:77-97): bytes of device
  * workspace clm_expm_* needs for order n (n >= 1). */
+clm_status_t clm_expm_buffer_size_s(int n, size_t *bytes);
 clm_status_t clm_expm_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_buffer_size_c(int n, size_t *bytes);
 clm_status_t clm_expm_buffer_size_z(int n, size_t *bytes);
 
 /* expm<T> (src/expm/expm.cppm:87-174): E = exp(A) by scaling and squaring.
@@ -129,7 +142,15 @@ clm_status_t clm_expm_buffer_size_z(int n, size_t *bytes);
  *   dInfo    device int[2]; the reference's getrf/getrs info. The Taylor
  *            route needs no LU, so both are written 0.
  *   plan     optional: the executed plan
- * n must lie in [1, 46340] (src/expm/detail.cppm:25-28). */
+ * n must lie in [1, 46340] (src/expm/detail.cppm:25-28). The float and
+ * complex<float> forms compute in FP32 (FP32 MFMA products) with the degree
+ * ladder at u = 2^-24. */
+clm_status_t clm_expm_s(clm_ctx *ctx, clm_stream_t stream, int n, const float *dA, int lda,
+                        float *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                        clm_expm_plan *plan);
+clm_status_t clm_expm_c(clm_ctx *ctx, clm_stream_t stream, int n, const clm_float_complex *dA,
+                        int lda, clm_float_complex *dE, int lde, void *dWork, size_t lwork_bytes,
+                        int *dInfo, clm_expm_plan *plan);
 clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
                         double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
                         clm_expm_plan *plan);
@@ -163,9 +184,14 @@ clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const do
                               const clm_split_opts *opts, clm_expm_plan *plan);
 
 /* ---- 1-norm --------------------------------------------------------------------
- * matrix_norm1<T> (src/expm/norm1.cppm:417-428): max_j sum_i |A(i,j)| with the
+ * matrix_norm1<T> (src/expm/norm1.cppm:45-56): max_j sum_i |A(i,j)| with the
  * true complex modulus; NaN propagates. dColsum is device scratch of n
- * doubles. Synchronizes the stream and writes the norm to *norm (host). */
+ * values of the real type (float for s/c, double for d/z), which the sums
+ * accumulate in. Synchronizes the stream and writes the norm to *norm (host). */
+clm_status_t clm_matrix_norm1_s(clm_stream_t stream, int n, const float *dA, int lda,
+                                float *dColsum, float *norm);
+clm_status_t clm_matrix_norm1_c(clm_stream_t stream, int n, const clm_float_complex *dA, int lda,
+                                float *dColsum, float *norm);
 clm_status_t clm_matrix_norm1_d(clm_stream_t stream, int n, const double *dA, int lda,
                                 double *dColsum, double *norm);
 clm_status_t clm_matrix_norm1_z(clm_stream_t stream, int n, const clm_double_complex *dA, int lda,
@@ -176,11 +202,15 @@ clm_status_t clm_matrix_norm1_z(clm_stream_t stream, int n, const clm_double_com
  * matrix. which: CLM_NORM_MAX_ABS, CLM_NORM_ONE, CLM_NORM_INF,
  * CLM_NORM_FROBENIUS (src/common/enums.h:154-159). The reference allocates its
  * per-column scratch on the stream; here the caller passes dScratch of
- * max(m, n) doubles. */
+ * max(m, n) values of the real type. */
 #define CLM_NORM_MAX_ABS 0
 #define CLM_NORM_ONE 1
 #define CLM_NORM_INF 2
 #define CLM_NORM_FROBENIUS 3
+clm_status_t clm_lange_s(clm_stream_t stream, int which, size_t m, size_t n, const float *dA,
+                         size_t lda, float *dResult, float *dScratch);
+clm_status_t clm_lange_c(clm_stream_t stream, int which, size_t m, size_t n,
+                         const clm_float_complex *dA, size_t lda, float *dResult, float *dScratch);
 clm_status_t clm_lange_d(clm_stream_t stream, int which, size_t m, size_t n, const double *dA,
                          size_t lda, double *dResult, double *dScratch);
 clm_status_t clm_lange_z(clm_stream_t stream, int which, size_t m, size_t n,
@@ -190,7 +220,12 @@ clm_status_t clm_lange_z(clm_stream_t stream, int which, size_t m, size_t n,
 /* lascl<T> TYPE='G' (src/lapack/lascl/interface.cppm:79-143): A *= cto/cfrom
  * through netlib's over/underflow-guarded chain of multipliers, in place,
  * asynchronously. A zero or NaN cfrom, or a NaN cto, is
- * CLM_STATUS_RUNTIME_INVALID_VALUE. */
+ * CLM_STATUS_RUNTIME_INVALID_VALUE. The multipliers are formed in the real
+ * type of the element (float for s/c). */
+clm_status_t clm_lascl_s(clm_stream_t stream, float cfrom, float cto, size_t m, size_t n,
+                         float *dA, size_t lda);
+clm_status_t clm_lascl_c(clm_stream_t stream, float cfrom, float cto, size_t m, size_t n,
+                         clm_float_complex *dA, size_t lda);
 clm_status_t clm_lascl_d(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
                          double *dA, size_t lda);
 clm_status_t clm_lascl_z(clm_stream_t stream, double cfrom, double cto, size_t m, size_t n,
@@ -209,14 +244,25 @@ typedef struct clm_ps_plan {
 int clm_paterson_stockmeyer_cost(int degree, int s);
 int clm_paterson_stockmeyer_block_size(int degree);
 clm_ps_plan clm_paterson_stockmeyer_plan(int degree, int s_requested);
+size_t clm_paterson_stockmeyer_power_stride_s(int n);
 size_t clm_paterson_stockmeyer_power_stride_d(int n);
+size_t clm_paterson_stockmeyer_power_stride_c(int n);
 size_t clm_paterson_stockmeyer_power_stride_z(int n);
+size_t clm_paterson_stockmeyer_buffer_size_s(int n, int degree, int s_requested);
 size_t clm_paterson_stockmeyer_buffer_size_d(int n, int degree, int s_requested);
+size_t clm_paterson_stockmeyer_buffer_size_c(int n, int degree, int s_requested);
 size_t clm_paterson_stockmeyer_buffer_size_z(int n, int degree, int s_requested);
 
 /* paterson_stockmeyer<T> (src/paterson_stockmeyer/interface.cppm:329-422):
  * P = sum_{k=0}^{degree} c_k A^k, coefficients ascending in DEVICE memory,
  * asynchronous on the context's stream. */
+clm_status_t clm_paterson_stockmeyer_s(clm_ctx *ctx, int n, const float *dCoeffs, int degree,
+                                       const float *dA, int lda, float *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested);
+clm_status_t clm_paterson_stockmeyer_c(clm_ctx *ctx, int n, const clm_float_complex *dCoeffs,
+                                       int degree, const clm_float_complex *dA, int lda,
+                                       clm_float_complex *dP, int ldp, void *dWork,
+                                       size_t work_bytes, int s_requested);
 clm_status_t clm_paterson_stockmeyer_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
                                        const double *dA, int lda, double *dP, int ldp,
                                        void *dWork, size_t work_bytes, int s_requested);
@@ -226,8 +272,15 @@ clm_status_t clm_paterson_stockmeyer_z(clm_ctx *ctx, int n, const clm_double_com
                                        size_t work_bytes, int s_requested);
 
 /* horner_bufferSize<T> / horner<T> (src/horner/interface.cppm:103-194). */
+size_t clm_horner_buffer_size_s(int n);
 size_t clm_horner_buffer_size_d(int n);
+size_t clm_horner_buffer_size_c(int n);
 size_t clm_horner_buffer_size_z(int n);
+clm_status_t clm_horner_s(clm_ctx *ctx, int n, const float *dCoeffs, int degree, const float *dA,
+                          int lda, float *dP, int ldp, void *dWork, size_t work_bytes);
+clm_status_t clm_horner_c(clm_ctx *ctx, int n, const clm_float_complex *dCoeffs, int degree,
+                          const clm_float_complex *dA, int lda, clm_float_complex *dP, int ldp,
+                          void *dWork, size_t work_bytes);
 clm_status_t clm_horner_d(clm_ctx *ctx, int n, const double *dCoeffs, int degree,
                           const double *dA, int lda, double *dP, int ldp, void *dWork,
                           size_t work_bytes);
@@ -240,6 +293,11 @@ clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs
  * src/paterson_stockmeyer/interface.cppm:372-373): C = alpha*A*B + beta*C for
  * an m x k A and a k x n B, on the MFMA kernels of this library, asynchronous
  * on the context's stream. beta == 0 never reads C. */
+clm_status_t clm_gemm_s(clm_ctx *ctx, int m, int n, int k, float alpha, const float *dA, int lda,
+                        const float *dB, int ldb, float beta, float *dC, int ldc);
+clm_status_t clm_gemm_c(clm_ctx *ctx, int m, int n, int k, clm_float_complex alpha,
+                        const clm_float_complex *dA, int lda, const clm_float_complex *dB,
+                        int ldb, clm_float_complex beta, clm_float_complex *dC, int ldc);
 clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
                         const double *dB, int ldb, double beta, double *dC, int ldc);
 /* the product on BF16 MFMA with split operands (split = CLM_SPLIT_BF16X3 / X6), FP64 in/out */

@@ -23,6 +23,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
@@ -56,6 +57,16 @@ def lib():
         for f in ("orc_lascl_d", "orc_lascl_z"):
             getattr(L, f).argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_size_t,
                                       ctypes.c_size_t, _dp, ctypes.c_size_t]
+        for f in ("orc_norm1_s", "orc_norm1_c"):
+            getattr(L, f).restype = ctypes.c_float
+            getattr(L, f).argtypes = [ctypes.c_int, _fp, ctypes.c_int]
+        for f in ("orc_lange_s", "orc_lange_c"):
+            getattr(L, f).restype = ctypes.c_float
+            getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, _fp,
+                                      ctypes.c_size_t]
+        for f in ("orc_lascl_s", "orc_lascl_c"):
+            getattr(L, f).argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_size_t,
+                                      ctypes.c_size_t, _fp, ctypes.c_size_t]
         L.orc_gemm_d.argtypes = [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp,
                                  ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int]
         L.orc_gemm_z.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
@@ -88,6 +99,14 @@ def lib():
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+def _is_single(a) -> bool:
+    return np.asarray(a).dtype in (np.float32, np.complex64)
 
 
 def _fortran(a, dtype):
@@ -134,7 +153,12 @@ def expm_plan(norm1: float, single: bool = False):
 
 # ---- norms -----------------------------------------------------------------
 def norm1(A) -> float:
+    """max_j sum_i |A_ij|; float32/complex64 inputs reduce in float."""
     c = _is_complex(A)
+    if _is_single(A):
+        A = _fortran(A, np.complex64 if c else np.float32)
+        f = lib().orc_norm1_c if c else lib().orc_norm1_s
+        return float(f(A.shape[0], _fptr(A), A.shape[0]))
     A = _fortran(A, np.complex128 if c else np.float64)
     n = A.shape[0]
     f = lib().orc_norm1_z if c else lib().orc_norm1_d
@@ -146,6 +170,11 @@ LANGE = {"max_abs": 0, "one": 1, "inf": 2, "frobenius": 3}
 
 def lange(which: str, A) -> float:
     c = _is_complex(A)
+    if _is_single(A):
+        A = _fortran(A, np.complex64 if c else np.float32)
+        m, n = A.shape
+        f = lib().orc_lange_c if c else lib().orc_lange_s
+        return float(f(LANGE[which], m, n, _fptr(A), max(m, 1)))
     A = _fortran(A, np.complex128 if c else np.float64)
     m, n = A.shape
     f = lib().orc_lange_z if c else lib().orc_lange_d
@@ -155,6 +184,12 @@ def lange(which: str, A) -> float:
 def lascl(cfrom: float, cto: float, A):
     """Returns (status, scaled copy)."""
     c = _is_complex(A)
+    if _is_single(A):
+        A = _fortran(A, np.complex64 if c else np.float32).copy(order="F")
+        m, n = A.shape
+        f = lib().orc_lascl_c if c else lib().orc_lascl_s
+        st = f(cfrom, cto, m, n, _fptr(A), max(m, 1))
+        return int(st), A
     A = _fortran(A, np.complex128 if c else np.float64).copy(order="F")
     m, n = A.shape
     f = lib().orc_lascl_z if c else lib().orc_lascl_d

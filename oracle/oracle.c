@@ -353,6 +353,169 @@ int orc_lascl_z(double cfrom, double cto, size_t m, size_t n, double *A, size_t 
 }
 
 /* =========================================================================
+ * Single precision (float / complex<float>): the same reductions and the same
+ * lascl chain in R = float, the reference's ComplexToRealType<T> for those
+ * element types (src/expm/norm1.cppm:45-56 accumulates in R; lascl's
+ * SMLNUM/BIGNUM are R's, src/lapack/lascl/interface.cppm:101-105).
+ * ========================================================================= */
+static inline float cabs_f(const float *z) { return hypotf(z[0], z[1]); }
+
+static inline float nan_max_f(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? b : a;
+}
+
+static float rc_fold_f(const float *col, size_t m, int cplx, int pre, int op) {
+  float sdata[RC_BLOCK];
+  const size_t nactive = MIN(m, (size_t)RC_BLOCK);
+  for (size_t tid = 0; tid < nactive; ++tid) {
+    float acc = 0.0f;
+    for (size_t i = tid; i < m; i += RC_BLOCK) {
+      float x;
+      if (pre == 0) {
+        x = cplx ? cabs_f(col + 2 * i) : fabsf(col[i]);
+      } else if (pre == 1) {
+        const float re = cplx ? col[2 * i] : col[i];
+        const float im = cplx ? col[2 * i + 1] : 0.0f;
+        x = re * re + im * im;
+      } else {
+        x = col[i];
+      }
+      if (i == tid) acc = x;
+      else acc = (op == 0) ? acc + x : nan_max_f(acc, x);
+    }
+    sdata[tid] = acc;
+  }
+  for (size_t s = RC_BLOCK / 2; s > 0; s >>= 1) {
+    for (size_t tid = 0; tid < s; ++tid) {
+      if (tid + s < nactive)
+        sdata[tid] = (op == 0) ? sdata[tid] + sdata[tid + s] : nan_max_f(sdata[tid], sdata[tid + s]);
+    }
+  }
+  return sdata[0];
+}
+
+static float max_reduce_f(const float *v, int n) {
+  float sdata[RC_BLOCK];
+  for (int tid = 0; tid < RC_BLOCK; ++tid) {
+    float best = 0.0f;
+    for (int i = tid; i < n; i += RC_BLOCK) {
+      const float x = v[i];
+      if (x > best || isnan(x)) best = x;
+    }
+    sdata[tid] = best;
+  }
+  for (int s = RC_BLOCK / 2; s > 0; s >>= 1) {
+    for (int tid = 0; tid < s; ++tid) {
+      const float other = sdata[tid + s];
+      if (other > sdata[tid] || isnan(other)) sdata[tid] = other;
+    }
+  }
+  return sdata[0];
+}
+
+static float norm1_impl_f(int n, const float *A, int lda, int cplx) {
+  if (n < 1) return 0.0f;
+  float *colsum = (float *)malloc(sizeof(float) * (size_t)n);
+  const size_t es = cplx ? 2 : 1;
+  for (int j = 0; j < n; ++j) colsum[j] = rc_fold_f(A + es * (size_t)j * lda, (size_t)n, cplx, 0, 0);
+  const float r = max_reduce_f(colsum, n);
+  free(colsum);
+  return r;
+}
+
+float orc_norm1_s(int n, const float *A, int lda) { return norm1_impl_f(n, A, lda, 0); }
+float orc_norm1_c(int n, const float *A, int lda) { return norm1_impl_f(n, A, lda, 1); }
+
+static float lange_impl_f(int which, size_t m, size_t n, const float *A, size_t lda, int cplx) {
+  if (m == 0 || n == 0) return 0.0f;
+  const size_t es = cplx ? 2 : 1;
+  float r = 0.0f;
+  if (which == 2) {
+    float *rows = (float *)malloc(sizeof(float) * m);
+    for (size_t i = 0; i < m; ++i) {
+      float acc = 0.0f;
+      for (size_t j = 0; j < n; ++j) {
+        const float *e = A + es * (i + j * lda);
+        acc += cplx ? cabs_f(e) : fabsf(e[0]);
+      }
+      rows[i] = acc;
+    }
+    r = rc_fold_f(rows, m, 0, 2, 1);
+    free(rows);
+    return r;
+  }
+  float *cols = (float *)malloc(sizeof(float) * n);
+  for (size_t j = 0; j < n; ++j) {
+    const float *col = A + es * j * lda;
+    if (which == 0) cols[j] = rc_fold_f(col, m, cplx, 0, 1);
+    else if (which == 1) cols[j] = rc_fold_f(col, m, cplx, 0, 0);
+    else cols[j] = rc_fold_f(col, m, cplx, 1, 0);
+  }
+  if (which == 3) r = sqrtf(rc_fold_f(cols, n, 0, 2, 0));
+  else r = rc_fold_f(cols, n, 0, 2, 1);
+  free(cols);
+  return r;
+}
+
+float orc_lange_s(int which, size_t m, size_t n, const float *A, size_t lda) {
+  return lange_impl_f(which, m, n, A, lda, 0);
+}
+float orc_lange_c(int which, size_t m, size_t n, const float *A, size_t lda) {
+  return lange_impl_f(which, m, n, A, lda, 1);
+}
+
+static int lascl_impl_f(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda,
+                        int cplx) {
+  if (cfrom == 0.0f || isnan(cfrom) || isnan(cto)) return -1;
+  if (m == 0 || n == 0) return 0;
+  const float smlnum = FLT_MIN;
+  const float bignum = 1.0f / smlnum;
+  float cfromc = cfrom, ctoc = cto;
+  int done = 0;
+  const size_t es = cplx ? 2 : 1;
+  do {
+    const float cfrom1 = cfromc * smlnum;
+    float mul;
+    if (cfrom1 == cfromc) {
+      mul = ctoc / cfromc;
+      done = 1;
+    } else {
+      const float cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        mul = ctoc;
+        done = 1;
+        cfromc = 1.0f;
+      } else if (fabsf(cfrom1) > fabsf(ctoc) && ctoc != 0.0f) {
+        mul = smlnum;
+        done = 0;
+        cfromc = cfrom1;
+      } else if (fabsf(cto1) > fabsf(cfromc)) {
+        mul = bignum;
+        done = 0;
+        ctoc = cto1;
+      } else {
+        mul = ctoc / cfromc;
+        done = 1;
+      }
+    }
+    for (size_t j = 0; j < n; ++j) {
+      float *col = A + es * j * lda;
+      for (size_t i = 0; i < es * m; ++i) col[i] = col[i] * mul;
+    }
+  } while (!done);
+  return 0;
+}
+
+int orc_lascl_s(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda) {
+  return lascl_impl_f(cfrom, cto, m, n, A, lda, 0);
+}
+int orc_lascl_c(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda) {
+  return lascl_impl_f(cfrom, cto, m, n, A, lda, 1);
+}
+
+/* =========================================================================
  * Deterministic GEMM. Parallel over row strips; every element of C is one
  * running sum over k = 0..n-1 in ascending order, so the result does not
  * depend on the thread count.

@@ -49,6 +49,15 @@ double orc_lange_z(int which, size_t m, size_t n, const double *A, size_t lda);
 int orc_lascl_d(double cfrom, double cto, size_t m, size_t n, double *A, size_t lda);
 int orc_lascl_z(double cfrom, double cto, size_t m, size_t n, double *A, size_t lda);
 
+/* The same three in single precision (accumulation and multipliers in float);
+ * complex<float> as interleaved float pairs. */
+float orc_norm1_s(int n, const float *A, int lda);
+float orc_norm1_c(int n, const float *A, int lda);
+float orc_lange_s(int which, size_t m, size_t n, const float *A, size_t lda);
+float orc_lange_c(int which, size_t m, size_t n, const float *A, size_t lda);
+int orc_lascl_s(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda);
+int orc_lascl_c(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda);
+
 /* ---- BLAS-3 / LAPACK pieces the reference takes from the vendor ---------- */
 /* C = alpha*A*B + beta*C, n x n, deterministic: each element sums k ascending,
  * independent of the thread count. beta == 0 never reads C. */

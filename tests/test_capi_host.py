@@ -46,11 +46,25 @@ def test_expm_plan_matches_oracle(norm):
     assert (q.m, q.s, q.num_gemms) == oracle.expm_plan(norm)
 
 
+@pytest.mark.parametrize("norm", [0.0, 1e-9, 1e-4, 0.1, 0.4258730034897931, 0.5, 1.0, 1.2, 4.0,
+                                  6.0, 11.25, 20.0, 100.0, 1e6])
+def test_single_plans_match_oracle(norm):
+    p = L.clm_expm_plan_s(norm)
+    t = oracle.taylor_plan(norm, single=True)
+    assert (p.m, p.s, p.num_gemms, p.ps_block) == (t["degree"], t["s"], t["num_gemms"],
+                                                   t["ps_block"])
+    # expm_plan<float>: comparisons in float (src/expm/plan.cppm:112-130)
+    q = L.clm_pade_plan_s(norm)
+    assert (q.m, q.s, q.num_gemms) == oracle.expm_plan(norm, single=True)
+
+
 def test_thresholds_match_oracle():
     for m in list(range(0, 35)):
         assert L.clm_taylor_theta_d(m) == oracle.taylor_theta(m)
+        assert L.clm_taylor_theta_s(m) == oracle.taylor_theta(m, single=True)
     for m in range(0, 15):
         assert L.clm_pade_theta_d(m) == oracle.pade_theta(m)
+        assert L.clm_pade_theta_s(m) == oracle.pade_theta(m, single=True)
 
 
 def test_ps_plan_functions_match_oracle():
@@ -89,6 +103,15 @@ def test_expm_and_horner_buffer_sizes():
         zblk = ((n * n * 16 + 255) // 256) * 256
         assert out.value == colsum + 6 * zblk
     assert L.clm_expm_buffer_size_d(0, ctypes.byref(out)) == ST.INVALID_VALUE
+    for n in (1, 7, 64, 2048):
+        colsum = ((n * 4 + 255) // 256) * 256
+        for f, es in ((L.clm_expm_buffer_size_s, 4), (L.clm_expm_buffer_size_c, 8)):
+            assert f(n, ctypes.byref(out)) == 0
+            assert out.value == colsum + 6 * (((n * n * es + 255) // 256) * 256)
+        assert L.clm_horner_buffer_size_s(n) == ((n * n * 4 + 255) // 256) * 256
+        assert L.clm_horner_buffer_size_c(n) == ((n * n * 8 + 255) // 256) * 256
+    assert L.clm_paterson_stockmeyer_power_stride_s(5) == 64  # 100 B -> 256 B
+    assert L.clm_paterson_stockmeyer_power_stride_c(5) == 32
 
 
 def _p(x):
@@ -101,7 +124,7 @@ def test_expm_rejects_bad_arguments_before_device():
     e = np.zeros(16)
     w = np.zeros(64)
     pa, pe, pw = (_p(x.ctypes.data) for x in (a, e, w))
-    for f in (L.clm_expm_d, L.clm_expm_z):
+    for f in (L.clm_expm_d, L.clm_expm_z, L.clm_expm_s, L.clm_expm_c):
         call = lambda n, lda, lde, work: f(None, None, n, pa, lda, pe, lde, work, 512, None, None)
         assert call(0, 4, 4, pw) == ST.INVALID_VALUE
         assert call(4, 3, 4, pw) == ST.INVALID_VALUE
@@ -115,7 +138,8 @@ def test_expm_rejects_bad_arguments_before_device():
 def test_ps_horner_reject_with_not_initialized():
     a = np.ones(16)
     pa = _p(a.ctypes.data)
-    for f in (L.clm_paterson_stockmeyer_d, L.clm_paterson_stockmeyer_z):
+    for f in (L.clm_paterson_stockmeyer_d, L.clm_paterson_stockmeyer_z,
+              L.clm_paterson_stockmeyer_s, L.clm_paterson_stockmeyer_c):
         assert f(None, 0, pa, 3, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # n < 1
         assert f(None, 4, pa, -1, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # degree
         assert f(None, 4, pa, 3, pa, 3, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # lda
@@ -124,7 +148,7 @@ def test_ps_horner_reject_with_not_initialized():
         assert f(None, 4, None, 3, pa, 4, pa, 4, pa, 10 ** 6, 0) == ST.NOT_INITIALIZED  # coeffs
         assert f(None, 4, pa, 3, pa, 4, pa, 4, None, 0, 0) == ST.NOT_INITIALIZED  # workspace
         assert f(None, 4, pa, 3, pa, 4, pa, 4, pa, 8, 0) == ST.NOT_INITIALIZED  # small workspace
-    for f in (L.clm_horner_d, L.clm_horner_z):
+    for f in (L.clm_horner_d, L.clm_horner_z, L.clm_horner_s, L.clm_horner_c):
         assert f(None, 0, pa, 3, pa, 4, pa, 4, pa, 10 ** 6) == ST.NOT_INITIALIZED
         assert f(None, 4, pa, 3, pa, 4, pa, 4, None, 0) == ST.NOT_INITIALIZED
         assert f(None, 4, pa, 3, None, 4, pa, 4, pa, 10 ** 6) == ST.NOT_INITIALIZED
@@ -133,13 +157,13 @@ def test_ps_horner_reject_with_not_initialized():
 def test_lascl_lange_gemm_argument_checks():
     a = np.ones(16)
     pa = _p(a.ctypes.data)
-    for f in (L.clm_lascl_d, L.clm_lascl_z):
+    for f in (L.clm_lascl_d, L.clm_lascl_z, L.clm_lascl_s, L.clm_lascl_c):
         assert f(None, 0.0, 1.0, 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
         assert f(None, float("nan"), 1.0, 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
         assert f(None, 1.0, float("nan"), 4, 4, pa, 4) == ST.RUNTIME_INVALID_VALUE
         assert f(None, 1.0, 2.0, 0, 4, pa, 4) == ST.SUCCESS  # empty: nothing enqueued
         assert f(None, 1.0, 2.0, 4, 4, pa, 3) == ST.RUNTIME_INVALID_VALUE
-    for f in (L.clm_lange_d, L.clm_lange_z):
+    for f in (L.clm_lange_d, L.clm_lange_z, L.clm_lange_s, L.clm_lange_c):
         assert f(None, 7, 4, 4, pa, 4, pa, pa) == ST.RUNTIME_INVALID_VALUE
         assert f(None, 1, 4, 4, pa, 3, pa, pa) == ST.RUNTIME_INVALID_VALUE
         assert f(None, 1, 4, 4, pa, 4, None, pa) == ST.RUNTIME_INVALID_VALUE
@@ -147,3 +171,7 @@ def test_lascl_lange_gemm_argument_checks():
     assert L.clm_gemm_d(None, -1, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4) == ST.INVALID_VALUE
     assert L.clm_gemm_d(None, 0, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4) == ST.SUCCESS
     assert L.clm_matrix_norm1_d(None, 0, pa, 4, pa, ctypes.byref(ctypes.c_double())) == ST.INVALID_VALUE
+    assert L.clm_gemm_s(None, 4, 4, 4, 1.0, pa, 3, pa, 4, 0.0, pa, 4) == ST.INVALID_VALUE
+    assert L.clm_matrix_norm1_s(None, 0, pa, 4, pa, ctypes.byref(ctypes.c_float())) == ST.INVALID_VALUE
+    one = _lib.FloatComplex(1.0, 0.0)
+    assert L.clm_gemm_c(None, 4, 4, 4, one, pa, 4, pa, 3, one, pa, 4) == ST.INVALID_VALUE

@@ -229,6 +229,58 @@ def test_lange_empty_and_nan():
         assert math.isnan(oracle.lange(w, a))
 
 
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+@pytest.mark.parametrize("n", [1, 3, 256, 257, 600])
+def test_norm1_single_against_numpy(n):
+    rng = np.random.default_rng(n + 7)
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    assert oracle.norm1(a) == pytest.approx(np.abs(a.astype(np.float64)).sum(0).max(),
+                                            rel=4 * n * EPS32)
+    z = (a + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    assert oracle.norm1(z) == pytest.approx(np.abs(z.astype(np.complex128)).sum(0).max(),
+                                            rel=4 * n * EPS32)
+
+
+def test_norm1_single_accumulates_in_float():
+    """column (1, 2^-24, 2^-24): the fold adds each 2^-24 to 1 separately
+    (lane 2 joins lane 0 at the tree step s = 2, lane 1 at s = 1), and each add
+    rounds back to 1 in float; a double accumulator would give 1 + 2^-23."""
+    a = np.zeros((3, 3), dtype=np.float32)
+    a[:, 0] = [1.0, 2.0 ** -24, 2.0 ** -24]
+    assert oracle.norm1(a) == 1.0
+    assert oracle.lange("one", a) == 1.0
+    assert oracle.norm1(a.astype(np.float64)) == 1.0 + 2.0 ** -23
+
+
+def test_lange_single_against_numpy():
+    rng = np.random.default_rng(11)
+    for shape in ((1, 1), (7, 3), (300, 260), (5, 600)):
+        a = rng.standard_normal(shape).astype(np.float32)
+        w = a.astype(np.float64)
+        for which, ref in (("max_abs", np.abs(w).max()), ("one", np.abs(w).sum(0).max()),
+                           ("inf", np.abs(w).sum(1).max()), ("frobenius", np.linalg.norm(w))):
+            assert oracle.lange(which, a) == pytest.approx(ref, rel=8 * max(shape) * EPS32)
+    nan = np.ones((4, 4), dtype=np.float32)
+    nan[1, 1] = np.nan
+    assert all(math.isnan(oracle.lange(w, nan)) for w in ("max_abs", "one", "inf", "frobenius"))
+
+
+def test_lascl_single_guarded_chain():
+    """cto/cfrom = 1e60 overflows float if formed directly; the chain (BIGNUM
+    first, then the representable rest) lands on 1e25 for 1e-35."""
+    a = np.full((2, 3), 1e-35, dtype=np.float32)
+    st, got = oracle.lascl(1e-30, 1e30, a)
+    assert st == 0 and got.dtype == np.float32
+    assert np.all(np.isfinite(got))
+    assert np.allclose(got, 1e25, rtol=8 * EPS32)
+    st, got = oracle.lascl(8.0, 1.0, np.full((2, 2), 3.0, dtype=np.float32))
+    assert st == 0 and np.all(got == np.float32(0.375))
+    for cfrom, cto in ((0.0, 1.0), (np.nan, 1.0), (1.0, np.nan)):
+        assert oracle.lascl(cfrom, cto, a)[0] == -1
+
+
 # ---- lascl --------------------------------------------------------------------------------------
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("s", [0, 1, 5, 7, 40])

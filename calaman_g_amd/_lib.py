@@ -73,6 +73,20 @@ class FloatComplex(ctypes.Structure):
     _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
 
 
+BATCH_BUCKETS = 8
+BATCH_TILE_32, BATCH_TILE_64, BATCH_TILE_128x64, BATCH_TILE_128 = 0, 1, 2, 3
+
+
+class BatchEntry(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("dtype", ctypes.c_int), ("A", ctypes.c_void_p),
+                ("lda", ctypes.c_int), ("E", ctypes.c_void_p), ("lde", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
+                ("status", ctypes.c_int), ("norm1", ctypes.c_double)]
+
+
 class CalamanError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -158,6 +172,15 @@ _SIGNATURES = {
     "clm_horner_c": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "clm_gemm_s": (_i, [_vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _i]),
     "clm_gemm_c": (_i, [_vp, _i, _i, _i, FloatComplex, _vp, _i, _vp, _i, FloatComplex, _vp, _i]),
+    # batched
+    "clm_expm_batched_plan_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchEntry)]),
+    "clm_expm_batched_plan_destroy": (_i, [_vp]),
+    "clm_expm_batched_buffer_size": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "clm_expm_batched": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
+    "clm_expm_batched_graph": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
+    "clm_expm_batched_get_tuning": (_i, [_i, ctypes.POINTER(_i)]),
+    "clm_expm_batched_set_tuning": (_i, [_i, ctypes.POINTER(_i)]),
+    "clm_expm_batched_autotune": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_f)]),
 }
 
 

@@ -1,0 +1,923 @@
+// batched.hip -- many matrix exponentials of uneven orders in ONE persistent
+// launch (no reference counterpart: the reference runs one expm per call).
+//
+// Design:
+//   * one workgroup of 256 lanes (4 wave64) owns one matrix at a time and runs
+//     its whole chain -- 1-norm, plan, 2^-s scaling, power bank, the
+//     Paterson-Stockmeyer Horner steps, the s squarings -- with workgroup
+//     barriers between the steps; every step reads and writes the matrix's own
+//     workspace, so workgroups never wait on one another;
+//   * the grid is persistent (as many workgroups as fit on the device at once)
+//     and takes matrices from a queue in decreasing cost order (n^3, x4 for
+//     complex): the largest chains start first, the small ones fill the tail;
+//   * the plan is chosen on the device from the device-computed 1-norm, with the
+//     same rules (expm_plan.hpp) and the same fold order as the single-matrix
+//     path, so nothing returns to the host and the launch can be captured in a
+//     hipGraph;
+//   * the products use the single-matrix kernels' tile helpers (gemm_tile.hpp)
+//     over a per-matrix choice of workgroup tile (32x32 .. 128x128) picked from a
+//     tuning table that clm_expm_batched_autotune measures;
+//   * each matrix therefore goes through the same arithmetic, in the same
+//     order, as clm_expm_{s,d,c,z} on its own: the results agree bit for bit.
+//
+// The kernel is instantiated per (element type, workgroup tile): one
+// instantiation holding every tile would carry the register demand of the
+// largest (measured: 256 VGPRs and spills, against 67 for the 64 x 64 FP32 tile
+// alone), so a plan groups its matrices by (type, tile) and launches one
+// persistent grid per group, back to back on the stream (one graph when
+// captured).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <type_traits>
+#include <vector>
+
+#include "expm_plan.hpp"
+#include "gemm_tile.hpp"
+
+namespace clm {
+
+struct BatchDesc {
+  const void *A;
+  void *E;
+  long long ws_off;  // byte offset of the matrix's workspace inside the caller's buffer
+  int n, lda, lde, dtype;
+  int index;  // position in the caller's entry array
+  int cfg;    // workgroup tile configuration (CLM_BATCH_TILE_*)
+};
+
+namespace {
+
+using namespace tile;
+
+constexpr int kNT = 256;
+constexpr int kBuckets = CLM_BATCH_BUCKETS;
+constexpr int kMaxBatchN = 4096;
+constexpr int kGroups = 16;          // (element type, tile) pairs: 4 x 4
+constexpr size_t kHeadBytes = 256;  // the groups' queue counters at the head of the workspace
+
+template <typename T>
+constexpr bool is_cplx() {
+  return std::is_same_v<T, zd> || std::is_same_v<T, zc>;
+}
+
+template <typename T>
+constexpr int dtype_of() {
+  if constexpr (std::is_same_v<T, double>) return CLM_DTYPE_F64;
+  else if constexpr (std::is_same_v<T, float>) return CLM_DTYPE_F32;
+  else if constexpr (std::is_same_v<T, zd>) return CLM_DTYPE_C128;
+  else return CLM_DTYPE_C64;
+}
+
+__device__ __forceinline__ double modulus(double x) { return fabs(x); }
+__device__ __forceinline__ double modulus(zd x) { return hypot(x.re, x.im); }
+__device__ __forceinline__ float modulus(float x) { return fabsf(x); }
+__device__ __forceinline__ float modulus(zc x) { return hypotf(x.re, x.im); }
+
+template <typename T>
+__host__ __device__ inline long stride_of(int n) {
+  return static_cast<long>(((static_cast<size_t>(n) * n * sizeof(T) + 255) & ~size_t(255)) /
+                           sizeof(T));
+}
+
+// per-matrix workspace: As | bank (max_bank_blocks()) | S, each an aligned n x n block
+template <typename T>
+size_t matrix_ws_bytes(int n) {
+  return static_cast<size_t>(2 + max_bank_blocks()) * stride_of<T>(n) * sizeof(T);
+}
+
+// ---- LDS budget per tile configuration -------------------------------------------
+template <typename S, int BM, int BN, int BK>
+constexpr size_t real_lds() {
+  using TL = RTile<S, BM, BN, kNT, BK>;
+  return 2 * (TL::A_STAGE + TL::B_STAGE) * sizeof(S);
+}
+template <typename C, int BM, int BN>
+constexpr size_t cplx_lds() {
+  using TL = CTile<C, BM, BN, kNT, kBK>;
+  return 2 * TL::STAGE * sizeof(typename CxReal<C>::type);
+}
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+
+// K slice of the 32 x 32 tile: 16 for FP64 (2 doubles per 16-byte vector),
+// 32 for FP32 (4 floats), so that each of the 256 lanes stages one vector
+template <typename S>
+constexpr int small_bk() {
+  return std::is_same_v<S, double> ? 16 : 32;
+}
+
+// the tile a configuration id stands for per element type: complex FP64 stops
+// at 64 x 64, 128 x 128 exists for real FP32 only (others take their largest)
+template <typename T>
+constexpr int effective_cfg(int cfg) {
+  if constexpr (std::is_same_v<T, zd>) return cfg > CLM_BATCH_TILE_64 ? CLM_BATCH_TILE_64 : cfg;
+  else if constexpr (std::is_same_v<T, float>) return cfg;
+  else return cfg > CLM_BATCH_TILE_128x64 ? CLM_BATCH_TILE_128x64 : cfg;
+}
+
+template <typename T, int CFG>
+constexpr size_t lds_bytes() {
+  constexpr int c = effective_cfg<T>(CFG);
+  if constexpr (is_cplx<T>()) {
+    if constexpr (c == CLM_BATCH_TILE_32) return cplx_lds<T, 32, 32>();
+    else if constexpr (c == CLM_BATCH_TILE_64) return cplx_lds<T, 64, 64>();
+    else return cplx_lds<T, 128, 64>();
+  } else {
+    if constexpr (c == CLM_BATCH_TILE_32) return real_lds<T, 32, 32, small_bk<T>()>();
+    else if constexpr (c == CLM_BATCH_TILE_64) return real_lds<T, 64, 64, kBK>();
+    else if constexpr (c == CLM_BATCH_TILE_128x64) return real_lds<T, 128, 64, kBK>();
+    else return real_lds<T, 128, 128, kBK>();
+  }
+}
+
+// A value every lane of the wave holds, re-declared uniform: the compiler then
+// keeps it (and what is computed from it) in scalar registers.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// ---- workgroup-wide building blocks ---------------------------------------------------
+// C = A*B (+ epilogue of g) over the whole n x n output, tile by tile, by all
+// 256 lanes; ends with a barrier, so C is complete for every lane afterwards.
+template <typename S, int BM, int BN, int WM, int WN, int BK>
+__device__ void wg_gemm_real(const GemmArgs<S> &g, char *lds) {
+  using TL = RTile<S, BM, BN, kNT, BK>;
+  using V = typename Mfma<S>::vec;
+  constexpr int WAVES_M = BM / WM;
+  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
+  constexpr int TM = WM / 16, TN = WN / 16;
+  S *As = reinterpret_cast<S *>(lds);
+  S *Bs = As + 2 * TL::A_STAGE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
+  const int KT = (g.K + BK - 1) / BK;
+  for (int t = 0; t < tiles_m * tiles_n; ++t) {
+    const int m0 = (t % tiles_m) * BM, n0 = (t / tiles_m) * BN;
+    typename Mfma<S>::acc acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<S>::zero();
+    V ra[TL::A_V], rb[TL::B_V];
+    __syncthreads();  // the previous tile's last slice is consumed before restaging
+    if (KT > 0) {
+      rgemm_gload<S, BM, BN, kNT, BK, false>(g, m0, n0, 0, tid, ra, rb);
+      rgemm_sstore<S, BM, BN, kNT, BK>(As, Bs, tid, ra, rb);
+      __syncthreads();
+      for (int kt = 0; kt < KT - 1; ++kt) {
+        const int cur = kt & 1;
+        rgemm_gload<S, BM, BN, kNT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
+        rgemm_compute<S, BM, BN, WM, WN, kNT, BK>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE,
+                                                  lane, wm0, wn0, acc);
+        rgemm_sstore<S, BM, BN, kNT, BK>(As + (cur ^ 1) * TL::A_STAGE,
+                                         Bs + (cur ^ 1) * TL::B_STAGE, tid, ra, rb);
+        __syncthreads();
+      }
+      const int last = (KT - 1) & 1;
+      rgemm_compute<S, BM, BN, WM, WN, kNT, BK>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE,
+                                                lane, wm0, wn0, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
+          const int col = n0 + wn0 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N) epilogue_store(g, row, col, S(acc[i][j][r]));
+        }
+  }
+  __syncthreads();
+}
+
+template <typename C, int BM, int BN, int WM, int WN>
+__device__ void wg_gemm_cplx(const GemmArgs<C> &g, char *lds) {
+  constexpr int BK = kBK;
+  using TL = CTile<C, BM, BN, kNT, BK>;
+  using S = typename CxReal<C>::type;
+  constexpr int WAVES_M = BM / WM;
+  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
+  constexpr int TM = WM / 16, TN = WN / 16;
+  S *L = reinterpret_cast<S *>(lds);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
+  const int KT = (g.K + BK - 1) / BK;
+  for (int t = 0; t < tiles_m * tiles_n; ++t) {
+    const int m0 = (t % tiles_m) * BM, n0 = (t / tiles_m) * BN;
+    typename Mfma<S>::acc cr[TM][TN], ci[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        cr[i][j] = Mfma<S>::zero();
+        ci[i][j] = Mfma<S>::zero();
+      }
+    C ra[TL::A_E], rb[TL::B_E];
+    __syncthreads();
+    if (KT > 0) {
+      cgemm_gload<C, BM, BN, kNT, BK, false>(g, m0, n0, 0, tid, ra, rb);
+      cgemm_sstore<C, BM, BN, kNT, BK>(L, tid, ra, rb);
+      __syncthreads();
+      for (int kt = 0; kt < KT - 1; ++kt) {
+        const int cur = kt & 1;
+        cgemm_gload<C, BM, BN, kNT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
+        cgemm_compute<C, BM, BN, WM, WN, kNT, BK>(L + cur * TL::STAGE, lane, wm0, wn0, cr, ci);
+        cgemm_sstore<C, BM, BN, kNT, BK>(L + (cur ^ 1) * TL::STAGE, tid, ra, rb);
+        __syncthreads();
+      }
+      cgemm_compute<C, BM, BN, WM, WN, kNT, BK>(L + ((KT - 1) & 1) * TL::STAGE, lane, wm0, wn0,
+                                                cr, ci);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
+          const int col = n0 + wn0 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N)
+            epilogue_store(g, row, col, C{S(cr[i][j][r]), S(ci[i][j][r])});
+        }
+  }
+  __syncthreads();
+}
+
+template <typename T, int CFG>
+__device__ void wg_gemm(const GemmArgs<T> &g, char *lds) {
+  constexpr int c = effective_cfg<T>(CFG);
+  if constexpr (is_cplx<T>()) {
+    if constexpr (c == CLM_BATCH_TILE_32) wg_gemm_cplx<T, 32, 32, 16, 16>(g, lds);
+    else if constexpr (c == CLM_BATCH_TILE_64) wg_gemm_cplx<T, 64, 64, 32, 32>(g, lds);
+    else wg_gemm_cplx<T, 128, 64, 64, 32>(g, lds);
+  } else {
+    if constexpr (c == CLM_BATCH_TILE_32) wg_gemm_real<T, 32, 32, 16, 16, small_bk<T>()>(g, lds);
+    else if constexpr (c == CLM_BATCH_TILE_64) wg_gemm_real<T, 64, 64, 32, 32, kBK>(g, lds);
+    else if constexpr (c == CLM_BATCH_TILE_128x64) wg_gemm_real<T, 128, 64, 64, 32, kBK>(g, lds);
+    else wg_gemm_real<T, 128, 128, 64, 64, kBK>(g, lds);
+  }
+}
+
+// The 1-norm in the fold order of the single-matrix reduction (256 virtual
+// lanes, each folding rows t, t+256, ... seeded by its first, then a halving
+// tree): virtual lane t + 64q sits in register q of lane t of a wave; the
+// tree's first two levels add registers, the remaining six are shuffles. A
+// virtual lane past n holds +0, which leaves every partial sum unchanged, so
+// the result is bit for bit the single-matrix kernel's. Each wave folds every
+// fourth column; the NaN-propagating max runs over the columns.
+template <typename T>
+__device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
+  using R = real_t<T>;
+  R *red = reinterpret_cast<R *>(lds);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  R best = 0;
+  for (int j = wave; j < n; j += 4) {
+    const T *col = A + static_cast<long>(j) * lda;
+    R v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = lane + 64 * q;
+      R acc = 0;
+      if (t < n) {
+        acc = modulus(col[t]);
+        for (int i = t + 256; i < n; i += 256) acc = acc + modulus(col[i]);
+      }
+      v[q] = acc;
+    }
+    v[0] = v[0] + v[2];
+    v[1] = v[1] + v[3];
+    v[0] = v[0] + v[1];
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      const R o = __shfl_down(v[0], s, 64);
+      if (lane < s) v[0] = v[0] + o;
+    }
+    const R cs = __shfl(v[0], 0, 64);
+    if (cs > best || isnan(cs)) best = cs;
+  }
+  __syncthreads();  // the LDS arena may still be read by a previous step
+  if (lane == 0) red[wave] = best;
+  __syncthreads();
+  R r = 0;
+  for (int w = 0; w < 4; ++w) {
+    const R v = red[w];
+    if (v > r || isnan(v)) r = v;
+  }
+  __syncthreads();
+  return r;
+}
+
+// out = f * in, column by column (f a power of two: exact)
+template <typename T>
+__device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, long ldo) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < n; j += 4)
+    for (int i = lane; i < n; i += 64)
+      out[i + static_cast<long>(j) * ldo] = ops<T>::scale(in[i + static_cast<long>(j) * ldi], f);
+  __syncthreads();
+}
+
+// B = c0 I + c1 X1 + sum_t c_t bank[t-2], the per-element order of the
+// single-matrix block build
+template <typename T>
+__device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms, const T *X1,
+                               long ldx1, const T *bank, long stride, long ldbank) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < n; j += 4)
+    for (int i = lane; i < n; i += 64) {
+      T acc = (i == j) ? coef[0] : ops<T>::zero();
+      if (nterms > 1) acc = ops<T>::mad(coef[1], X1[i + static_cast<long>(j) * ldx1], acc);
+      for (int t = 2; t < nterms; ++t)
+        acc = ops<T>::mad(coef[t], bank[static_cast<long>(t - 2) * stride + i + j * ldbank], acc);
+      B[i + static_cast<long>(j) * ldb] = acc;
+    }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ GemmArgs<T> square_args(int n, const T *A, long lda, const T *B, long ldb, T *C,
+                                   long ldc) {
+  GemmArgs<T> g;
+  g.M = g.N = g.K = n;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.alpha = ops<T>::from_real(1.0);
+  return g;
+}
+
+// One matrix's chain as a single sequence of products, so the product code is
+// instantiated once per tile configuration (not once per phase):
+//   steps [0, nbank)             power bank A^2 .. A^(nbank+1) of the scaled A,
+//   (block B_r built here)
+//   steps [nbank, nbank + r)     Paterson-Stockmeyer Horner steps P <- P X + B_j,
+//                                B_j formed in the product's epilogue,
+//   steps [nbank + r, + s)       squarings.
+// This is the single-matrix evaluator's order of operations exactly; the
+// polynomial lands where the squarings end in E (the parity of s picks it).
+template <typename T, int CFG>
+__device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *lds,
+                         clm_batch_result *results) {
+  using R = real_t<T>;
+  const int n = d.n;
+  const T *A = static_cast<const T *>(d.A);
+  T *E = static_cast<T *>(d.E);
+  const long stride = stride_of<T>(n);
+  T *As = reinterpret_cast<T *>(work + d.ws_off);
+  T *bank = As + stride;
+  T *S = bank + static_cast<long>(max_bank_blocks()) * stride;
+
+  const R norm = uniform(wg_norm1(n, A, d.lda, lds));
+  if (!isfinite(norm)) {
+    if (threadIdx.x == 0 && results)
+      results[d.index] = clm_batch_result{0, 0, 0, CLM_STATUS_INVALID_VALUE, double(norm)};
+    return;  // uniform: every lane holds the same norm
+  }
+  const clm_expm_plan p = taylor_plan(double(norm), std::is_same_v<R, float>);
+  const int degree = uniform(p.m), nsq = uniform(p.s);
+  if (threadIdx.x == 0 && results)
+    results[d.index] =
+        clm_batch_result{degree, nsq, uniform(p.num_gemms), CLM_STATUS_SUCCESS, double(norm)};
+
+  const T *src = A;
+  long ld_src = d.lda;
+  if (nsq > 0) {
+    wg_scale_copy(n, ldexp(1.0, -nsq), A, d.lda, As, n);
+    src = As;
+    ld_src = n;
+  }
+  const clm_ps_plan pp = ps_plan(degree, 0);
+  const int bs = uniform(pp.s), nhorner = uniform(pp.r), nbank = uniform(pp.num_powers);
+  const T *X = (bs >= 2) ? bank + static_cast<long>(bs - 2) * stride : src;
+  const long ldx = (bs >= 2) ? long(n) : ld_src;
+  T *target = (nsq % 2 == 0) ? E : S;
+  const long ld_target = (nsq % 2 == 0) ? long(d.lde) : long(n);
+  T *other = (nsq % 2 == 0) ? S : E;
+  const long ld_other = (nsq % 2 == 0) ? long(n) : long(d.lde);
+  // Horner ping-pong: starts where r steps end in target
+  int cur = (nhorner % 2 == 0) ? 0 : 1;
+  const int total = nbank + nhorner + nsq;
+  T *res = target, *spare = other;  // squaring operands once the polynomial is done
+  long ld_res = ld_target, ld_spare = ld_other;
+  for (int q = 0; q <= total; ++q) {
+    if (q == nbank) {
+      const int terms_r = min(degree - nhorner * bs + 1, bs);
+      wg_build_block(n, cur == 0 ? target : other, cur == 0 ? ld_target : ld_other,
+                     coef + static_cast<long>(nhorner) * bs, terms_r, src, ld_src, bank, stride,
+                     static_cast<long>(n));
+    }
+    if (q == total) break;
+    GemmArgs<T> g;
+    if (q < nbank) {
+      const int k = q + 2;
+      const T *left = (k == 2) ? src : bank + static_cast<long>(k - 3) * stride;
+      g = square_args(n, left, (k == 2) ? ld_src : long(n), src, ld_src,
+                      bank + static_cast<long>(k - 2) * stride, long(n));
+    } else if (q < nbank + nhorner) {
+      const int j = nhorner - 1 - (q - nbank);
+      T *from = cur == 0 ? target : other;
+      T *to = cur == 0 ? other : target;
+      g = square_args(n, from, cur == 0 ? ld_target : ld_other, X, ldx, to,
+                      cur == 0 ? ld_other : ld_target);
+      g.coef = coef + static_cast<long>(j) * bs;
+      g.nterms = min(degree - j * bs + 1, bs);
+      g.X1 = src;
+      g.ldx1 = ld_src;
+      g.bank = bank;
+      g.bank_stride = stride;
+      g.ldbank = n;
+      cur = 1 - cur;
+    } else {
+      g = square_args(n, res, ld_res, res, ld_res, spare, ld_spare);
+      T *t = res;
+      res = spare;
+      spare = t;
+      const long tl = ld_res;
+      ld_res = ld_spare;
+      ld_spare = tl;
+    }
+    wg_gemm<T, CFG>(g, lds);
+  }
+}
+
+// The persistent kernel of one (element type, tile) group: each workgroup
+// takes the next matrix from the group's queue (one atomic per matrix, by lane
+// 0, broadcast through LDS) until the queue is drained; every workgroup reaches
+// the exit once the counter passes `count`.
+template <typename T, int CFG>
+__global__ void __launch_bounds__(kNT)
+    expm_batched_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
+                        clm_batch_result *results, const T *coef) {
+  __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T, CFG>()];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    // uniform across the workgroup: readfirstlane tells the compiler so, which
+    // keeps the descriptor and every size and pointer derived from it in SGPRs
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= count) break;
+    const BatchDesc d = desc[item];
+    expm_one<T, CFG>(d, work, coef, lds, results);
+  }
+}
+
+__global__ void reset_queue_kernel(int *q) {
+  if (threadIdx.x < kGroups) q[threadIdx.x] = 0;
+}
+
+// deterministic test matrices for the tuner: entries in [-0.5, 0.5) / n
+template <typename T>
+__global__ void fill_tuning_matrices(T *A, int n, int count, unsigned seed) {
+  const long total = static_cast<long>(n) * n * count;
+  for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    unsigned h = static_cast<unsigned>(i) * 2654435761u ^ seed;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    const double u = (h & 0xffffff) / double(1 << 24) - 0.5;
+    if constexpr (is_cplx<T>()) {
+      unsigned g = h * 3266489917u;
+      g ^= g >> 16;
+      const double w = (g & 0xffffff) / double(1 << 24) - 0.5;
+      A[i] = T{static_cast<decltype(T::re)>(u / n), static_cast<decltype(T::re)>(w / n)};
+    } else {
+      A[i] = static_cast<T>(u / n);
+    }
+  }
+}
+
+// ---- tuning table ---------------------------------------------------------------------
+// cfg[precision: 0 FP32, 1 FP64][0 real, 1 complex][bucket], bucket b covers
+// n in (32 b, 32 (b + 1)], the last one everything above.
+std::mutex g_tune_mutex;
+int g_tune[2][2][kBuckets];
+bool g_tune_init = false;
+
+void tune_defaults() {
+  for (int p = 0; p < 2; ++p)
+    for (int c = 0; c < 2; ++c)
+      for (int b = 0; b < kBuckets; ++b) {
+        const int nmax = 32 * (b + 1);
+        int cfg = nmax <= 32 ? CLM_BATCH_TILE_32 : nmax <= 96 ? CLM_BATCH_TILE_64
+                                                               : CLM_BATCH_TILE_128x64;
+        g_tune[p][c][b] = cfg;
+      }
+  g_tune_init = true;
+}
+
+int bucket_of(int n) {
+  const int b = (n - 1) / 32;
+  return b < kBuckets ? b : kBuckets - 1;
+}
+
+bool is_complex_code(int dtype) { return dtype == CLM_DTYPE_C64 || dtype == CLM_DTYPE_C128; }
+int precision_of(int dtype) { return (dtype == CLM_DTYPE_F64 || dtype == CLM_DTYPE_C128) ? 1 : 0; }
+bool valid_dtype(int dtype) {
+  return dtype == CLM_DTYPE_F32 || dtype == CLM_DTYPE_F64 || dtype == CLM_DTYPE_C64 ||
+         dtype == CLM_DTYPE_C128;
+}
+
+size_t ws_bytes_of(int dtype, int n) {
+  switch (dtype) {
+  case CLM_DTYPE_F32: return matrix_ws_bytes<float>(n);
+  case CLM_DTYPE_F64: return matrix_ws_bytes<double>(n);
+  case CLM_DTYPE_C64: return matrix_ws_bytes<zc>(n);
+  default: return matrix_ws_bytes<zd>(n);
+  }
+}
+
+// group id of an (element type, tile) pair; dtype index F32 0, F64 1, C64 2, C128 3
+int group_of(int dtype, int cfg) {
+  const int t = dtype == CLM_DTYPE_F32 ? 0 : dtype == CLM_DTYPE_F64 ? 1 : dtype == CLM_DTYPE_C64 ? 2 : 3;
+  return t * 4 + cfg;
+}
+
+// workgroups resident at once on the device for one kernel instantiation
+template <typename T, int CFG>
+int resident_workgroups(int device) {
+  static int cached[64] = {};
+  int &r = cached[device & 63];
+  if (r == 0) {
+    int blocks = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_batched_kernel<T, CFG>, kNT, 0) !=
+            hipSuccess ||
+        blocks < 1)
+      blocks = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus < 1)
+      cus = 256;
+    r = blocks * cus;
+  }
+  return r;
+}
+
+template <typename T, int CFG>
+hipError_t launch_group(hipStream_t st, int device, const BatchDesc *desc, int count, int *queue,
+                        char *work, clm_batch_result *results) {
+  const T *coef = inv_factorials<T>(device);
+  if (!coef) return hipErrorInvalidValue;
+  const int grid = std::min(count, resident_workgroups<T, CFG>(device));
+  hipLaunchKernelGGL((expm_batched_kernel<T, CFG>), dim3(grid), dim3(kNT), 0, st, desc, count,
+                     queue, work, results, coef);
+  return hipGetLastError();
+}
+
+// the occupancy and coefficient-table lookups a launch needs, done once ahead
+// of it (so none of them runs inside a stream capture)
+template <typename T>
+bool prepare_type(int cfg, int device) {
+  if (!inv_factorials<T>(device)) return false;
+  switch (cfg) {
+  case CLM_BATCH_TILE_32: return resident_workgroups<T, CLM_BATCH_TILE_32>(device) > 0;
+  case CLM_BATCH_TILE_64: return resident_workgroups<T, CLM_BATCH_TILE_64>(device) > 0;
+  case CLM_BATCH_TILE_128x64: return resident_workgroups<T, CLM_BATCH_TILE_128x64>(device) > 0;
+  default: return resident_workgroups<T, CLM_BATCH_TILE_128>(device) > 0;
+  }
+}
+
+bool prepare_group(int g, int device) {
+  switch (g / 4) {
+  case 0: return prepare_type<float>(g % 4, device);
+  case 1: return prepare_type<double>(g % 4, device);
+  case 2: return prepare_type<zc>(g % 4, device);
+  default: return prepare_type<zd>(g % 4, device);
+  }
+}
+
+template <typename T>
+hipError_t launch_type(int cfg, hipStream_t st, int device, const BatchDesc *desc, int count,
+                       int *queue, char *work, clm_batch_result *results) {
+  switch (cfg) {
+  case CLM_BATCH_TILE_32:
+    return launch_group<T, CLM_BATCH_TILE_32>(st, device, desc, count, queue, work, results);
+  case CLM_BATCH_TILE_64:
+    return launch_group<T, CLM_BATCH_TILE_64>(st, device, desc, count, queue, work, results);
+  case CLM_BATCH_TILE_128x64:
+    return launch_group<T, CLM_BATCH_TILE_128x64>(st, device, desc, count, queue, work, results);
+  default:
+    return launch_group<T, CLM_BATCH_TILE_128>(st, device, desc, count, queue, work, results);
+  }
+}
+
+}  // namespace
+}  // namespace clm
+
+// ---- plan object --------------------------------------------------------------------------
+struct clm_batch_plan {
+  int device = 0;
+  int count = 0;
+  int group_count[clm::kGroups] = {};  // matrices per (element type, tile) group
+  int group_start[clm::kGroups] = {};
+  clm::BatchDesc *d_desc = nullptr;    // grouped, each group in decreasing cost
+  size_t ws_bytes = 0;
+  std::vector<clm::BatchDesc> host_desc;
+  // the captured launch (clm_expm_batched_graph)
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  void *graph_work = nullptr;
+  void *graph_results = nullptr;
+};
+
+namespace clm {
+namespace {
+
+clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *forced_cfg,
+                        clm_batch_plan **out) {
+  if (out == nullptr || count < 1 || entries == nullptr) return CLM_STATUS_INVALID_VALUE;
+  for (int i = 0; i < count; ++i) {
+    const clm_batch_entry &e = entries[i];
+    if (e.n < 1 || e.n > kMaxBatchN || e.lda < e.n || e.lde < e.n || e.A == nullptr ||
+        e.E == nullptr || !valid_dtype(e.dtype))
+      return CLM_STATUS_INVALID_VALUE;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CLM_STATUS_NOT_INITIALIZED;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (!g_tune_init) tune_defaults();
+  }
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int i) {
+    const double n = entries[i].n;
+    return n * n * n * (is_complex_code(entries[i].dtype) ? 4.0 : 1.0);
+  };
+  std::vector<int> cfg(count);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    for (int i = 0; i < count; ++i) {
+      const clm_batch_entry &e = entries[i];
+      cfg[i] = forced_cfg ? *forced_cfg
+                          : g_tune[precision_of(e.dtype)][is_complex_code(e.dtype)][bucket_of(e.n)];
+    }
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int ga = group_of(entries[a].dtype, cfg[a]), gb = group_of(entries[b].dtype, cfg[b]);
+    if (ga != gb) return ga < gb;
+    return cost(a) > cost(b);
+  });
+  auto *plan = new clm_batch_plan;
+  plan->device = dev;
+  plan->count = count;
+  plan->host_desc.resize(count);
+  size_t off = kHeadBytes;
+  for (int k = 0; k < count; ++k) {
+    const int i = order[k];
+    const clm_batch_entry &e = entries[i];
+    BatchDesc &d = plan->host_desc[k];
+    d.A = e.A;
+    d.E = e.E;
+    d.ws_off = static_cast<long long>(off);
+    d.n = e.n;
+    d.lda = e.lda;
+    d.lde = e.lde;
+    d.dtype = e.dtype;
+    d.index = i;
+    d.cfg = cfg[i];
+    off += Layout::align(ws_bytes_of(e.dtype, e.n));
+    plan->group_count[group_of(e.dtype, cfg[i])] += 1;
+  }
+  for (int g = 1; g < kGroups; ++g)
+    plan->group_start[g] = plan->group_start[g - 1] + plan->group_count[g - 1];
+  for (int g = 0; g < kGroups; ++g)
+    if (plan->group_count[g] > 0 && !prepare_group(g, dev)) {
+      delete plan;
+      return CLM_STATUS_EXECUTION_FAILED;
+    }
+  plan->ws_bytes = off;
+  if (hipMalloc(&plan->d_desc, sizeof(BatchDesc) * count) != hipSuccess) {
+    delete plan;
+    return CLM_STATUS_ALLOC_FAILED;
+  }
+  if (hipMemcpy(plan->d_desc, plan->host_desc.data(), sizeof(BatchDesc) * count,
+                hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(plan->d_desc);
+    delete plan;
+    return CLM_STATUS_EXECUTION_FAILED;
+  }
+  *out = plan;
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t launch_plan(clm_ctx *ctx, hipStream_t st, const clm_batch_plan *plan, void *work,
+                         size_t bytes, clm_batch_result *results) {
+  if (plan == nullptr || work == nullptr) return CLM_STATUS_INVALID_VALUE;
+  if (bytes < plan->ws_bytes) return CLM_STATUS_ALLOC_FAILED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  int *queue = static_cast<int *>(work);
+  char *base = static_cast<char *>(work);
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, queue);
+  CLM_TRY_HIP(hipGetLastError());
+  static constexpr int kTypes[4] = {CLM_DTYPE_F32, CLM_DTYPE_F64, CLM_DTYPE_C64, CLM_DTYPE_C128};
+  for (int g = 0; g < kGroups; ++g) {
+    const int count = plan->group_count[g];
+    if (count == 0) continue;
+    const BatchDesc *desc = plan->d_desc + plan->group_start[g];
+    const int cfg = g % 4;
+    hipError_t e;
+    switch (kTypes[g / 4]) {
+    case CLM_DTYPE_F32: e = launch_type<float>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    case CLM_DTYPE_F64: e = launch_type<double>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    case CLM_DTYPE_C64: e = launch_type<zc>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    default: e = launch_type<zd>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    }
+    CLM_TRY_HIP(e);
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
+void destroy_plan(clm_batch_plan *plan) {
+  if (!plan) return;
+  if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
+  if (plan->d_desc) (void)hipFree(plan->d_desc);
+  delete plan;
+}
+
+// one homogeneous tuning batch: `count` matrices of order n and type dtype, tile
+// configuration cfg; returns the median of `reps` timed launches in ms
+template <typename T>
+clm_status_t time_config(clm_ctx *ctx, hipStream_t st, int dtype, int n, int count, int cfg,
+                         int reps, float *ms) {
+  const size_t mat = static_cast<size_t>(n) * n * sizeof(T);
+  char *buf = nullptr;
+  if (hipMalloc(&buf, 2 * mat * count) != hipSuccess) return CLM_STATUS_ALLOC_FAILED;
+  std::vector<clm_batch_entry> entries(count);
+  for (int i = 0; i < count; ++i)
+    entries[i] = clm_batch_entry{n, dtype, buf + mat * i, n, buf + mat * (count + i), n};
+  hipLaunchKernelGGL(fill_tuning_matrices<T>, dim3(1024), dim3(256), 0, st,
+                     reinterpret_cast<T *>(buf), n, count, 12345u);
+  clm_batch_plan *plan = nullptr;
+  clm_status_t status = build_plan(count, entries.data(), &cfg, &plan);
+  void *work = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<float> times;
+  if (status == CLM_STATUS_SUCCESS && hipMalloc(&work, plan->ws_bytes) != hipSuccess)
+    status = CLM_STATUS_ALLOC_FAILED;
+  if (status == CLM_STATUS_SUCCESS &&
+      (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    status = CLM_STATUS_EXECUTION_FAILED;
+  for (int r = -1; status == CLM_STATUS_SUCCESS && r < reps; ++r) {
+    if (hipEventRecord(e0, st) != hipSuccess) status = CLM_STATUS_EXECUTION_FAILED;
+    if (status == CLM_STATUS_SUCCESS) status = launch_plan(ctx, st, plan, work, plan->ws_bytes, nullptr);
+    if (status == CLM_STATUS_SUCCESS &&
+        (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      status = CLM_STATUS_EXECUTION_FAILED;
+    float t = 0.f;
+    if (status == CLM_STATUS_SUCCESS && r >= 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+      times.push_back(t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (work) (void)hipFree(work);
+  destroy_plan(plan);
+  (void)hipFree(buf);
+  if (status != CLM_STATUS_SUCCESS) return status;
+  std::sort(times.begin(), times.end());
+  *ms = times.empty() ? 0.f : times[times.size() / 2];
+  return CLM_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace clm
+
+extern "C" {
+
+clm_status_t clm_expm_batched_plan_create(clm_batch_plan **plan, int count,
+                                          const clm_batch_entry *entries) {
+  return clm::build_plan(count, entries, nullptr, plan);
+}
+
+clm_status_t clm_expm_batched_plan_destroy(clm_batch_plan *plan) {
+  clm::destroy_plan(plan);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_batched_buffer_size(const clm_batch_plan *plan, size_t *bytes) {
+  if (plan == nullptr || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = plan->ws_bytes;
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_batched(clm_ctx *ctx, clm_stream_t stream, const clm_batch_plan *plan,
+                              void *dWork, size_t lwork_bytes, clm_batch_result *dResults) {
+  return clm::launch_plan(ctx, static_cast<hipStream_t>(stream), plan, dWork, lwork_bytes,
+                          dResults);
+}
+
+clm_status_t clm_expm_batched_graph(clm_ctx *ctx, clm_stream_t stream, clm_batch_plan *plan,
+                                    void *dWork, size_t lwork_bytes, clm_batch_result *dResults) {
+  if (plan == nullptr || dWork == nullptr || stream == nullptr) return CLM_STATUS_INVALID_VALUE;
+  if (lwork_bytes < plan->ws_bytes) return CLM_STATUS_ALLOC_FAILED;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool stale = plan->graph_exec == nullptr || plan->graph_stream != st ||
+                     plan->graph_work != dWork || plan->graph_results != dResults;
+  if (stale) {
+    if (plan->graph_exec) {
+      (void)hipGraphExecDestroy(plan->graph_exec);
+      plan->graph_exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    CLM_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const clm_status_t status = clm::launch_plan(ctx, st, plan, dWork, lwork_bytes, dResults);
+    const hipError_t end = hipStreamEndCapture(st, &graph);
+    if (status != CLM_STATUS_SUCCESS) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return status;
+    }
+    if (end != hipSuccess || graph == nullptr) return CLM_STATUS_EXECUTION_FAILED;
+    const hipError_t inst = hipGraphInstantiate(&plan->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+      plan->graph_exec = nullptr;
+      return CLM_STATUS_EXECUTION_FAILED;
+    }
+    plan->graph_stream = st;
+    plan->graph_work = dWork;
+    plan->graph_results = dResults;
+  }
+  CLM_TRY_HIP(hipGraphLaunch(plan->graph_exec, st));
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg) {
+  if (!clm::valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lock(clm::g_tune_mutex);
+  if (!clm::g_tune_init) clm::tune_defaults();
+  for (int b = 0; b < clm::kBuckets; ++b)
+    cfg[b] = clm::g_tune[clm::precision_of(dtype)][clm::is_complex_code(dtype)][b];
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg) {
+  if (!clm::valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
+  for (int b = 0; b < clm::kBuckets; ++b)
+    if (cfg[b] < CLM_BATCH_TILE_32 || cfg[b] > CLM_BATCH_TILE_128) return CLM_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lock(clm::g_tune_mutex);
+  if (!clm::g_tune_init) clm::tune_defaults();
+  for (int b = 0; b < clm::kBuckets; ++b)
+    clm::g_tune[clm::precision_of(dtype)][clm::is_complex_code(dtype)][b] = cfg[b];
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
+                                       int *cfg_out, float *ms_out) {
+  if (!clm::valid_dtype(dtype) || nmax < 1 || nmax > clm::kMaxBatchN)
+    return CLM_STATUS_INVALID_VALUE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int chosen[clm::kBuckets];
+  CLM_TRY(clm_expm_batched_get_tuning(dtype, chosen));
+  const bool single = clm::precision_of(dtype) == 0;
+  const bool cplx = clm::is_complex_code(dtype);
+  // candidate tiles: 128 x 128 exists for real FP32 only, 128 x 64 for all but
+  // complex FP64
+  int max_cfg = CLM_BATCH_TILE_128x64;
+  if (single && !cplx) max_cfg = CLM_BATCH_TILE_128;
+  if (!single && cplx) max_cfg = CLM_BATCH_TILE_64;
+  for (int b = 0; b < clm::kBuckets && 32 * b < nmax; ++b) {
+    const int n = std::min(32 * b + 16, nmax);
+    // enough matrices to keep every workgroup slot busy twice over
+    const int count = 1024;
+    float best = 0.f;
+    for (int cfg = CLM_BATCH_TILE_32; cfg <= max_cfg; ++cfg) {
+      float ms = 0.f;
+      clm_status_t s;
+      switch (dtype) {
+      case CLM_DTYPE_F32: s = clm::time_config<float>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
+      case CLM_DTYPE_F64: s = clm::time_config<double>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
+      case CLM_DTYPE_C64: s = clm::time_config<clm::zc>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
+      default: s = clm::time_config<clm::zd>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
+      }
+      if (s != CLM_STATUS_SUCCESS) return s;
+      if (ms_out) ms_out[b * 4 + cfg] = ms;
+      if (cfg == CLM_BATCH_TILE_32 || ms < best) {
+        best = ms;
+        chosen[b] = cfg;
+      }
+    }
+  }
+  if (cfg_out)
+    for (int b = 0; b < clm::kBuckets; ++b) cfg_out[b] = chosen[b];
+  return clm_expm_batched_set_tuning(dtype, chosen);
+}
+
+}  // extern "C"

@@ -20,27 +20,13 @@
 #include <type_traits>
 
 #include "clm_internal.hpp"
-
-struct clm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-};
+#include "expm_plan.hpp"
 
 namespace clm {
 namespace {
-
-#define CLM_TRY(expr)                                             \
-  do {                                                            \
-    const clm_status_t clm_try_status_ = (expr);                  \
-    if (clm_try_status_ != CLM_STATUS_SUCCESS) return clm_try_status_; \
-  } while (0)
-#define CLM_TRY_HIP(expr)                                         \
-  do {                                                            \
-    if ((expr) != hipSuccess) return CLM_STATUS_EXECUTION_FAILED; \
-  } while (0)
-
 std::mutex g_ctx_mutex;
 clm_ctx *g_default_ctx[64] = {};
+}  // namespace
 
 clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
   if (ctx) {
@@ -59,6 +45,8 @@ clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
   *out = g_default_ctx[dev];
   return CLM_STATUS_SUCCESS;
 }
+
+namespace {
 
 template <typename T>
 constexpr int dtype_code() {
@@ -89,43 +77,6 @@ hipError_t launch_prec(hipStream_t st, const GemmArgs<T> &g, int prec) {
   return launch_gemm(st, g);
 }
 
-// ---- Paterson-Stockmeyer plan (src/paterson_stockmeyer/interface.cppm:117-210) ----
-int ps_cost(int degree, int s) {
-  if (s < 1 || degree < 1) return 0;
-  const int r = degree / s;
-  const int max_power = (r >= 1) ? s : degree;
-  const int num_powers = (max_power >= 2) ? (max_power - 1) : 0;
-  return num_powers + r;
-}
-
-int ps_block_size(int degree) {
-  if (degree < 1) return 1;
-  int best_s = 1, best = ps_cost(degree, 1);
-  for (int s = 2; s <= degree + 1; ++s) {
-    const int c = ps_cost(degree, s);
-    if (c < best) {
-      best = c;
-      best_s = s;
-    }
-  }
-  return best_s;
-}
-
-clm_ps_plan ps_plan(int degree, int s_requested) {
-  clm_ps_plan p{1, 0, 0, 0, 0};
-  if (degree < 0) return p;
-  int s = (s_requested > 0) ? s_requested : ps_block_size(degree);
-  if (s > degree + 1) s = degree + 1;
-  if (s < 1) s = 1;
-  p.s = s;
-  p.r = degree / s;
-  const int max_power = (p.r >= 1) ? s : degree;
-  p.num_powers = (max_power >= 2) ? (max_power - 1) : 0;
-  p.num_blocks = p.num_powers + ((p.r >= 1) ? 1 : 0);
-  p.num_gemms = p.num_powers + p.r;
-  return p;
-}
-
 template <typename T>
 size_t power_stride(int n) {
   const size_t order = static_cast<size_t>(n < 1 ? 1 : n);
@@ -142,65 +93,12 @@ size_t ps_buffer_size(int n, int degree, int s_requested) {
   return layout.total();
 }
 
-// ---- Taylor ladder -----------------------------------------------------------------------
-// theta_m: the largest x with sum_{k>m} |h_k| x^(k-1) <= 2^-53 for
-// h(x) = log(exp(-x) T_m(x)) -- the backward-error criterion the reference
-// applies to its Pade rungs (src/expm/plan.cppm:29-41), evaluated by an exact
-// rational series and bisection. The degrees are those where the
-// Paterson-Stockmeyer product count rises by one.
-constexpr int kNumTaylor = 10;
-constexpr int kTaylorDegrees[kNumTaylor] = {1, 2, 3, 5, 8, 11, 15, 19, 24, 29};
-constexpr double kTaylorTheta[kNumTaylor] = {
-    2.220446049250313e-16, 2.580956802971767e-8, 1.386347866119121e-5, 2.400876357887274e-3,
-    4.991228871115323e-2,  2.142358068451711e-1, 6.410835233041199e-1, 1.260381060642639,
-    2.21904886936509,      3.310172839890271};
-
-// the same thresholds at u = 2^-24 (the precision variant's accuracy target)
-constexpr double kTaylorThetaS[kNumTaylor] = {
-    1.192092800768788e-7, 5.978858893805233e-4, 1.123386473528671e-2, 1.30848716459947e-1,
-    5.800524627688768e-1, 1.223479542424143,    2.21704439497472,     3.279561212635997,
-    4.649782224100757,    6.041358758192571};
-
-// the largest power bank any ladder degree needs
-int max_bank_blocks() {
-  int mx = 0;
-  for (int i = 0; i < kNumTaylor; ++i) {
-    const int np = ps_plan(kTaylorDegrees[i], 0).num_powers;
-    if (np > mx) mx = np;
-  }
-  return mx;
-}
-
-clm_expm_plan taylor_plan(double norm1, const double *theta = kTaylorTheta) {
-  int best = -1, best_cost = 0, best_s = 0;
-  for (int i = 0; i < kNumTaylor; ++i) {
-    int s = 0;
-    if (norm1 > theta[i]) {
-      s = static_cast<int>(std::ceil(std::log2(norm1 / theta[i])));
-      if (s < 0) s = 0;
-    }
-    const int cost = ps_plan(kTaylorDegrees[i], 0).num_gemms + s;
-    if (best < 0 || cost < best_cost || (cost == best_cost && s < best_s)) {
-      best = i;
-      best_cost = cost;
-      best_s = s;
-    }
-  }
-  clm_expm_plan p{};
-  p.m = kTaylorDegrees[best];
-  p.s = best_s;
-  p.num_gemms = best_cost;
-  p.method = CLM_METHOD_TAYLOR_PS;
-  p.ps_block = ps_plan(p.m, 0).s;
-  p.dtype = 0;
-  p.norm1 = norm1;
-  return p;
-}
-
 // The precision variant's schedule. Degree from the 2^-24 thresholds; polynomial
 // products split 3 ways while s <= 3 (their error is amplified 2^s by the
-// squarings), 6 ways beyond; only the last squaring(s) split, 6 ways -- an error
-// made in squaring k of s is amplified 2^(s-k), so the early ones stay FP64. The
+// squarings), 6 ways beyond; only the last squaring splits, 6 ways -- an error
+// made in squaring k of s is amplified 2^(s-k), so the early ones stay FP64 --
+// and with s > 8 every squaring stays FP64 (the squaring phase then dominates
+// the error budget). The
 // degree minimises the products weighted by their cost relative to an FP64
 // product (3-way split 0.2, 6-way 0.35), so cheap BF16 polynomial products are
 // traded for expensive FP64 squarings.
@@ -208,7 +106,7 @@ struct SplitChoice {
   int ps_split, split_squarings;
 };
 SplitChoice split_schedule(int s, const clm_split_opts *opts) {
-  SplitChoice c{(s <= 3) ? CLM_SPLIT_BF16X3 : CLM_SPLIT_BF16X6, (s >= 1) ? 1 : 0};
+  SplitChoice c{(s <= 3) ? CLM_SPLIT_BF16X3 : CLM_SPLIT_BF16X6, (s >= 1 && s <= 8) ? 1 : 0};
   if (opts && (opts->ps_split == CLM_SPLIT_BF16X3 || opts->ps_split == CLM_SPLIT_BF16X6))
     c.ps_split = opts->ps_split;
   if (opts && opts->split_squarings >= 0)
@@ -221,13 +119,13 @@ clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
   double best_cost = 0.0;
   for (int i = 0; i < kNumTaylor; ++i) {
     int s = 0;
-    if (norm1 > kTaylorThetaS[i]) {
-      s = static_cast<int>(std::ceil(std::log2(norm1 / kTaylorThetaS[i])));
+    if (norm1 > taylor_theta_at(i, true)) {
+      s = static_cast<int>(std::ceil(std::log2(norm1 / taylor_theta_at(i, true))));
       if (s < 0) s = 0;
     }
     const SplitChoice c = split_schedule(s, opts);
     const double w_ps = (c.ps_split == CLM_SPLIT_BF16X3) ? 0.2 : 0.35;
-    const double cost = ps_plan(kTaylorDegrees[i], 0).num_gemms * w_ps +
+    const double cost = ps_plan(taylor_degree(i), 0).num_gemms * w_ps +
                         (s - c.split_squarings) * 1.0 + c.split_squarings * 0.35;
     if (best < 0 || cost < best_cost - 1e-12 || (std::fabs(cost - best_cost) <= 1e-12 && s < best_s)) {
       best = i;
@@ -236,7 +134,7 @@ clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
     }
   }
   clm_expm_plan p{};
-  p.m = kTaylorDegrees[best];
+  p.m = taylor_degree(best);
   p.s = best_s;
   p.num_gemms = ps_plan(p.m, 0).num_gemms + best_s;
   p.method = CLM_METHOD_TAYLOR_PS;
@@ -547,9 +445,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
 
   // the degree ladder at the element type's unit roundoff (2^-53 or 2^-24)
-  clm_expm_plan p = use_split      ? split_plan(norm, split)
-                    : is_single<T>() ? taylor_plan(norm, kTaylorThetaS)
-                                     : taylor_plan(norm);
+  clm_expm_plan p = use_split ? split_plan(norm, split) : taylor_plan(norm, is_single<T>());
   p.dtype = dtype_code<T>();
   if (plan) *plan = p;
 
@@ -830,7 +726,7 @@ clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream) {
   return CLM_STATUS_SUCCESS;
 }
 
-clm_expm_plan clm_expm_plan_d(double norm1) { return clm::taylor_plan(norm1); }
+clm_expm_plan clm_expm_plan_d(double norm1) { return clm::taylor_plan(norm1, false); }
 
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts) {
   return clm::split_plan(norm1, opts);
@@ -838,7 +734,7 @@ clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts) {
 
 double clm_taylor_theta_s(int m) {
   for (int i = 0; i < clm::kNumTaylor; ++i)
-    if (clm::kTaylorDegrees[i] == m) return clm::kTaylorThetaS[i];
+    if (clm::taylor_degree(i) == m) return clm::taylor_theta_at(i, true);
   return 0.0;
 }
 
@@ -861,7 +757,7 @@ clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const do
 
 double clm_taylor_theta_d(int m) {
   for (int i = 0; i < clm::kNumTaylor; ++i)
-    if (clm::kTaylorDegrees[i] == m) return clm::kTaylorTheta[i];
+    if (clm::taylor_degree(i) == m) return clm::taylor_theta_at(i, false);
   return 0.0;
 }
 
@@ -995,9 +891,7 @@ clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex al
 }
 
 // ---- single precision: float and complex<float> ---------------------------------------
-clm_expm_plan clm_expm_plan_s(double norm1) {
-  return clm::taylor_plan(norm1, clm::kTaylorThetaS);
-}
+clm_expm_plan clm_expm_plan_s(double norm1) { return clm::taylor_plan(norm1, true); }
 double clm_pade_theta_s(int m) { return clm::pade_theta_single(m); }
 clm_expm_plan clm_pade_plan_s(float norm1) { return clm::pade_plan_single(norm1); }
 

@@ -9,7 +9,26 @@
 
 #include "calaman_expm.h"
 
+// A context: the device it was created on and the stream it enqueues on.
+struct clm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+#define CLM_TRY(expr)                                                  \
+  do {                                                                 \
+    const clm_status_t clm_try_status_ = (expr);                       \
+    if (clm_try_status_ != CLM_STATUS_SUCCESS) return clm_try_status_; \
+  } while (0)
+#define CLM_TRY_HIP(expr)                                         \
+  do {                                                            \
+    if ((expr) != hipSuccess) return CLM_STATUS_EXECUTION_FAILED; \
+  } while (0)
+
 namespace clm {
+
+// ctx, or the library's default context of the current device when ctx is null
+clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out);
 
 // n*n must fit an int in the reference's flat-index kernels (src/expm/detail.cppm:25-28);
 // the bound is kept so the two libraries accept the same orders.

@@ -1,0 +1,287 @@
+// gemm_tile.hpp -- the MFMA tile engine shared by the stand-alone product
+// kernels (gemm.hip) and the batched persistent expm kernel (batched.hip):
+// MFMA traits per element type, LDS staging layouts, the global -> register
+// fetch of a K slice, its register -> LDS store, and the MFMA compute of one
+// slice. A kernel composes them into its own loop; both kernels use the same
+// helpers, so one output element accumulates its k terms in the same order
+// (ascending, four per MFMA step) whichever kernel computes it.
+//
+// MFMA operand layout (16x16x4): A lane l holds A[l%16][l/16], B lane l holds
+// B[l/16][l%16]; D lane l holds column l%16 at rows l/16 + 4r for the FP64
+// instruction and rows 4(l/16) + r for the FP32 one (r = 0..3).
+#pragma once
+
+#include "clm_internal.hpp"
+
+namespace clm {
+namespace tile {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBK = 16;
+
+template <typename S>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc = d4;
+  using vec = double2;  // one 16-byte access
+  static constexpr int EV = 2;
+  __device__ static acc zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <>
+struct Mfma<float> {
+  using acc = f4;
+  using vec = float4;
+  static constexpr int EV = 4;
+  __device__ static acc zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
+  const int total = tiles_m * tiles_n;
+  int t = bid;
+  if ((total & 7) == 0) t = (bid & 7) * (total >> 3) + (bid >> 3);
+  constexpr int G = 4;
+  const int per_group = G * tiles_n;
+  const int group = t / per_group;
+  const int first_m = group * G;
+  const int gsize = min(tiles_m - first_m, G);
+  const int r = t - group * per_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+template <typename T>
+__device__ __forceinline__ void epilogue_store(const GemmArgs<T> &g, int row, int col, T acc) {
+  T v = ops<T>::mul(g.alpha, acc);
+  if (g.Cin) v = ops<T>::add(v, ops<T>::mul(g.beta, g.Cin[row + (long)col * g.ldcin]));
+  if (g.nterms > 0) {
+    T e = (row == col) ? g.coef[0] : ops<T>::zero();
+    if (g.nterms > 1) e = ops<T>::mad(g.coef[1], g.X1[row + (long)col * g.ldx1], e);
+    for (int t = 2; t < g.nterms; ++t)
+      e = ops<T>::mad(g.coef[t], g.bank[(long)(t - 2) * g.bank_stride + row + (long)col * g.ldbank],
+                      e);
+    v = ops<T>::add(v, e);
+  }
+  g.C[row + (long)col * g.ldc] = v;
+}
+
+// ------------------------------------------------------------------------------------
+// real (S = double or float)
+// ------------------------------------------------------------------------------------
+template <typename S, int BM, int BN, int NT, int BK>
+struct RTile {
+  static constexpr int EV = Mfma<S>::EV;
+  static constexpr int LDA_S = BM + 16;  // As[k][m]
+  static constexpr int LDB_S = BK + EV;  // Bs[n][k]
+  static constexpr int A_V = BM * BK / EV / NT;  // 16-byte vectors per lane per slice
+  static constexpr int B_V = BN * BK / EV / NT;
+  static constexpr int A_STAGE = BK * LDA_S;
+  static constexpr int B_STAGE = BN * LDB_S;
+  static_assert(A_V >= 1 && B_V >= 1 && (BM * BK / EV) % NT == 0 && (BN * BK / EV) % NT == 0,
+                "tile staging must split evenly over the lanes");
+};
+
+template <typename S, int BM, int BN, int NT, int BK, bool kFast>
+__device__ __forceinline__ void rgemm_gload(
+    const GemmArgs<S> &g, int m0, int n0, int k0, int tid,
+    typename Mfma<S>::vec (&ra)[RTile<S, BM, BN, NT, BK>::A_V],
+    typename Mfma<S>::vec (&rb)[RTile<S, BM, BN, NT, BK>::B_V]) {
+  using TL = RTile<S, BM, BN, NT, BK>;
+  using V = typename Mfma<S>::vec;
+  constexpr int EV = TL::EV;
+#pragma unroll
+  for (int it = 0; it < TL::A_V; ++it) {
+    const int idx = tid + it * NT;
+    const int mm = (idx % (BM / EV)) * EV, k = idx / (BM / EV);
+    if constexpr (kFast) {
+      ra[it] = *reinterpret_cast<const V *>(g.A + (m0 + mm) + (long)(k0 + k) * g.lda);
+    } else {
+      const int gm = m0 + mm, gk = k0 + k;
+      S v[EV];
+#pragma unroll
+      for (int e = 0; e < EV; ++e)
+        v[e] = (gk < g.K && gm + e < g.M) ? g.A[gm + e + (long)gk * g.lda] : S(0);
+      ra[it] = *reinterpret_cast<const V *>(v);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_V; ++it) {
+    const int idx = tid + it * NT;
+    const int kk = (idx % (BK / EV)) * EV, nn = idx / (BK / EV);
+    if constexpr (kFast) {
+      rb[it] = *reinterpret_cast<const V *>(g.B + (k0 + kk) + (long)(n0 + nn) * g.ldb);
+    } else {
+      const int gk = k0 + kk, gn = n0 + nn;
+      S v[EV];
+#pragma unroll
+      for (int e = 0; e < EV; ++e)
+        v[e] = (gn < g.N && gk + e < g.K) ? g.B[gk + e + (long)gn * g.ldb] : S(0);
+      rb[it] = *reinterpret_cast<const V *>(v);
+    }
+  }
+}
+
+template <typename S, int BM, int BN, int NT, int BK>
+__device__ __forceinline__ void rgemm_sstore(
+    S *As, S *Bs, int tid, const typename Mfma<S>::vec (&ra)[RTile<S, BM, BN, NT, BK>::A_V],
+    const typename Mfma<S>::vec (&rb)[RTile<S, BM, BN, NT, BK>::B_V]) {
+  using TL = RTile<S, BM, BN, NT, BK>;
+  using V = typename Mfma<S>::vec;
+  constexpr int EV = TL::EV;
+#pragma unroll
+  for (int it = 0; it < TL::A_V; ++it) {
+    const int idx = tid + it * NT;
+    const int mm = (idx % (BM / EV)) * EV, k = idx / (BM / EV);
+    *reinterpret_cast<V *>(&As[k * TL::LDA_S + mm]) = ra[it];
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_V; ++it) {
+    const int idx = tid + it * NT;
+    const int kk = (idx % (BK / EV)) * EV, nn = idx / (BK / EV);
+    *reinterpret_cast<V *>(&Bs[nn * TL::LDB_S + kk]) = rb[it];
+  }
+}
+
+template <typename S, int BM, int BN, int WM, int WN, int NT, int BK>
+__device__ __forceinline__ void rgemm_compute(const S *As, const S *Bs, int lane, int wm0, int wn0,
+                                              typename Mfma<S>::acc (&acc)[WM / 16][WN / 16]) {
+  using TL = RTile<S, BM, BN, NT, BK>;
+  constexpr int TM = WM / 16, TN = WN / 16;
+#pragma unroll
+  for (int kk = 0; kk < BK; kk += 4) {
+    const int kr = kk + (lane >> 4);
+    S a[TM], b[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) a[i] = As[kr * TL::LDA_S + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) b[j] = Bs[(wn0 + j * 16 + (lane & 15)) * TL::LDB_S + kr];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<S>::mma(a[i], b[j], acc[i][j]);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// complex (C = zd or zc): four real MFMA products per 16x16 tile and k-step,
+//   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br,
+// with real and imaginary planes staged separately in LDS, double-buffered.
+// ------------------------------------------------------------------------------------
+template <typename C>
+struct CxReal;
+template <>
+struct CxReal<zd> {
+  using type = double;
+};
+template <>
+struct CxReal<zc> {
+  using type = float;
+};
+
+template <typename C, int BM, int BN, int NT, int BK>
+struct CTile {
+  using S = typename CxReal<C>::type;
+  static constexpr int LDA_S = BM + 16;
+  static constexpr int LDB_S = BK + Mfma<S>::EV;
+  static constexpr int A_E = BM * BK / NT;  // complex elements per lane per slice
+  static constexpr int B_E = BN * BK / NT;
+  static constexpr int A_PLANE = BK * LDA_S;
+  static constexpr int B_PLANE = BN * LDB_S;
+  static constexpr int STAGE = 2 * (A_PLANE + B_PLANE);  // re/im planes of A and B
+  static_assert(A_E >= 1 && B_E >= 1 && (BM * BK) % NT == 0 && (BN * BK) % NT == 0,
+                "tile staging must split evenly over the lanes");
+};
+
+template <typename C, int BM, int BN, int NT, int BK, bool kFast>
+__device__ __forceinline__ void cgemm_gload(const GemmArgs<C> &g, int m0, int n0, int k0, int tid,
+                                            C (&ra)[CTile<C, BM, BN, NT, BK>::A_E],
+                                            C (&rb)[CTile<C, BM, BN, NT, BK>::B_E]) {
+  using TL = CTile<C, BM, BN, NT, BK>;
+#pragma unroll
+  for (int it = 0; it < TL::A_E; ++it) {
+    const int e = tid + it * NT;
+    const int gm = m0 + e % BM, gk = k0 + e / BM;
+    if (kFast || (gm < g.M && gk < g.K)) ra[it] = g.A[gm + (long)gk * g.lda];
+    else ra[it] = ops<C>::zero();
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_E; ++it) {
+    const int e = tid + it * NT;
+    const int gk = k0 + e % BK, gn = n0 + e / BK;
+    if (kFast || (gk < g.K && gn < g.N)) rb[it] = g.B[gk + (long)gn * g.ldb];
+    else rb[it] = ops<C>::zero();
+  }
+}
+
+template <typename C, int BM, int BN, int NT, int BK>
+__device__ __forceinline__ void cgemm_sstore(typename CxReal<C>::type *L, int tid,
+                                             const C (&ra)[CTile<C, BM, BN, NT, BK>::A_E],
+                                             const C (&rb)[CTile<C, BM, BN, NT, BK>::B_E]) {
+  using TL = CTile<C, BM, BN, NT, BK>;
+#pragma unroll
+  for (int it = 0; it < TL::A_E; ++it) {
+    const int e = tid + it * NT;
+    const int s = (e / BM) * TL::LDA_S + (e % BM);
+    L[s] = ra[it].re;
+    L[TL::A_PLANE + s] = ra[it].im;
+  }
+#pragma unroll
+  for (int it = 0; it < TL::B_E; ++it) {
+    const int e = tid + it * NT;
+    const int s = (e / BK) * TL::LDB_S + (e % BK);
+    L[2 * TL::A_PLANE + s] = rb[it].re;
+    L[2 * TL::A_PLANE + TL::B_PLANE + s] = rb[it].im;
+  }
+}
+
+template <typename C, int BM, int BN, int WM, int WN, int NT, int BK>
+__device__ __forceinline__ void cgemm_compute(
+    const typename CxReal<C>::type *L, int lane, int wm0, int wn0,
+    typename Mfma<typename CxReal<C>::type>::acc (&cr)[WM / 16][WN / 16],
+    typename Mfma<typename CxReal<C>::type>::acc (&ci)[WM / 16][WN / 16]) {
+  using TL = CTile<C, BM, BN, NT, BK>;
+  using S = typename CxReal<C>::type;
+  constexpr int TM = WM / 16, TN = WN / 16;
+#pragma unroll
+  for (int kk = 0; kk < BK; kk += 4) {
+    const int kr = kk + (lane >> 4);
+    S ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int s = kr * TL::LDA_S + wm0 + i * 16 + (lane & 15);
+      ar[i] = L[s];
+      ai[i] = L[TL::A_PLANE + s];
+      nai[i] = -ai[i];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int s = (wn0 + j * 16 + (lane & 15)) * TL::LDB_S + kr;
+      br[j] = L[2 * TL::A_PLANE + s];
+      bi[j] = L[2 * TL::A_PLANE + TL::B_PLANE + s];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        cr[i][j] = Mfma<S>::mma(ar[i], br[j], cr[i][j]);
+        ci[i][j] = Mfma<S>::mma(ar[i], bi[j], ci[i][j]);
+        cr[i][j] = Mfma<S>::mma(nai[i], bi[j], cr[i][j]);
+        ci[i][j] = Mfma<S>::mma(ai[i], br[j], ci[i][j]);
+      }
+  }
+}
+
+}  // namespace tile
+}  // namespace clm

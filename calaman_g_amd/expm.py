@@ -271,6 +271,107 @@ def gemm_cm(ctx: Context, m: int, n: int, k: int, alpha, dA: torch.Tensor, lda: 
                                                   ldb, b, _ptr(dC), ldc))
 
 
+# ---- batched expm -----------------------------------------------------------------------------
+_DTYPE_CODE = {_F32: _lib.DTYPE_F32, _F64: _lib.DTYPE_F64, _C64: _lib.DTYPE_C64,
+               _C128: _lib.DTYPE_C128}
+
+
+class BatchPlan:
+    """clm_batch_plan over fixed column-major device buffers: ``entries`` is a list
+    of (n, A, lda, E, lde) with A and E torch tensors (storage of the column-major
+    matrices; element type from A.dtype). The plan keeps references to the
+    tensors, so they stay alive while it can launch."""
+
+    def __init__(self, entries):
+        self._keep = []
+        arr = (_lib.BatchEntry * len(entries))()
+        for i, (n, a, lda, e, lde) in enumerate(entries):
+            _require_device(a, "A")
+            if e.dtype != a.dtype:
+                raise TypeError("A and E differ in dtype")
+            arr[i] = _lib.BatchEntry(int(n), _DTYPE_CODE[a.dtype], a.data_ptr(), int(lda),
+                                     e.data_ptr(), int(lde))
+            self._keep += [a, e]
+        self.count = len(entries)
+        h = ctypes.c_void_p()
+        check(lib().clm_expm_batched_plan_create(ctypes.byref(h), len(entries), arr),
+              "clm_expm_batched_plan_create")
+        self.handle = h
+        nbytes = ctypes.c_size_t(0)
+        check(lib().clm_expm_batched_buffer_size(h, ctypes.byref(nbytes)), "buffer_size")
+        self.buffer_size = int(nbytes.value)
+        self.device = entries[0][1].device if entries else None
+
+    def workspace(self) -> torch.Tensor:
+        return torch.empty(self.buffer_size, dtype=torch.uint8, device=self.device)
+
+    def results_buffer(self) -> torch.Tensor:
+        """device storage for count clm_batch_result records (24 bytes each)"""
+        return torch.zeros(self.count * ctypes.sizeof(_lib.BatchResult), dtype=torch.uint8,
+                           device=self.device)
+
+    @staticmethod
+    def decode_results(buf: torch.Tensor) -> list:
+        raw = buf.cpu().numpy().tobytes()
+        size = ctypes.sizeof(_lib.BatchResult)
+        out = []
+        for i in range(len(raw) // size):
+            r = _lib.BatchResult.from_buffer_copy(raw[i * size:(i + 1) * size])
+            out.append({f: getattr(r, f) for f, _ in r._fields_})
+        return out
+
+    def launch(self, work: torch.Tensor, results: Optional[torch.Tensor] = None,
+               stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None,
+               graph: bool = False) -> int:
+        f = lib().clm_expm_batched_graph if graph else lib().clm_expm_batched
+        return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), self.handle,
+                     _ptr(work), work.numel(), _ptr(results)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().clm_expm_batched_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batched_tuning(dtype: torch.dtype) -> list:
+    arr = (ctypes.c_int * _lib.BATCH_BUCKETS)()
+    check(lib().clm_expm_batched_get_tuning(_DTYPE_CODE[dtype], arr), "get_tuning")
+    return list(arr)
+
+
+def set_batched_tuning(dtype: torch.dtype, cfg) -> None:
+    arr = (ctypes.c_int * _lib.BATCH_BUCKETS)(*cfg)
+    check(lib().clm_expm_batched_set_tuning(_DTYPE_CODE[dtype], arr), "set_tuning")
+
+
+def batched_autotune(dtype: torch.dtype, nmax: int = 256,
+                     stream: Optional[torch.cuda.Stream] = None) -> tuple:
+    """Times every workgroup tile per order bucket up to nmax; returns (table, ms[bucket][tile])."""
+    cfg = (ctypes.c_int * _lib.BATCH_BUCKETS)()
+    ms = (ctypes.c_float * (4 * _lib.BATCH_BUCKETS))()
+    check(lib().clm_expm_batched_autotune(None, ctypes.c_void_p(_stream_ptr(stream)),
+                                          _DTYPE_CODE[dtype], int(nmax), cfg, ms), "autotune")
+    return list(cfg), [list(ms[4 * b:4 * b + 4]) for b in range(_lib.BATCH_BUCKETS)]
+
+
+def expm_batched(mats, stream: Optional[torch.cuda.Stream] = None):
+    """exp of every square device matrix in ``mats`` (mixed orders and element types)
+    in one batched launch; returns the list of results (row-major, like expm())."""
+    cms = [to_cm(a) for a in mats]
+    outs = [torch.empty_like(c) for c in cms]
+    plan = BatchPlan([(a.shape[0], c, a.shape[0], e, a.shape[0]) for a, c, e in zip(mats, cms, outs)])
+    work = plan.workspace()
+    check(plan.launch(work, stream=stream), "expm_batched")
+    torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+    return [e.t() for e in outs]
+
+
 # ---- convenience layer (row-major torch matrices) ----------------------------------------------
 def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
          return_plan: bool = False):
@@ -378,5 +479,6 @@ __all__ = [
     "pade_theta", "pade_plan", "matrix_norm1", "matrix_norm1_cm", "lange", "lange_cm", "lascl",
     "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",
-    "horner_buffer_size", "to_cm", "from_cm",
+    "horner_buffer_size", "to_cm", "from_cm", "BatchPlan", "expm_batched", "batched_tuning",
+    "set_batched_tuning", "batched_autotune",
 ]

@@ -165,13 +165,14 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
  * (CLM_SPLIT_BF16X3: 2 terms, 3 products; CLM_SPLIT_BF16X6: 3 terms, 6 products).
  * The 1-norm, the plan and the 2^-s scaling stay FP64; the degree comes from the
  * Taylor thresholds at u = 2^-24; the squarings run FP64 except the last
- * `split_squarings`, whose errors the remaining squarings amplify least. Target:
+ * `split_squarings`, whose errors the remaining squarings amplify least (none
+ * when s > 8). Target:
  * relative error <= 1e-6 against FP64. */
 #define CLM_SPLIT_BF16X3 3
 #define CLM_SPLIT_BF16X6 6
 typedef struct clm_split_opts {
   int ps_split;        /* 0 = auto (X3 when s <= 3, else X6), or CLM_SPLIT_BF16X3 / X6 */
-  int split_squarings; /* -1 = auto (1 when s >= 1), else 0..s */
+  int split_squarings; /* -1 = auto (1 when 1 <= s <= 8, else 0), else 0..s */
 } clm_split_opts;
 /* the plan clm_expm_split_d executes for a 1-norm under opts (NULL = auto) */
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts);
@@ -182,6 +183,69 @@ clm_status_t clm_expm_split_buffer_size_d(int n, size_t *bytes);
 clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
                               double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
                               const clm_split_opts *opts, clm_expm_plan *plan);
+
+/* ---- batched expm: many orders and types in one persistent launch ------------------
+ * No reference counterpart (the reference runs one expm per call). Every matrix
+ * of a batch goes through the algorithm of clm_expm_{s,d,c,z} -- the same
+ * 1-norm, plan, scaling, Paterson-Stockmeyer products and squarings -- and its
+ * result agrees bit for bit with the single-matrix call on the same input. One
+ * workgroup runs one matrix's whole chain; a persistent grid takes the matrices
+ * largest first. The plan is chosen on the device, nothing is read back, so the
+ * launch can be captured in a hipGraph (clm_expm_batched_graph does so).
+ *
+ * Usage: create a plan from a HOST array of entries (device pointers inside),
+ * size and allocate the workspace, then launch as often as wanted; the plan
+ * holds the entries, so changing a pointer or an order means a new plan. */
+typedef struct clm_batch_entry {
+  int n;         /* order, 1..4096 */
+  int dtype;     /* CLM_DTYPE_F32 / F64 / C64 / C128 */
+  const void *A; /* device, column-major, leading dimension lda >= n */
+  int lda;
+  void *E;       /* device output, leading dimension lde >= n; must not overlap A */
+  int lde;
+} clm_batch_entry;
+/* per-matrix report, indexed like the entries: the executed plan, or status
+ * CLM_STATUS_INVALID_VALUE (and E untouched) for a non-finite 1-norm */
+typedef struct clm_batch_result {
+  int m, s, num_gemms;
+  int status;
+  double norm1;
+} clm_batch_result;
+typedef struct clm_batch_plan clm_batch_plan;
+
+/* workgroup tile of the per-matrix products, chosen per order bucket
+ * (bucket b: n in (32b, 32(b+1)], the last bucket everything above) */
+#define CLM_BATCH_BUCKETS 8
+#define CLM_BATCH_TILE_32 0     /* 32 x 32 */
+#define CLM_BATCH_TILE_64 1     /* 64 x 64 */
+#define CLM_BATCH_TILE_128x64 2 /* 128 x 64 (complex FP64: 64 x 64) */
+#define CLM_BATCH_TILE_128 3    /* 128 x 128 (real FP32 only; others: their largest) */
+
+/* Validates the entries (CLM_STATUS_INVALID_VALUE before any device call),
+ * orders them by cost, takes each one's tile from the tuning table, and copies
+ * the descriptors to the current device. */
+clm_status_t clm_expm_batched_plan_create(clm_batch_plan **plan, int count,
+                                          const clm_batch_entry *entries);
+clm_status_t clm_expm_batched_plan_destroy(clm_batch_plan *plan);
+/* device workspace bytes for one launch of the plan */
+clm_status_t clm_expm_batched_buffer_size(const clm_batch_plan *plan, size_t *bytes);
+/* One launch per precision present (FP32 pair, FP64 pair) on `stream`, plus a
+ * one-lane queue reset; asynchronous; dResults optional (device array of count). */
+clm_status_t clm_expm_batched(clm_ctx *ctx, clm_stream_t stream, const clm_batch_plan *plan,
+                              void *dWork, size_t lwork_bytes, clm_batch_result *dResults);
+/* The same launch captured once into a hipGraph (per stream, workspace and
+ * results pointer) and replayed; stream must not be the null stream. */
+clm_status_t clm_expm_batched_graph(clm_ctx *ctx, clm_stream_t stream, clm_batch_plan *plan,
+                                    void *dWork, size_t lwork_bytes, clm_batch_result *dResults);
+/* The tuning table (CLM_BATCH_BUCKETS tile ids per element type) and the tuner
+ * that fills it: for each bucket up to nmax it times a batch of 1024 matrices
+ * of the bucket's middle order under every tile and keeps the fastest. ms_out
+ * (optional, 4 * CLM_BATCH_BUCKETS floats) receives ms[bucket * 4 + tile].
+ * Plans created afterwards use the new table. */
+clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg);
+clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg);
+clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
+                                       int *cfg_out, float *ms_out);
 
 /* ---- 1-norm --------------------------------------------------------------------
  * matrix_norm1<T> (src/expm/norm1.cppm:45-56): max_j sum_i |A(i,j)| with the

@@ -175,3 +175,36 @@ def test_lascl_lange_gemm_argument_checks():
     assert L.clm_matrix_norm1_s(None, 0, pa, 4, pa, ctypes.byref(ctypes.c_float())) == ST.INVALID_VALUE
     one = _lib.FloatComplex(1.0, 0.0)
     assert L.clm_gemm_c(None, 4, 4, 4, one, pa, 4, pa, 3, one, pa, 4) == ST.INVALID_VALUE
+
+
+def test_batched_plan_rejects_bad_entries_before_device():
+    h = ctypes.c_void_p()
+    e = (_lib.BatchEntry * 2)()
+    assert L.clm_expm_batched_plan_create(ctypes.byref(h), 0, e) == ST.INVALID_VALUE
+    assert L.clm_expm_batched_plan_create(None, 1, e) == ST.INVALID_VALUE
+    good = _lib.BatchEntry(4, ST.DTYPE_F32, 4096, 4, 8192, 4)
+    for bad in (_lib.BatchEntry(0, ST.DTYPE_F32, 4096, 4, 8192, 4),      # n < 1
+                _lib.BatchEntry(4097, ST.DTYPE_F32, 4096, 4097, 8192, 4097),  # n too large
+                _lib.BatchEntry(4, ST.DTYPE_F32, 4096, 3, 8192, 4),      # lda < n
+                _lib.BatchEntry(4, ST.DTYPE_F32, 4096, 4, 8192, 3),      # lde < n
+                _lib.BatchEntry(4, ST.DTYPE_F32, None, 4, 8192, 4),      # no A
+                _lib.BatchEntry(4, 9, 4096, 4, 8192, 4)):               # no such dtype
+        e[0], e[1] = good, bad
+        assert L.clm_expm_batched_plan_create(ctypes.byref(h), 2, e) == ST.INVALID_VALUE
+    size = ctypes.c_size_t()
+    assert L.clm_expm_batched_buffer_size(None, ctypes.byref(size)) == ST.INVALID_VALUE
+    assert L.clm_expm_batched(None, None, None, None, 0, None) == ST.INVALID_VALUE
+
+
+def test_batched_tuning_table_round_trip():
+    cfg = (ctypes.c_int * _lib.BATCH_BUCKETS)()
+    assert L.clm_expm_batched_get_tuning(ST.DTYPE_C64, cfg) == 0
+    saved = list(cfg)
+    assert all(0 <= c <= 3 for c in saved)
+    new = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([1] * _lib.BATCH_BUCKETS))
+    assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, new) == 0
+    assert L.clm_expm_batched_get_tuning(ST.DTYPE_C64, cfg) == 0 and list(cfg) == [1] * 8
+    bad = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([4] * _lib.BATCH_BUCKETS))
+    assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, bad) == ST.INVALID_VALUE
+    assert L.clm_expm_batched_set_tuning(7, new) == ST.INVALID_VALUE
+    assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, (ctypes.c_int * 8)(*saved)) == 0

@@ -81,7 +81,9 @@ def test_expm_split_meets_1e6(dev, n, target):
     got, plan = _run_split(a, dev)
     err = rel(got, oracle.expm(a))
     assert err <= 1e-6, (plan, err)
-    assert plan["ps_split"] in (3, 6) and plan["split_squarings"] == min(plan["s"], 1)
+    assert plan["ps_split"] in (3, 6)
+    # the last squaring splits; with s > 8 every squaring stays FP64
+    assert plan["split_squarings"] == (1 if 1 <= plan["s"] <= 8 else 0)
     # the plan is the one the host predicts
     assert plan == {**cg.expm_split_plan(plan["norm1"]), "dtype": 1}
 

@@ -10,6 +10,15 @@ reported in "breakdown", and so is the BF16-split precision variant
 FP64 and its relative error against the FP64 oracle (the metric's target is
 <= 1e-6).
 
+The batched workload runs in the same process and is reported under
+"batched": 4096 matrices with n uniform in [16, 256], half real FP32
+(N(0,1)/sqrt(n), ||A||_1 log-uniform in [0.1, 20]) and half complex64
+anti-Hermitian i t (G + G^H)/2 with t uniform in [0.01, 1], all through ONE
+plan of the batched persistent expm (tiles from its autotuner, launch captured
+in a hipGraph); matrices/s over the timed launches, and the FLOP the executed
+plans issue (sum of num_gemms x 2n^3, x4 for complex) per second against the
+FP32 MFMA peak.
+
 Also reported, on the same JSON line:
   roofline      the dominant kernel (the 128x128 FP64 MFMA GEMM that every
                 polynomial product and squaring runs on): 2 n^3 FLOP per launch
@@ -153,6 +162,90 @@ def traffic_probe(dev, n=N, reps=5):
     log(f"traffic probe done: copy 268435456 B each way, {reps} GEMM launches n={n}")
 
 
+PEAK_FP32_TFLOPS = 157.3
+BATCH_COUNT = 4096
+
+
+def make_batch(count=BATCH_COUNT, seed=3):
+    """The batched workload, as column-major numpy arrays (list of (n, dtype, a_cm))."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(16, 257))
+        if i % 2 == 0:
+            a = rng.standard_normal((n, n)) / np.sqrt(n)
+            target = float(np.exp(rng.uniform(np.log(0.1), np.log(20.0))))
+            a *= target / np.abs(a).sum(0).max()
+            out.append((n, "f32", np.ascontiguousarray(a.T, dtype=np.float32)))
+        else:
+            g = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+            t = float(rng.uniform(0.01, 1.0))
+            a = 1j * t * (g + g.conj().T) / 2
+            out.append((n, "c64", np.ascontiguousarray(a.T, dtype=np.complex64)))
+    return out
+
+
+def batched_leg(dev, steps, warmup):
+    """One plan over the 4096-matrix batch, autotuned tiles, graph-captured launches."""
+    import numpy as np
+    import torch
+    import calaman_g_amd as cg
+    from calaman_g_amd.expm import BatchPlan
+
+    t0 = time.perf_counter()
+    tune_f32, _ = cg.batched_autotune(torch.float32, nmax=256)
+    tune_c64, _ = cg.batched_autotune(torch.complex64, nmax=256)
+    tune_s = time.perf_counter() - t0
+    batch = make_batch()
+    tdt = {"f32": torch.float32, "c64": torch.complex64}
+    entries = []
+    for n, kind, a in batch:
+        da = torch.from_numpy(a).to(dev)
+        entries.append((n, da, n, torch.empty_like(da), n))
+    plan = BatchPlan(entries)
+    work = plan.workspace()
+    res = plan.results_buffer()
+    stream = torch.cuda.Stream()
+    for _ in range(warmup):
+        if plan.launch(work, res, stream=stream, graph=True) != 0:
+            raise RuntimeError("batched launch failed")
+    stream.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        plan.launch(work, res, stream=stream, graph=True)
+    e1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    sec = max(wall, e0.elapsed_time(e1) * 1e-3) / steps
+    results = BatchPlan.decode_results(res)
+    flop = 0.0
+    for (n, kind, _), r in zip(batch, results):
+        if r["status"] != 0:
+            raise RuntimeError(f"batched member failed: {r}")
+        flop += r["num_gemms"] * 2.0 * n ** 3 * (4 if kind == "c64" else 1)
+    achieved = flop / sec / 1e12
+    # spot check: a few members against the single-matrix entry point (bitwise)
+    mism = 0
+    for i in (0, 1, 2, 3, len(batch) - 1):
+        n, kind, a = batch[i]  # a: column-major storage of A
+        got = entries[i][3].cpu().numpy().T
+        single = cg.expm(torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)).cpu().numpy()
+        mism += int(not np.array_equal(got, single))
+    ss = [r["s"] for r in results]
+    return {"matrices_per_s": round(len(batch) / sec, 1), "ms_per_launch": round(sec * 1e3, 3),
+            "count": len(batch), "n_range": [16, 256], "dtypes": ["f32", "c64"],
+            "flop_per_launch": flop, "mfma_tflops": round(achieved, 2),
+            "mfma_peak_tflops": PEAK_FP32_TFLOPS,
+            "mfma_frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "s_max": max(ss), "s_mean": round(float(np.mean(ss)), 2),
+            "tiles": {"f32": tune_f32, "c64": tune_c64}, "autotune_s": round(tune_s, 2),
+            "graph": True, "spot_check_bitwise_mismatches": mism}
+
+
 def cpu_baseline(mats):
     import oracle
     threads = oracle.num_threads()
@@ -203,6 +296,9 @@ def main():
     roof = gemm_roofline(dev)
     log(f"roofline: {roof}")
 
+    batched = batched_leg(dev, max(3, args.steps // 2), 2)
+    log(f"batched: {batched}")
+
     cpu = None
     errs = None
     serrs = None
@@ -245,7 +341,8 @@ def main():
                                           "split_squarings": p["split_squarings"]}
                                  for t, p in zip(NORMS, splans)},
             "complex128_expm_per_s": round(len(NORMS) / zsec, 3),
-            "complex128_ms_per_step": round(zsec * 1e3, 4)},
+            "complex128_ms_per_step": round(zsec * 1e3, 4),
+            "batched": batched},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -82,6 +82,13 @@ class BatchEntry(ctypes.Structure):
                 ("lda", ctypes.c_int), ("E", ctypes.c_void_p), ("lde", ctypes.c_int)]
 
 
+class BatchAdEntry(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("dtype", ctypes.c_int), ("A", ctypes.c_void_p),
+                ("lda", ctypes.c_int), ("E", ctypes.c_void_p), ("lde", ctypes.c_int),
+                ("G", ctypes.c_void_p), ("ldg", ctypes.c_int), ("dA", ctypes.c_void_p),
+                ("ldda", ctypes.c_int)]
+
+
 class BatchResult(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
                 ("status", ctypes.c_int), ("norm1", ctypes.c_double)]
@@ -181,6 +188,9 @@ _SIGNATURES = {
     "clm_expm_batched_get_tuning": (_i, [_i, ctypes.POINTER(_i)]),
     "clm_expm_batched_set_tuning": (_i, [_i, ctypes.POINTER(_i)]),
     "clm_expm_batched_autotune": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_f)]),
+    "clm_expm_batched_ad_plan_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchAdEntry),
+                                             _i]),
+    "clm_expm_batched_backward": (_i, [_vp, _vp, _vp, _vp, _sz]),
 }
 
 

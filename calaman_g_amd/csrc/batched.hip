@@ -46,6 +46,13 @@ struct BatchDesc {
   int n, lda, lde, dtype;
   int index;  // position in the caller's entry array
   int cfg;    // workgroup tile configuration (CLM_BATCH_TILE_*)
+  // adjoint plans only (tape_off < 0 otherwise)
+  const void *G;  // cotangent of E
+  void *dA;       // gradient with respect to A
+  long long tape_off;
+  int ldg, ldda;
+  int ckpt;  // checkpoint slots for the squaring chain
+  int pad_;
 };
 
 namespace {
@@ -86,6 +93,29 @@ __host__ __device__ inline long stride_of(int n) {
 template <typename T>
 size_t matrix_ws_bytes(int n) {
   return static_cast<size_t>(2 + max_bank_blocks()) * stride_of<T>(n) * sizeof(T);
+}
+
+// The adjoint's tape, per matrix, after a 256-byte head {m, s, c, status}:
+//   P[1..r_max]      the Horner iterates P_r (= B_r) .. P_1
+//   CK[0..K)         squaring checkpoints R_0, R_c, R_2c, ... (c = ceil(s / K))
+//   U, V             cotangent ping-pong of the backward sweep
+//   W0, W1           squarings recomputed from a checkpoint
+//   Xb               cotangent of the scaled matrix X0 = A / 2^s
+//   Yb[0..nbank)     cotangents of the power bank X0^2 ..
+__host__ __device__ inline int max_horner_steps() {
+  int mx = 0;
+  for (int i = 0; i < kNumTaylor; ++i) {
+    const int r = ps_plan(taylor_degree(i), 0).r;
+    if (r > mx) mx = r;
+  }
+  return mx;
+}
+__host__ __device__ inline int tape_blocks(int ckpt) {
+  return max_horner_steps() + ckpt + 5 + max_bank_blocks();
+}
+template <typename T>
+size_t tape_bytes(int n, int ckpt) {
+  return 256 + static_cast<size_t>(tape_blocks(ckpt)) * stride_of<T>(n) * sizeof(T);
 }
 
 // ---- LDS budget per tile configuration -------------------------------------------
@@ -271,6 +301,154 @@ __device__ void wg_gemm(const GemmArgs<T> &g, char *lds) {
   }
 }
 
+// ---- products with (conjugate-)transposed operands, for the adjoint ------------------
+// C = op(A) op(B) (+ epilogue of g), op = identity or conjugate transpose (plain
+// transpose for real types). Each operand is staged through scalar loads laid
+// out so that consecutive lanes read consecutive addresses of the stored matrix
+// (along k for a transposed A, along n for a transposed B), then written into
+// the same LDS layouts the NN kernels use, so the MFMA compute helpers are
+// shared.
+__device__ __forceinline__ float conj_if(float x, bool) { return x; }
+__device__ __forceinline__ double conj_if(double x, bool) { return x; }
+__device__ __forceinline__ zc conj_if(zc x, bool c) { return c ? zc{x.re, -x.im} : x; }
+__device__ __forceinline__ zd conj_if(zd x, bool c) { return c ? zd{x.re, -x.im} : x; }
+
+template <typename T, int BM, int BN, int BK, bool TA, bool TB>
+__device__ __forceinline__ void op_gload(const GemmArgs<T> &g, int m0, int n0, int k0, int tid,
+                                         T (&ra)[BM * BK / kNT], T (&rb)[BN * BK / kNT]) {
+#pragma unroll
+  for (int it = 0; it < BM * BK / kNT; ++it) {
+    const int e = tid + it * kNT;
+    const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
+    const int gm = m0 + mm, gk = k0 + kk;
+    T v = ops<T>::zero();
+    if (gm < g.M && gk < g.K)
+      v = conj_if(TA ? g.A[gk + static_cast<long>(gm) * g.lda] : g.A[gm + static_cast<long>(gk) * g.lda], TA);
+    ra[it] = v;
+  }
+#pragma unroll
+  for (int it = 0; it < BN * BK / kNT; ++it) {
+    const int e = tid + it * kNT;
+    const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
+    const int gn = n0 + nn, gk = k0 + kk;
+    T v = ops<T>::zero();
+    if (gn < g.N && gk < g.K)
+      v = conj_if(TB ? g.B[gn + static_cast<long>(gk) * g.ldb] : g.B[gk + static_cast<long>(gn) * g.ldb], TB);
+    rb[it] = v;
+  }
+}
+
+// register -> LDS in the layouts of RTile / CTile: A at As[k][m] (row BM + 16),
+// B at Bs[n][k] (row BK + one vector); complex operands as a re plane followed by
+// an im plane of the same layout
+template <typename T, int BM, int BN, int BK, bool TA, bool TB>
+__device__ __forceinline__ void op_sstore(real_t<T> *As, real_t<T> *Bs, int tid,
+                                          const T (&ra)[BM * BK / kNT],
+                                          const T (&rb)[BN * BK / kNT]) {
+  using R = real_t<T>;
+  constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
+  constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
+#pragma unroll
+  for (int it = 0; it < BM * BK / kNT; ++it) {
+    const int e = tid + it * kNT;
+    const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
+    const int sidx = kk * LDA_S + mm;
+    if constexpr (is_cplx<T>()) {
+      As[sidx] = ra[it].re;
+      As[A_PLANE + sidx] = ra[it].im;
+    } else {
+      As[sidx] = ra[it];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < BN * BK / kNT; ++it) {
+    const int e = tid + it * kNT;
+    const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
+    const int sidx = nn * LDB_S + kk;
+    if constexpr (is_cplx<T>()) {
+      Bs[sidx] = rb[it].re;
+      Bs[B_PLANE + sidx] = rb[it].im;
+    } else {
+      Bs[sidx] = rb[it];
+    }
+  }
+}
+
+// C = op(A) op(B) (+ epilogue) over the whole output by the workgroup; same
+// tiles and compute helpers as wg_gemm_real / wg_gemm_cplx
+template <typename T, int BM, int BN, int WM, int WN, int BK, bool TA, bool TB>
+__device__ void wg_gemm_op(const GemmArgs<T> &g, char *lds) {
+  using R = real_t<T>;
+  constexpr int WAVES_M = BM / WM;
+  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
+  static_assert((BM * BK) % kNT == 0 && (BN * BK) % kNT == 0, "staging");
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
+  constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
+  constexpr int NPL = is_cplx<T>() ? 2 : 1;
+  constexpr int STAGE = NPL * (A_PLANE + B_PLANE);
+  R *L = reinterpret_cast<R *>(lds);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
+  const int KT = (g.K + BK - 1) / BK;
+  for (int t = 0; t < tiles_m * tiles_n; ++t) {
+    const int m0 = (t % tiles_m) * BM, n0 = (t / tiles_m) * BN;
+    typename Mfma<R>::acc cr[TM][TN], ci[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        cr[i][j] = Mfma<R>::zero();
+        ci[i][j] = Mfma<R>::zero();
+      }
+    T ra[BM * BK / kNT], rb[BN * BK / kNT];
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+      R *stage = L + (kt & 1) * STAGE;
+      R *As = stage, *Bs = stage + NPL * A_PLANE;
+      op_gload<T, BM, BN, BK, TA, TB>(g, m0, n0, kt * BK, tid, ra, rb);
+      op_sstore<T, BM, BN, BK, TA, TB>(As, Bs, tid, ra, rb);
+      __syncthreads();
+      if constexpr (is_cplx<T>()) {
+        cgemm_compute<T, BM, BN, WM, WN, kNT, BK>(stage, lane, wm0, wn0, cr, ci);
+      } else {
+        rgemm_compute<T, BM, BN, WM, WN, kNT, BK>(As, Bs, lane, wm0, wn0, cr);
+      }
+      // the next slice goes to the other stage: no second barrier needed
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm0 + i * 16 + Mfma<R>::row(lane, r);
+          const int col = n0 + wn0 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N) {
+            if constexpr (is_cplx<T>())
+              epilogue_store(g, row, col, T{R(cr[i][j][r]), R(ci[i][j][r])});
+            else
+              epilogue_store(g, row, col, R(cr[i][j][r]));
+          }
+        }
+  }
+  __syncthreads();
+}
+
+template <typename T, int CFG, bool TA, bool TB>
+__device__ void wg_gemm_ops(const GemmArgs<T> &g, char *lds) {
+  constexpr int c = effective_cfg<T>(CFG);
+  if constexpr (c == CLM_BATCH_TILE_32) {
+    if constexpr (is_cplx<T>()) wg_gemm_op<T, 32, 32, 16, 16, kBK, TA, TB>(g, lds);
+    else wg_gemm_op<T, 32, 32, 16, 16, small_bk<T>(), TA, TB>(g, lds);
+  }
+  else if constexpr (c == CLM_BATCH_TILE_64) wg_gemm_op<T, 64, 64, 32, 32, kBK, TA, TB>(g, lds);
+  else if constexpr (c == CLM_BATCH_TILE_128x64 || is_cplx<T>())
+    wg_gemm_op<T, 128, 64, 64, 32, kBK, TA, TB>(g, lds);
+  else wg_gemm_op<T, 128, 128, 64, 64, kBK, TA, TB>(g, lds);
+}
+
 // The 1-norm in the fold order of the single-matrix reduction (256 virtual
 // lanes, each folding rows t, t+256, ... seeded by its first, then a halving
 // tree): virtual lane t + 64q sits in register q of lane t of a wave; the
@@ -347,6 +525,45 @@ __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms,
   __syncthreads();
 }
 
+// dst = src, column by column
+template <typename T>
+__device__ void wg_copy(int n, const T *src, long lds_, T *dst, long ldd) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < n; j += 4)
+    for (int i = lane; i < n; i += 64)
+      dst[i + static_cast<long>(j) * ldd] = src[i + static_cast<long>(j) * lds_];
+  __syncthreads();
+}
+
+// the adjoint tape of one matrix (layout above tape_blocks)
+template <typename T>
+struct Tape {
+  int *head;  // {m, s, c, status}
+  T *base;    // first block
+  long stride;
+  int ckpt;
+  __device__ T *horner(int j) const { return base + static_cast<long>(j - 1) * stride; }  // P_j, j >= 1
+  __device__ T *checkpoint(int i) const {
+    return base + static_cast<long>(max_horner_steps() + i) * stride;
+  }
+  __device__ T *scratch(int i) const {  // U, V, W0, W1, Xb
+    return base + static_cast<long>(max_horner_steps() + ckpt + i) * stride;
+  }
+  __device__ T *bank_bar(int i) const {  // cotangent of X0^(i+2)
+    return base + static_cast<long>(max_horner_steps() + ckpt + 5 + i) * stride;
+  }
+};
+
+template <typename T>
+__device__ Tape<T> tape_of(const BatchDesc &d, char *work) {
+  Tape<T> t;
+  t.head = reinterpret_cast<int *>(work + d.tape_off);
+  t.base = reinterpret_cast<T *>(work + d.tape_off + 256);
+  t.stride = stride_of<T>(d.n);
+  t.ckpt = d.ckpt;
+  return t;
+}
+
 template <typename T>
 __device__ GemmArgs<T> square_args(int n, const T *A, long lda, const T *B, long ldb, T *C,
                                    long ldc) {
@@ -383,10 +600,14 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   T *bank = As + stride;
   T *S = bank + static_cast<long>(max_bank_blocks()) * stride;
 
+  const bool ad = d.tape_off >= 0;
+  const Tape<T> tape = tape_of<T>(d, work);
+
   const R norm = uniform(wg_norm1(n, A, d.lda, lds));
   if (!isfinite(norm)) {
     if (threadIdx.x == 0 && results)
       results[d.index] = clm_batch_result{0, 0, 0, CLM_STATUS_INVALID_VALUE, double(norm)};
+    if (threadIdx.x == 0 && ad) tape.head[3] = CLM_STATUS_INVALID_VALUE;
     return;  // uniform: every lane holds the same norm
   }
   const clm_expm_plan p = taylor_plan(double(norm), std::is_same_v<R, float>);
@@ -394,6 +615,14 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   if (threadIdx.x == 0 && results)
     results[d.index] =
         clm_batch_result{degree, nsq, uniform(p.num_gemms), CLM_STATUS_SUCCESS, double(norm)};
+  // checkpoint spacing: R_0, R_c, R_2c, ... fill at most ckpt slots
+  const int cstep = ad ? max(1, (nsq + d.ckpt - 1) / d.ckpt) : 1;
+  if (threadIdx.x == 0 && ad) {
+    tape.head[0] = degree;
+    tape.head[1] = nsq;
+    tape.head[2] = cstep;
+    tape.head[3] = CLM_STATUS_SUCCESS;
+  }
 
   const T *src = A;
   long ld_src = d.lda;
@@ -418,10 +647,13 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   for (int q = 0; q <= total; ++q) {
     if (q == nbank) {
       const int terms_r = min(degree - nhorner * bs + 1, bs);
-      wg_build_block(n, cur == 0 ? target : other, cur == 0 ? ld_target : ld_other,
-                     coef + static_cast<long>(nhorner) * bs, terms_r, src, ld_src, bank, stride,
-                     static_cast<long>(n));
+      // with a tape, the Horner iterates P_r .. P_1 stay in their own blocks
+      T *blk = (ad && nhorner >= 1) ? tape.horner(nhorner) : (cur == 0 ? target : other);
+      const long ldb = (ad && nhorner >= 1) ? long(n) : (cur == 0 ? ld_target : ld_other);
+      wg_build_block(n, blk, ldb, coef + static_cast<long>(nhorner) * bs, terms_r, src, ld_src,
+                     bank, stride, static_cast<long>(n));
     }
+    if (ad && nsq > 0 && q == nbank + nhorner) wg_copy(n, res, ld_res, tape.checkpoint(0), long(n));
     if (q == total) break;
     GemmArgs<T> g;
     if (q < nbank) {
@@ -433,8 +665,14 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
       const int j = nhorner - 1 - (q - nbank);
       T *from = cur == 0 ? target : other;
       T *to = cur == 0 ? other : target;
-      g = square_args(n, from, cur == 0 ? ld_target : ld_other, X, ldx, to,
-                      cur == 0 ? ld_other : ld_target);
+      long ld_from = cur == 0 ? ld_target : ld_other, ld_to = cur == 0 ? ld_other : ld_target;
+      if (ad) {
+        from = tape.horner(j + 1);
+        ld_from = n;
+        to = (j == 0) ? target : tape.horner(j);
+        ld_to = (j == 0) ? ld_target : long(n);
+      }
+      g = square_args(n, from, ld_from, X, ldx, to, ld_to);
       g.coef = coef + static_cast<long>(j) * bs;
       g.nterms = min(degree - j * bs + 1, bs);
       g.X1 = src;
@@ -453,6 +691,150 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
       ld_spare = tl;
     }
     wg_gemm<T, CFG>(g, lds);
+    if (ad && q >= nbank + nhorner) {
+      const int k1 = q - (nbank + nhorner) + 1;  // this product made R_k1
+      if (k1 < nsq && k1 % cstep == 0) wg_copy(n, res, ld_res, tape.checkpoint(k1 / cstep), long(n));
+    }
+  }
+}
+
+// ---- the adjoint -------------------------------------------------------------------
+// Reverse mode through the executed chain (the gradient of the computed
+// function, which is the Frechet adjoint L(A^H, G) to within the truncation
+// and rounding of the forward): with G the cotangent of E = R_s,
+//   squarings  R_{k+1} = R_k^2          R_k^- = R_{k+1}^- R_k^H + R_k^H R_{k+1}^-
+//   Horner     P_j = P_{j+1} X + B_j    P_{j+1}^- = P_j^- X^H,  X^- += P_{j+1}^H P_j^-,
+//                                       B_j^- = P_j^-
+//   blocks     B_j = sum_t c_t X0^t     (X0^t)^- += conj(c_t) B_j^-
+//   bank       Y_k = Y_{k-1} X0         Y_{k-1}^- += Y_k^- X0^H,  X0^- += Y_{k-1}^H Y_k^-
+//   scaling    A^- = 2^-s X0^-
+// The R_k come from the checkpoints (squared forward again between them), the
+// Horner iterates and the power bank from the forward's tape and workspace.
+
+// dst_t += conj(c_t) * bar for the block terms t = 1 .. terms-1 (t = 1 into Xb,
+// t >= 2 into the bank cotangents)
+template <typename T>
+__device__ void wg_block_adjoint(int n, const T *bar, long ldbar, const T *c, int terms, T *Xb,
+                                 const Tape<T> &tape) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < n; j += 4)
+    for (int i = lane; i < n; i += 64) {
+      const T b = bar[i + static_cast<long>(j) * ldbar];
+      const long at = i + static_cast<long>(j) * n;
+      if (terms > 1) Xb[at] = ops<T>::mad(conj_if(c[1], true), b, Xb[at]);
+      for (int t = 2; t < terms; ++t) {
+        T *y = tape.bank_bar(t - 2);
+        y[at] = ops<T>::mad(conj_if(c[t], true), b, y[at]);
+      }
+    }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void wg_fill(int n, T *dst, long ld, T v) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < n; j += 4)
+    for (int i = lane; i < n; i += 64) dst[i + static_cast<long>(j) * ld] = v;
+  __syncthreads();
+}
+
+template <typename T>
+__device__ GemmArgs<T> accumulate(GemmArgs<T> g) {
+  g.Cin = g.C;
+  g.ldcin = g.ldc;
+  g.beta = ops<T>::from_real(1.0);
+  return g;
+}
+
+template <typename T, int CFG>
+__device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char *lds) {
+  using R = real_t<T>;
+  const int n = d.n;
+  const Tape<T> tape = tape_of<T>(d, work);
+  T *dA = static_cast<T *>(d.dA);
+  const T *A = static_cast<const T *>(d.A);
+  if (uniform(tape.head[3]) != CLM_STATUS_SUCCESS) {
+    const R nan = __builtin_nan("");
+    wg_fill(n, dA, d.ldda, ops<T>::from_real(nan));
+    return;
+  }
+  const int degree = uniform(tape.head[0]), nsq = uniform(tape.head[1]);
+  const int cstep = uniform(tape.head[2]);
+  const long stride = stride_of<T>(n);
+  T *As = reinterpret_cast<T *>(work + d.ws_off);
+  const T *bank = As + stride;
+  const T *X0 = nsq > 0 ? As : A;
+  const long ldx0 = nsq > 0 ? long(n) : long(d.lda);
+  const clm_ps_plan pp = ps_plan(degree, 0);
+  const int bs = uniform(pp.s), nhorner = uniform(pp.r), nbank = uniform(pp.num_powers);
+  T *U = tape.scratch(0), *V = tape.scratch(1), *W0 = tape.scratch(2), *W1 = tape.scratch(3);
+  T *Xb = tape.scratch(4);
+
+  // squarings, last first
+  const T *bar = static_cast<const T *>(d.G);
+  long ldbar = d.ldg;
+  for (int k = nsq - 1; k >= 0; --k) {
+    const int base = k / cstep;
+    const T *Rk = tape.checkpoint(base);
+    for (int have = base * cstep, w = 0; have < k; ++have, w ^= 1) {
+      T *to = w ? W1 : W0;
+      wg_gemm<T, CFG>(square_args(n, Rk, long(n), Rk, long(n), to, long(n)), lds);
+      Rk = to;
+    }
+    T *out = (bar == U) ? V : U;
+    wg_gemm_ops<T, CFG, false, true>(square_args(n, bar, ldbar, Rk, long(n), out, long(n)), lds);
+    wg_gemm_ops<T, CFG, true, false>(accumulate(square_args(n, Rk, long(n), bar, ldbar, out, long(n))),
+                                     lds);
+    bar = out;
+    ldbar = n;
+  }
+
+  // Horner steps and the blocks they consume
+  wg_fill(n, Xb, long(n), ops<T>::zero());
+  for (int t = 0; t < nbank; ++t) wg_fill(n, tape.bank_bar(t), long(n), ops<T>::zero());
+  const T *X = (bs >= 2) ? bank + static_cast<long>(bs - 2) * stride : X0;
+  const long ldx = (bs >= 2) ? long(n) : ldx0;
+  T *Xbar = (bs >= 2) ? tape.bank_bar(bs - 2) : Xb;
+  for (int j = 0; j <= nhorner; ++j) {
+    wg_block_adjoint(n, bar, ldbar, coef + static_cast<long>(j) * bs,
+                     min(degree - j * bs + 1, bs), Xb, tape);
+    if (j == nhorner) break;
+    wg_gemm_ops<T, CFG, true, false>(
+        accumulate(square_args(n, tape.horner(j + 1), long(n), bar, ldbar, Xbar, long(n))), lds);
+    T *out = (bar == U) ? V : U;
+    wg_gemm_ops<T, CFG, false, true>(square_args(n, bar, ldbar, X, ldx, out, long(n)), lds);
+    bar = out;
+    ldbar = n;
+  }
+
+  // power bank, highest power first
+  for (int k = nbank + 1; k >= 2; --k) {
+    T *ykb = tape.bank_bar(k - 2);
+    T *ykm1b = (k - 1 >= 2) ? tape.bank_bar(k - 3) : Xb;
+    const T *ykm1 = (k - 1 >= 2) ? bank + static_cast<long>(k - 3) * stride : X0;
+    const long ldy = (k - 1 >= 2) ? long(n) : ldx0;
+    wg_gemm_ops<T, CFG, false, true>(accumulate(square_args(n, ykb, long(n), X0, ldx0, ykm1b, long(n))),
+                                     lds);
+    wg_gemm_ops<T, CFG, true, false>(accumulate(square_args(n, ykm1, ldy, ykb, long(n), Xb, long(n))),
+                                     lds);
+  }
+  wg_scale_copy(n, ldexp(1.0, -nsq), Xb, long(n), dA, long(d.ldda));
+}
+
+template <typename T, int CFG>
+__global__ void __launch_bounds__(kNT)
+    expm_adjoint_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
+                        const T *coef) {
+  __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T, CFG>()];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= count) break;
+    const BatchDesc d = desc[item];
+    adjoint_one<T, CFG>(d, work, coef, lds);
   }
 }
 
@@ -479,6 +861,8 @@ __global__ void __launch_bounds__(kNT)
   }
 }
 
+// zeroes the kGroups queue counters at q (a kernel, not a memset: it is captured
+// into graphs)
 __global__ void reset_queue_kernel(int *q) {
   if (threadIdx.x < kGroups) q[threadIdx.x] = 0;
 }
@@ -534,6 +918,15 @@ int precision_of(int dtype) { return (dtype == CLM_DTYPE_F64 || dtype == CLM_DTY
 bool valid_dtype(int dtype) {
   return dtype == CLM_DTYPE_F32 || dtype == CLM_DTYPE_F64 || dtype == CLM_DTYPE_C64 ||
          dtype == CLM_DTYPE_C128;
+}
+
+size_t tape_bytes_of(int dtype, int n, int ckpt) {
+  switch (dtype) {
+  case CLM_DTYPE_F32: return tape_bytes<float>(n, ckpt);
+  case CLM_DTYPE_F64: return tape_bytes<double>(n, ckpt);
+  case CLM_DTYPE_C64: return tape_bytes<zc>(n, ckpt);
+  default: return tape_bytes<zd>(n, ckpt);
+  }
 }
 
 size_t ws_bytes_of(int dtype, int n) {
@@ -603,6 +996,44 @@ bool prepare_group(int g, int device) {
   }
 }
 
+template <typename T, int CFG>
+hipError_t launch_adjoint_group(hipStream_t st, int device, const BatchDesc *desc, int count,
+                                int *queue, char *work) {
+  const T *coef = inv_factorials<T>(device);
+  if (!coef) return hipErrorInvalidValue;
+  static int cached[64] = {};
+  int &resident = cached[device & 63];
+  if (resident == 0) {
+    int blocks = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_adjoint_kernel<T, CFG>, kNT, 0) !=
+            hipSuccess ||
+        blocks < 1)
+      blocks = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus < 1)
+      cus = 256;
+    resident = blocks * cus;
+  }
+  hipLaunchKernelGGL((expm_adjoint_kernel<T, CFG>), dim3(std::min(count, resident)), dim3(kNT), 0,
+                     st, desc, count, queue, work, coef);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adjoint_type(int cfg, hipStream_t st, int device, const BatchDesc *desc,
+                               int count, int *queue, char *work) {
+  switch (cfg) {
+  case CLM_BATCH_TILE_32:
+    return launch_adjoint_group<T, CLM_BATCH_TILE_32>(st, device, desc, count, queue, work);
+  case CLM_BATCH_TILE_64:
+    return launch_adjoint_group<T, CLM_BATCH_TILE_64>(st, device, desc, count, queue, work);
+  case CLM_BATCH_TILE_128x64:
+    return launch_adjoint_group<T, CLM_BATCH_TILE_128x64>(st, device, desc, count, queue, work);
+  default:
+    return launch_adjoint_group<T, CLM_BATCH_TILE_128>(st, device, desc, count, queue, work);
+  }
+}
+
 template <typename T>
 hipError_t launch_type(int cfg, hipStream_t st, int device, const BatchDesc *desc, int count,
                        int *queue, char *work, clm_batch_result *results) {
@@ -628,6 +1059,8 @@ struct clm_batch_plan {
   int group_count[clm::kGroups] = {};  // matrices per (element type, tile) group
   int group_start[clm::kGroups] = {};
   clm::BatchDesc *d_desc = nullptr;    // grouped, each group in decreasing cost
+  bool ad = false;                     // records the adjoint tape; backward allowed
+  int ckpt = 0;
   size_t ws_bytes = 0;
   std::vector<clm::BatchDesc> host_desc;
   // the captured launch (clm_expm_batched_graph)
@@ -641,14 +1074,18 @@ namespace clm {
 namespace {
 
 clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *forced_cfg,
-                        clm_batch_plan **out) {
+                        clm_batch_plan **out, const clm_batch_ad_entry *ad = nullptr,
+                        int ckpt = 0) {
   if (out == nullptr || count < 1 || entries == nullptr) return CLM_STATUS_INVALID_VALUE;
   for (int i = 0; i < count; ++i) {
     const clm_batch_entry &e = entries[i];
     if (e.n < 1 || e.n > kMaxBatchN || e.lda < e.n || e.lde < e.n || e.A == nullptr ||
         e.E == nullptr || !valid_dtype(e.dtype))
       return CLM_STATUS_INVALID_VALUE;
+    if (ad && (ad[i].G == nullptr || ad[i].dA == nullptr || ad[i].ldg < e.n || ad[i].ldda < e.n))
+      return CLM_STATUS_INVALID_VALUE;
   }
+  if (ad && (ckpt < 1 || ckpt > 64)) return CLM_STATUS_INVALID_VALUE;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return CLM_STATUS_NOT_INITIALIZED;
   {
@@ -694,6 +1131,21 @@ clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *fo
     d.index = i;
     d.cfg = cfg[i];
     off += Layout::align(ws_bytes_of(e.dtype, e.n));
+    d.G = nullptr;
+    d.dA = nullptr;
+    d.tape_off = -1;
+    d.ldg = d.ldda = 0;
+    d.ckpt = 0;
+    d.pad_ = 0;
+    if (ad) {
+      d.G = ad[i].G;
+      d.dA = ad[i].dA;
+      d.ldg = ad[i].ldg;
+      d.ldda = ad[i].ldda;
+      d.ckpt = ckpt;
+      d.tape_off = static_cast<long long>(off);
+      off += Layout::align(tape_bytes_of(e.dtype, e.n, ckpt));
+    }
     plan->group_count[group_of(e.dtype, cfg[i])] += 1;
   }
   for (int g = 1; g < kGroups; ++g)
@@ -704,6 +1156,8 @@ clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *fo
       return CLM_STATUS_EXECUTION_FAILED;
     }
   plan->ws_bytes = off;
+  plan->ad = ad != nullptr;
+  plan->ckpt = ckpt;
   if (hipMalloc(&plan->d_desc, sizeof(BatchDesc) * count) != hipSuccess) {
     delete plan;
     return CLM_STATUS_ALLOC_FAILED;
@@ -740,6 +1194,36 @@ clm_status_t launch_plan(clm_ctx *ctx, hipStream_t st, const clm_batch_plan *pla
     case CLM_DTYPE_F64: e = launch_type<double>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
     case CLM_DTYPE_C64: e = launch_type<zc>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
     default: e = launch_type<zd>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    }
+    CLM_TRY_HIP(e);
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
+// the adjoint sweep of an adjoint plan; reads the tape the last forward launch of
+// the same plan on the same workspace recorded
+clm_status_t launch_backward(clm_ctx *ctx, hipStream_t st, const clm_batch_plan *plan, void *work,
+                             size_t bytes) {
+  if (plan == nullptr || work == nullptr || !plan->ad) return CLM_STATUS_INVALID_VALUE;
+  if (bytes < plan->ws_bytes) return CLM_STATUS_ALLOC_FAILED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  int *queue = static_cast<int *>(work) + kGroups;  // the second set of counters
+  char *base = static_cast<char *>(work);
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, queue);
+  CLM_TRY_HIP(hipGetLastError());
+  static constexpr int kTypes[4] = {CLM_DTYPE_F32, CLM_DTYPE_F64, CLM_DTYPE_C64, CLM_DTYPE_C128};
+  for (int g = 0; g < kGroups; ++g) {
+    const int count = plan->group_count[g];
+    if (count == 0) continue;
+    const BatchDesc *desc = plan->d_desc + plan->group_start[g];
+    const int cfg = g % 4;
+    hipError_t e;
+    switch (kTypes[g / 4]) {
+    case CLM_DTYPE_F32: e = launch_adjoint_type<float>(cfg, st, plan->device, desc, count, queue + g, base); break;
+    case CLM_DTYPE_F64: e = launch_adjoint_type<double>(cfg, st, plan->device, desc, count, queue + g, base); break;
+    case CLM_DTYPE_C64: e = launch_adjoint_type<zc>(cfg, st, plan->device, desc, count, queue + g, base); break;
+    default: e = launch_adjoint_type<zd>(cfg, st, plan->device, desc, count, queue + g, base); break;
     }
     CLM_TRY_HIP(e);
   }
@@ -805,6 +1289,22 @@ extern "C" {
 clm_status_t clm_expm_batched_plan_create(clm_batch_plan **plan, int count,
                                           const clm_batch_entry *entries) {
   return clm::build_plan(count, entries, nullptr, plan);
+}
+
+clm_status_t clm_expm_batched_ad_plan_create(clm_batch_plan **plan, int count,
+                                             const clm_batch_ad_entry *entries, int checkpoints) {
+  if (count < 1 || entries == nullptr) return CLM_STATUS_INVALID_VALUE;
+  std::vector<clm_batch_entry> base(count);
+  for (int i = 0; i < count; ++i)
+    base[i] = clm_batch_entry{entries[i].n, entries[i].dtype, entries[i].A, entries[i].lda,
+                              entries[i].E, entries[i].lde};
+  return clm::build_plan(count, base.data(), nullptr, plan, entries, checkpoints);
+}
+
+clm_status_t clm_expm_batched_backward(clm_ctx *ctx, clm_stream_t stream,
+                                       const clm_batch_plan *plan, void *dWork,
+                                       size_t lwork_bytes) {
+  return clm::launch_backward(ctx, static_cast<hipStream_t>(stream), plan, dWork, lwork_bytes);
 }
 
 clm_status_t clm_expm_batched_plan_destroy(clm_batch_plan *plan) {

@@ -282,20 +282,33 @@ class BatchPlan:
     matrices; element type from A.dtype). The plan keeps references to the
     tensors, so they stay alive while it can launch."""
 
-    def __init__(self, entries):
+    def __init__(self, entries, adjoint=None, checkpoints: int = 4):
+        """adjoint: optional list of (G, ldg, dA, ldda) per entry -- an adjoint plan
+        whose forward records the tape clm_expm_batched_backward reads."""
         self._keep = []
-        arr = (_lib.BatchEntry * len(entries))()
+        self.adjoint = adjoint is not None
+        ent = _lib.BatchAdEntry if self.adjoint else _lib.BatchEntry
+        arr = (ent * len(entries))()
         for i, (n, a, lda, e, lde) in enumerate(entries):
             _require_device(a, "A")
             if e.dtype != a.dtype:
                 raise TypeError("A and E differ in dtype")
-            arr[i] = _lib.BatchEntry(int(n), _DTYPE_CODE[a.dtype], a.data_ptr(), int(lda),
-                                     e.data_ptr(), int(lde))
+            fields = [int(n), _DTYPE_CODE[a.dtype], a.data_ptr(), int(lda), e.data_ptr(), int(lde)]
             self._keep += [a, e]
+            if self.adjoint:
+                g, ldg, da, ldda = adjoint[i]
+                fields += [g.data_ptr(), int(ldg), da.data_ptr(), int(ldda)]
+                self._keep += [g, da]
+            arr[i] = ent(*fields)
         self.count = len(entries)
         h = ctypes.c_void_p()
-        check(lib().clm_expm_batched_plan_create(ctypes.byref(h), len(entries), arr),
-              "clm_expm_batched_plan_create")
+        if self.adjoint:
+            check(lib().clm_expm_batched_ad_plan_create(ctypes.byref(h), len(entries), arr,
+                                                        int(checkpoints)),
+                  "clm_expm_batched_ad_plan_create")
+        else:
+            check(lib().clm_expm_batched_plan_create(ctypes.byref(h), len(entries), arr),
+                  "clm_expm_batched_plan_create")
         self.handle = h
         nbytes = ctypes.c_size_t(0)
         check(lib().clm_expm_batched_buffer_size(h, ctypes.byref(nbytes)), "buffer_size")
@@ -327,6 +340,13 @@ class BatchPlan:
         return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), self.handle,
                      _ptr(work), work.numel(), _ptr(results)))
 
+    def backward(self, work: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
+                 ctx: Optional[Context] = None) -> int:
+        """clm_expm_batched_backward: dA from G, after a forward launch on `work`"""
+        return int(lib().clm_expm_batched_backward(ctx.handle if ctx else None,
+                                                   ctypes.c_void_p(_stream_ptr(stream)),
+                                                   self.handle, _ptr(work), work.numel()))
+
     def close(self):
         if getattr(self, "handle", None):
             lib().clm_expm_batched_plan_destroy(self.handle)
@@ -337,6 +357,63 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class _BatchedAdjointState:
+    """Device buffers and the adjoint plan for one (batch, n, dtype, device): the
+    column-major inputs, outputs, cotangents and gradients of a (B, n, n) batch."""
+
+    def __init__(self, b, n, dtype, device, checkpoints):
+        mk = lambda: torch.empty((b, n, n), dtype=dtype, device=device)
+        self.A, self.E, self.G, self.dA = mk(), mk(), mk(), mk()
+        ents = [(n, self.A[i], n, self.E[i], n) for i in range(b)]
+        adj = [(self.G[i], n, self.dA[i], n) for i in range(b)]
+        self.plan = BatchPlan(ents, adjoint=adj, checkpoints=checkpoints)
+        self.work = self.plan.workspace()
+
+
+_ad_states: dict = {}
+
+
+def _ad_state(b, n, dtype, device, checkpoints):
+    key = (b, n, dtype, str(device), checkpoints)
+    st = _ad_states.get(key)
+    if st is None:
+        st = _BatchedAdjointState(b, n, dtype, device, checkpoints)
+        _ad_states[key] = st
+    return st
+
+
+class ExpmBatchedFunction(torch.autograd.Function):
+    """exp of a (B, n, n) batch on the batched persistent kernel, with the
+    hand-written adjoint (clm_expm_batched_backward) as its backward. The state
+    (buffers, plan, tape) is cached per shape: a second forward of the same shape
+    overwrites the tape of the first, so backward each forward before the next."""
+
+    @staticmethod
+    def forward(ctx, a, checkpoints=4):
+        b, n, _ = a.shape
+        st = _ad_state(b, n, a.dtype, a.device, int(checkpoints))
+        # column-major storage of each A_i is A_i^T in row-major order
+        st.A.copy_(a.detach().transpose(1, 2))
+        check(st.plan.launch(st.work), "expm_batched forward")
+        ctx.state = st
+        return st.E.transpose(1, 2).clone()
+
+    @staticmethod
+    def backward(ctx, grad):
+        st = ctx.state
+        st.G.copy_(grad.transpose(1, 2))
+        check(st.plan.backward(st.work), "expm_batched backward")
+        return st.dA.transpose(1, 2).clone(), None
+
+
+def expm_batched_autograd(a: torch.Tensor, checkpoints: int = 4) -> torch.Tensor:
+    """exp of each matrix of a (B, n, n) device batch (float32/64, complex64/128),
+    differentiable through the hand-written Frechet adjoint."""
+    if a.dim() != 3 or a.shape[1] != a.shape[2]:
+        raise ValueError("expects a (B, n, n) batch")
+    return ExpmBatchedFunction.apply(a, checkpoints)
 
 
 def batched_tuning(dtype: torch.dtype) -> list:
@@ -480,5 +557,5 @@ __all__ = [
     "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",
     "horner_buffer_size", "to_cm", "from_cm", "BatchPlan", "expm_batched", "batched_tuning",
-    "set_batched_tuning", "batched_autotune",
+    "set_batched_tuning", "batched_autotune", "ExpmBatchedFunction", "expm_batched_autograd",
 ]

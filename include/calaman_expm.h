@@ -247,6 +247,39 @@ clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg);
 clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
                                        int *cfg_out, float *ms_out);
 
+/* ---- batched adjoint (Frechet derivative backward) -----------------------------------
+ * No reference counterpart. For L(E) with cotangent G = dL/dE (the conjugate
+ * Wirtinger convention of PyTorch for complex types), the backward writes
+ * dA = dL/dA by reverse mode through the executed chain: the squarings (from
+ * `checkpoints` saved powers, squared forward again between them), the
+ * Paterson-Stockmeyer Horner steps and blocks, the power bank and the scaling.
+ * It equals the Frechet adjoint L(A^H, G) to within the forward's accuracy.
+ *
+ * An adjoint plan's forward launch (clm_expm_batched / _graph) also records a
+ * tape in the workspace; clm_expm_batched_backward reads it, so the backward
+ * runs after the forward of the same plan on the same workspace, with A
+ * unchanged. G and dA are device matrices per entry. */
+typedef struct clm_batch_ad_entry {
+  int n;
+  int dtype;
+  const void *A;
+  int lda;
+  void *E;
+  int lde;
+  const void *G; /* cotangent of E, leading dimension ldg >= n */
+  int ldg;
+  void *dA;      /* gradient output, leading dimension ldda >= n */
+  int ldda;
+} clm_batch_ad_entry;
+/* checkpoints: 1..64 saved squaring powers per matrix (the spacing adapts to
+ * each matrix's s: c = ceil(s / checkpoints), recomputation costs up to c - 1
+ * squarings per backward step) */
+clm_status_t clm_expm_batched_ad_plan_create(clm_batch_plan **plan, int count,
+                                             const clm_batch_ad_entry *entries, int checkpoints);
+clm_status_t clm_expm_batched_backward(clm_ctx *ctx, clm_stream_t stream,
+                                       const clm_batch_plan *plan, void *dWork,
+                                       size_t lwork_bytes);
+
 /* ---- 1-norm --------------------------------------------------------------------
  * matrix_norm1<T> (src/expm/norm1.cppm:45-56): max_j sum_i |A(i,j)| with the
  * true complex modulus; NaN propagates. dColsum is device scratch of n

@@ -287,6 +287,31 @@ def expm_taylor(A, return_plan: bool = False):
     return (E, tuple(plan)) if return_plan else E
 
 
+# ---- Frechet derivative -----------------------------------------------------------
+def frechet(A, E):
+    """L(A, E), the Frechet derivative of exp at A in direction E, as the (1, 2)
+    block of exp([[A, E], [0, A]]) (Mathias 1996; Higham 2008, Thm 3.6), with
+    the oracle's expm of the 2n x 2n block matrix."""
+    c = _is_complex(A) or _is_complex(E)
+    dt = np.complex128 if c else np.float64
+    A = np.asarray(A, dtype=dt)
+    E = np.asarray(E, dtype=dt)
+    n = A.shape[0]
+    big = np.zeros((2 * n, 2 * n), dtype=dt)
+    big[:n, :n] = A
+    big[n:, n:] = A
+    big[:n, n:] = E
+    return np.asarray(expm(big))[:n, n:].copy()
+
+
+def frechet_adjoint(A, G):
+    """The gradient of Re tr(G^H exp(A)) with respect to A, in the convention
+    grad = dL/dRe + i dL/dIm: L(A^H, G), the adjoint of E -> L(A, E) under the
+    Frobenius inner product."""
+    A = np.asarray(A)
+    return frechet(A.conj().T, G)
+
+
 # ---- polynomials ------------------------------------------------------------------
 def ps_plan(degree: int, s_requested: int = 0):
     out = (ctypes.c_int * 5)()

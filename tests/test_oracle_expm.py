@@ -344,3 +344,60 @@ def test_lu_reports_singularity():
     a = np.ones((4, 4))
     _, info = oracle.lu_solve(a, np.ones((4, 1)))
     assert info > 0
+
+
+# ---- Frechet derivative (the adjoint's oracle) ---------------------------------------------
+def _golden_frechet():
+    import json
+    import os
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    man = json.load(open(os.path.join(g, "manifest.json")))
+    data = np.load(os.path.join(g, "frechet_scipy.npz"))
+    return [(c, data) for c in man["frechet"]]
+
+
+@pytest.mark.parametrize("case", _golden_frechet(), ids=lambda c: f"{c[0]['key']}-{c[0]['kind']}")
+def test_frechet_matches_scipy_golden(case):
+    c, data = case
+    k = c["key"]
+    got = oracle.frechet(data[f"{k}_A"], data[f"{k}_dir"])
+    ref = data[f"{k}_L"]
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_frechet_against_central_differences(cplx):
+    rng = np.random.default_rng(21)
+    n = 7
+    a = rng.standard_normal((n, n)) * 0.4
+    e = rng.standard_normal((n, n))
+    if cplx:
+        a = a + 1j * rng.standard_normal((n, n)) * 0.4
+        e = e + 1j * rng.standard_normal((n, n))
+    h = 1e-6
+    fd = (oracle.expm(a + h * e) - oracle.expm(a - h * e)) / (2 * h)
+    lf = oracle.frechet(a, e)
+    assert np.linalg.norm(lf - fd) <= 1e-8 * np.linalg.norm(lf)
+    # linear in the direction
+    assert np.allclose(oracle.frechet(a, 2.5 * e), 2.5 * lf, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_frechet_adjoint_is_the_gradient(cplx):
+    """Re <grad, D> equals the central difference of L(A) = Re tr(G^H exp(A))
+    along D, and the commuting case exp'(A)[E] = E exp(A) for A = a I."""
+    rng = np.random.default_rng(22)
+    n = 6
+    a = rng.standard_normal((n, n)) * 0.5
+    g = rng.standard_normal((n, n))
+    d = rng.standard_normal((n, n))
+    if cplx:
+        a, g, d = (x + 1j * rng.standard_normal((n, n)) * s
+                   for x, s in ((a, 0.5), (g, 1.0), (d, 1.0)))
+    grad = oracle.frechet_adjoint(a, g)
+    loss = lambda x: np.real(np.trace(g.conj().T @ oracle.expm(x)))
+    h = 1e-6
+    fd = (loss(a + h * d) - loss(a - h * d)) / (2 * h)
+    assert np.real(np.vdot(grad, d)) == pytest.approx(fd, rel=1e-7)
+    # A = 0.3 I commutes with everything: L(A, E) = e^0.3 E exactly
+    assert np.allclose(oracle.frechet(0.3 * np.eye(n), d), np.exp(0.3) * d, rtol=1e-14)

@@ -107,7 +107,8 @@ struct SplitChoice {
 };
 SplitChoice split_schedule(int s, const clm_split_opts *opts) {
   SplitChoice c{(s <= 3) ? CLM_SPLIT_BF16X3 : CLM_SPLIT_BF16X6, (s >= 1 && s <= 8) ? 1 : 0};
-  if (opts && (opts->ps_split == CLM_SPLIT_BF16X3 || opts->ps_split == CLM_SPLIT_BF16X6))
+  if (opts && (opts->ps_split == CLM_SPLIT_BF16X3 || opts->ps_split == CLM_SPLIT_BF16X6 ||
+               opts->ps_split == CLM_SPLIT_FP16X3))
     c.ps_split = opts->ps_split;
   if (opts && opts->split_squarings >= 0)
     c.split_squarings = opts->split_squarings < s ? opts->split_squarings : s;
@@ -124,9 +125,10 @@ clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
       if (s < 0) s = 0;
     }
     const SplitChoice c = split_schedule(s, opts);
-    const double w_ps = (c.ps_split == CLM_SPLIT_BF16X3) ? 0.2 : 0.35;
+    const double w_ps = (c.ps_split == CLM_SPLIT_BF16X6) ? 0.35 : 0.2;
+    const double w_sq = (c.ps_split == CLM_SPLIT_FP16X3) ? 0.2 : 0.35;
     const double cost = ps_plan(taylor_degree(i), 0).num_gemms * w_ps +
-                        (s - c.split_squarings) * 1.0 + c.split_squarings * 0.35;
+                        (s - c.split_squarings) * 1.0 + c.split_squarings * w_sq;
     if (best < 0 || cost < best_cost - 1e-12 || (std::fabs(cost - best_cost) <= 1e-12 && s < best_s)) {
       best = i;
       best_cost = cost;
@@ -313,21 +315,27 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
 }
 
 // ---- the precision variant's plane workspace ----------------------------------------------
-// bfloat16 planes (up to 3 per matrix), each plane n*n padded to 256 bytes:
-//   [PR  | 3 planes]  row-major planes of the current left operand
-//   [PCA | 3 planes]  column-major planes of A/2^s (right operand of the power bank),
-//                     reused for the squarings' right operand
-//   [PCX | 3 planes]  column-major planes of X = (A/2^s)^b (right operand of the
-//                     Paterson-Stockmeyer Horner steps)
+// 16-bit planes (up to 3 per matrix), each plane n*n padded to 256 bytes, with
+// the power-of-two exponents the half format scales rows / columns by:
+//   [PR  | 3 planes | n ints]  row-major planes of the current left operand
+//   [PCA | 3 planes | n ints]  column-major planes of A/2^s (right operand of the
+//                              power bank), reused for the squarings' right operand
+//   [PCX | 3 planes | n ints]  column-major planes of X = (A/2^s)^b (right operand
+//                              of the Paterson-Stockmeyer Horner steps)
+struct PlaneSet {
+  unsigned short *p = nullptr;
+  int *e = nullptr;  // half format: per-row (row-major set) or per-column exponents
+};
 struct SplitWs {
-  unsigned short *PR = nullptr, *PCA = nullptr, *PCX = nullptr;
+  PlaneSet PR, PCA, PCX;
   long pstride = 0;
   size_t carve(void *base, int n) {
     Layout l(base);
     pstride = static_cast<long>(Layout::align(static_cast<size_t>(n) * n * 2) / 2);
-    PR = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
-    PCA = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
-    PCX = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
+    for (PlaneSet *ps : {&PR, &PCA, &PCX}) {
+      ps->p = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
+      ps->e = l.take<int>(static_cast<size_t>(n));
+    }
     return l.total();
   }
 };
@@ -338,16 +346,26 @@ size_t expm_split_buffer_size(int n) {
   return ws.carve(nullptr, n) + sp.carve(nullptr, n);
 }
 
+// planes of X as a right operand (pc, column-major) and / or a left operand (pr,
+// row-major); the half format first finds the row / column exponents
 clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
-                          unsigned short *pc, unsigned short *pr, long pstride) {
-  CLM_TRY_HIP(launch_split_planes(st, split, n, n, X, ldx, pc, pr, pstride));
+                          const PlaneSet *pc, const PlaneSet *pr, long pstride) {
+  if (split == CLM_SPLIT_FP16X3) {
+    CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr, pc ? pc->e : nullptr));
+    CLM_TRY_HIP(launch_split_planes_half(st, n, n, X, ldx, pc ? pc->p : nullptr,
+                                         pr ? pr->p : nullptr, pstride, pr ? pr->e : nullptr,
+                                         pc ? pc->e : nullptr));
+  } else {
+    CLM_TRY_HIP(launch_split_planes(st, split, n, n, X, ldx, pc ? pc->p : nullptr,
+                                    pr ? pr->p : nullptr, pstride));
+  }
   return CLM_STATUS_SUCCESS;
 }
 
 // the split product C = PR-planes x PC-planes (+ epilogue in g)
-clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const unsigned short *pa,
-                         const unsigned short *pb, long pstride) {
-  CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, pa, pb, pstride));
+clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const PlaneSet &a,
+                         const PlaneSet &b, long pstride) {
+  CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, a.p, b.p, pstride, a.e, b.e));
   return CLM_STATUS_SUCCESS;
 }
 
@@ -361,7 +379,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   const clm_ps_plan p = ps_plan(degree, s_requested);
   const bool need_a_planes = p.num_powers > 0 || (p.r >= 1 && p.s < 2);
   if (need_a_planes)
-    CLM_TRY(split_planes(st, split, n, A, lda, sp.PCA, p.num_powers > 0 ? sp.PR : nullptr,
+    CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, p.num_powers > 0 ? &sp.PR : nullptr,
                          sp.pstride));
   auto plain = [&](double *C, long ldc) {
     GemmArgs<double> g;
@@ -374,15 +392,15 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   for (int k = 2; k <= p.num_powers + 1; ++k) {
     if (k > 2)
       CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(k - 3) * stride, n, nullptr,
-                           sp.PR, sp.pstride));
+                           &sp.PR, sp.pstride));
     CLM_TRY(gemm_planes(st, plain(bank + static_cast<long>(k - 2) * stride, n), split, sp.PR,
                         sp.PCA, sp.pstride));
   }
-  const unsigned short *pcx = sp.PCA;
+  const PlaneSet *pcx = &sp.PCA;
   if (p.r >= 1 && p.s >= 2) {
-    CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(p.s - 2) * stride, n, sp.PCX,
+    CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(p.s - 2) * stride, n, &sp.PCX,
                          nullptr, sp.pstride));
-    pcx = sp.PCX;
+    pcx = &sp.PCX;
   }
   double *buf[2] = {P, scratch};
   const long ld[2] = {ldp, ld_scratch};
@@ -395,7 +413,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
                                  terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
   for (int j = p.r - 1; j >= 0; --j) {
     const int next = 1 - cur;
-    CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, sp.PR, sp.pstride));
+    CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp.pstride));
     GemmArgs<double> g = plain(buf[next], ld[next]);
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
@@ -404,7 +422,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
     g.bank = bank;
     g.bank_stride = stride;
     g.ldbank = n;
-    CLM_TRY(gemm_planes(st, g, split, sp.PR, pcx, sp.pstride));
+    CLM_TRY(gemm_planes(st, g, split, sp.PR, *pcx, sp.pstride));
     cur = next;
   }
   return CLM_STATUS_SUCCESS;
@@ -484,13 +502,15 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
     const bool split_sq = k >= p.s - p.split_squarings;
     if constexpr (std::is_same_v<T, double>) {
       if (split_sq) {
-        CLM_TRY(split_planes(st, CLM_SPLIT_BF16X6, n, cur, ldc, sp.PCA, sp.PR, sp.pstride));
+        // squarings split 6-way in bfloat16, or in halves when the polynomial is
+        const int sq = (p.ps_split == CLM_SPLIT_FP16X3) ? CLM_SPLIT_FP16X3 : CLM_SPLIT_BF16X6;
+        CLM_TRY(split_planes(st, sq, n, cur, ldc, &sp.PCA, &sp.PR, sp.pstride));
         GemmArgs<double> g;
         g.M = g.N = g.K = n;
         g.C = nxt;
         g.ldc = ldn;
         g.alpha = 1.0;
-        CLM_TRY(gemm_planes(st, g, CLM_SPLIT_BF16X6, sp.PR, sp.PCA, sp.pstride));
+        CLM_TRY(gemm_planes(st, g, sq, sp.PR, sp.PCA, sp.pstride));
         std::swap(cur, nxt);
         std::swap(ldc, ldn);
         continue;
@@ -748,7 +768,7 @@ clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const do
                               double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
                               const clm_split_opts *opts, clm_expm_plan *plan) {
   if (opts && ((opts->ps_split != 0 && opts->ps_split != CLM_SPLIT_BF16X3 &&
-                opts->ps_split != CLM_SPLIT_BF16X6) ||
+                opts->ps_split != CLM_SPLIT_BF16X6 && opts->ps_split != CLM_SPLIT_FP16X3) ||
                opts->split_squarings < -1))
     return CLM_STATUS_INVALID_VALUE;
   return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,

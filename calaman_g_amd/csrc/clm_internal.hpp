@@ -171,7 +171,17 @@ hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols
 // PB = column-major planes of B (K x N); g.A / g.B are not read
 hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
                                     const unsigned short *PA, const unsigned short *PB,
-                                    long pstride);
+                                    long pstride, const int *row_e = nullptr,
+                                    const int *col_e = nullptr);
+// IEEE-half planes (split = CLM_SPLIT_FP16X3): the largest-magnitude exponent of
+// every row (row_e) and column (col_e) of X, either may be null ...
+hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const double *X, long ldx,
+                                 int *row_e, int *col_e);
+// ... and the two half planes of X scaled by them: pc column-major by col_e
+// (right operand), pr row-major by row_e (left operand); either may be null
+hipError_t launch_split_planes_half(hipStream_t stream, int rows, int cols, const double *X,
+                                    long ldx, unsigned short *pc, unsigned short *pr, long pstride,
+                                    const int *row_e, const int *col_e);
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g);
 template <typename T>
 hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);

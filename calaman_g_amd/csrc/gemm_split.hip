@@ -31,6 +31,7 @@ namespace clm {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSplitBK = 32;
@@ -304,6 +305,108 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---- IEEE-half planes with power-of-two scales -----------------------------------------
+// The half format has 11 significant bits (bf16: 8) but a narrow exponent range,
+// so each row of a left operand and each column of a right operand is scaled by
+// a power of two that brings its largest magnitude into [2^13, 2^14) before the
+// split: x' = 2^e x, h = half(fl32(x')), m = half(fl32(x') - h). Two terms carry
+// ~22 bits of x, and the three products hh + hm + mh keep ~22 bits of the
+// product (the dropped mm term is 2^-22 relative) -- close to the 6-product
+// bfloat16 split at half its MFMA work. Products of scaled halves stay far below
+// the FP32 range (2^28 per term); the epilogue divides the accumulator by
+// 2^(e_row + e_col), exactly.
+constexpr int kNoExponent = -(1 << 30);  // an all-zero row / column
+
+// exponent of the largest |x| of each row (row_e) and column (col_e) of X:
+// 64 x 64 tiles, one atomicMax per row and column and tile
+__global__ void __launch_bounds__(256)
+    max_exponent_kernel(int rows, int cols, const double *__restrict__ X, long ldx, int *row_e,
+                        int *col_e) {
+  __shared__ int red[4][64];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = i0 + lane;
+  int rmax = kNoExponent;
+  for (int jj = wave; jj < 64; jj += 4) {
+    const int j = j0 + jj;
+    int e = kNoExponent;
+    if (i < rows && j < cols) {
+      const double v = fabs(X[i + static_cast<long>(j) * ldx]);
+      if (v > 0.0) e = ilogb(v);
+    }
+    rmax = max(rmax, e);
+    int c = e;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
+    if (col_e && lane == 0 && j < cols && c > kNoExponent) atomicMax(&col_e[j], c);
+  }
+  red[wave][lane] = rmax;
+  __syncthreads();
+  if (row_e && wave == 0 && i < rows) {
+    const int r = max(max(red[0][lane], red[1][lane]), max(red[2][lane], red[3][lane]));
+    if (r > kNoExponent) atomicMax(&row_e[i], r);
+  }
+}
+
+// the scale exponent of a row / column whose largest |x| has exponent e
+__device__ __forceinline__ int half_scale(int e) { return e == kNoExponent ? 0 : 13 - e; }
+
+__device__ __forceinline__ unsigned short half_bits(float f) {
+  const _Float16 h = static_cast<_Float16>(f);
+  return __builtin_bit_cast(unsigned short, h);
+}
+__device__ __forceinline__ float half_value(unsigned short b) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, b));
+}
+
+__device__ __forceinline__ void split_half(double x, int scale, unsigned short (&p)[2]) {
+  const float f = static_cast<float>(ldexp(x, scale));
+  p[0] = half_bits(f);
+  p[1] = half_bits(f - half_value(p[0]));
+}
+
+// half planes of X: pc column-major scaled by col_e (right operand), pr
+// row-major scaled by row_e (left operand); the same tiling as split_planes_kernel
+__global__ void __launch_bounds__(256)
+    split_planes_half_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
+                             unsigned short *__restrict__ pc, unsigned short *__restrict__ pr,
+                             long pstride, const int *__restrict__ row_e,
+                             const int *__restrict__ col_e) {
+  __shared__ unsigned short tile[2][64][65];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int jj = ty; jj < 64; jj += 4) {
+    const int i = i0 + tx, j = j0 + jj;
+    unsigned short p[2] = {0, 0}, q[2] = {0, 0};
+    if (i < rows && j < cols) {
+      const double x = X[i + (long)j * ldx];
+      if (pc) {
+        split_half(x, half_scale(col_e[j]), p);
+        pc[i + (long)j * rows] = p[0];
+        pc[pstride + i + (long)j * rows] = p[1];
+      }
+      if (pr) split_half(x, half_scale(row_e[i]), q);
+    }
+    tile[0][jj][tx] = q[0];
+    tile[1][jj][tx] = q[1];
+  }
+  if (!pr) return;
+  __syncthreads();
+  for (int ii = ty; ii < 64; ii += 4) {
+    const int i = i0 + ii, j = j0 + tx;
+    if (i < rows && j < cols) {
+      pr[(long)i * cols + j] = tile[0][tx][ii];
+      pr[pstride + (long)i * cols + j] = tile[1][tx][ii];
+    }
+  }
+}
+
+__global__ void fill_exponents_kernel(int *a, int na, int *b, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a && i < na) a[i] = kNoExponent;
+  if (b && i < nb) b[i] = kNoExponent;
+}
+
 constexpr int kPlaneA = 128 * kSplitRow;  // bf16 per A plane in LDS (128 rows)
 constexpr int kPlaneB = 128 * kSplitRow;
 
@@ -353,21 +456,40 @@ __device__ __forceinline__ void planes_sstore(unsigned short *S, int tid, const 
     }
 }
 
-template <int SPLIT>
+// the 16x16x32 MFMA of a plane format: bf16 or (FMT = 1) IEEE half
+template <int FMT>
+struct PlaneMfma;
+template <>
+struct PlaneMfma<0> {
+  using vec = bf16x8;
+  __device__ static f4 mma(vec a, vec b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct PlaneMfma<1> {
+  using vec = f16x8;
+  __device__ static f4 mma(vec a, vec b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int SPLIT, int FMT>
 __device__ __forceinline__ void planes_compute(const unsigned short *S, int lane, int wm0, int wn0,
                                                f4 (&acc)[4][4]) {
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  using V = typename PlaneMfma<FMT>::vec;
   const int koff = 8 * (lane >> 4);
-  bf16x8 a[NPL][4], b[NPL][4];
+  V a[NPL][4], b[NPL][4];
 #pragma unroll
   for (int t = 0; t < NPL; ++t) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      a[t][i] = *reinterpret_cast<const bf16x8 *>(
+      a[t][i] = *reinterpret_cast<const V *>(
           &S[t * kPlaneA + (wm0 + i * 16 + (lane & 15)) * kSplitRow + koff]);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      b[t][j] = *reinterpret_cast<const bf16x8 *>(
+      b[t][j] = *reinterpret_cast<const V *>(
           &S[NPL * kPlaneA + t * kPlaneB + (wn0 + j * 16 + (lane & 15)) * kSplitRow + koff]);
   }
 #pragma unroll
@@ -375,24 +497,25 @@ __device__ __forceinline__ void planes_compute(const unsigned short *S, int lane
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if constexpr (SPLIT == 6) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = PlaneMfma<FMT>::mma(a[2][i], b[0][j], acc[i][j]);
+        acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[1][j], acc[i][j]);
+        acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[2][j], acc[i][j]);
       }
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+      acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[0][j], acc[i][j]);
+      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[1][j], acc[i][j]);
+      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
     }
 }
 
 // C = alpha * A * B (+ epilogue) from the planes of A (row-major, row stride K)
 // and of B (column-major, column stride K). Same tile, wave and MFMA shape as
 // dgemm_split_bf16; LDS double-buffered, one barrier per K slice of 32.
-template <int SPLIT, bool kFast>
+template <int SPLIT, bool kFast, int FMT>
 __global__ void __launch_bounds__(256)
     dgemm_split_planes(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
                        const unsigned short *__restrict__ PB, const long pstride,
-                       const int tiles_m, const int tiles_n) {
+                       const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
+                       const int *__restrict__ col_e) {
   constexpr int BM = 128, BN = 128, WM = 64, WN = 64;
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -429,11 +552,11 @@ __global__ void __launch_bounds__(256)
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
       planes_gload<NPL, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * kSplitBK, tid, ra, rb);
-      planes_compute<SPLIT>(L + cur * STAGE, lane, wm0, wn0, acc);
+      planes_compute<SPLIT, FMT>(L + cur * STAGE, lane, wm0, wn0, acc);
       planes_sstore<NPL>(L + (cur ^ 1) * STAGE, tid, ra, rb);
       __syncthreads();
     }
-    planes_compute<SPLIT>(L + ((KT - 1) & 1) * STAGE, lane, wm0, wn0, acc);
+    planes_compute<SPLIT, FMT>(L + ((KT - 1) & 1) * STAGE, lane, wm0, wn0, acc);
   }
 
 #pragma unroll
@@ -444,8 +567,12 @@ __global__ void __launch_bounds__(256)
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm0 + i * 16 + d_row(lane, r);
         const int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (kFast || (row < g.M && col < g.N))
-          epilogue_store_s(g, row, col, static_cast<double>(acc[i][j][r]));
+        if (kFast || (row < g.M && col < g.N)) {
+          double v = static_cast<double>(acc[i][j][r]);
+          // half planes carry power-of-two row / column scales: undo them exactly
+          if constexpr (FMT == 1) v = ldexp(v, -(half_scale(row_e[row]) + half_scale(col_e[col])));
+          epilogue_store_s(g, row, col, v);
+        }
       }
 }
 
@@ -484,27 +611,47 @@ hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols
 
 hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
                                     const unsigned short *PA, const unsigned short *PB,
-                                    long pstride) {
+                                    long pstride, const int *row_e, const int *col_e) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
   const bool fast = g.M % 128 == 0 && g.N % 128 == 0 && g.K % kSplitBK == 0 &&
                     aligned16s(PA) && aligned16s(PB) && (pstride % 8) == 0;
   const dim3 grid(tiles_m * tiles_n), block(256);
-  if (split == 6) {
-    if (fast)
-      hipLaunchKernelGGL((dgemm_split_planes<6, true>), grid, block, 0, stream, g, PA, PB, pstride,
-                         tiles_m, tiles_n);
-    else
-      hipLaunchKernelGGL((dgemm_split_planes<6, false>), grid, block, 0, stream, g, PA, PB,
-                         pstride, tiles_m, tiles_n);
+#define CLM_SPLIT_LAUNCH(S, F, FMT)                                                           \
+  hipLaunchKernelGGL((dgemm_split_planes<S, F, FMT>), grid, block, 0, stream, g, PA, PB, pstride, \
+                     tiles_m, tiles_n, row_e, col_e)
+  if (split == CLM_SPLIT_FP16X3) {
+    if (fast) CLM_SPLIT_LAUNCH(3, true, 1);
+    else CLM_SPLIT_LAUNCH(3, false, 1);
+  } else if (split == CLM_SPLIT_BF16X6) {
+    if (fast) CLM_SPLIT_LAUNCH(6, true, 0);
+    else CLM_SPLIT_LAUNCH(6, false, 0);
   } else {
-    if (fast)
-      hipLaunchKernelGGL((dgemm_split_planes<3, true>), grid, block, 0, stream, g, PA, PB, pstride,
-                         tiles_m, tiles_n);
-    else
-      hipLaunchKernelGGL((dgemm_split_planes<3, false>), grid, block, 0, stream, g, PA, PB,
-                         pstride, tiles_m, tiles_n);
+    if (fast) CLM_SPLIT_LAUNCH(3, true, 0);
+    else CLM_SPLIT_LAUNCH(3, false, 0);
   }
+#undef CLM_SPLIT_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const double *X, long ldx,
+                                 int *row_e, int *col_e) {
+  if (rows <= 0 || cols <= 0 || (!row_e && !col_e)) return hipSuccess;
+  const int nmax = rows > cols ? rows : cols;
+  hipLaunchKernelGGL(fill_exponents_kernel, dim3((nmax + 255) / 256), dim3(256), 0, stream, row_e,
+                     rows, col_e, cols);
+  hipLaunchKernelGGL(max_exponent_kernel, dim3((rows + 63) / 64, (cols + 63) / 64), dim3(256), 0,
+                     stream, rows, cols, X, ldx, row_e, col_e);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_planes_half(hipStream_t stream, int rows, int cols, const double *X,
+                                    long ldx, unsigned short *pc, unsigned short *pr, long pstride,
+                                    const int *row_e, const int *col_e) {
+  if (rows <= 0 || cols <= 0 || (!pc && !pr)) return hipSuccess;
+  const dim3 grid((rows + 63) / 64, (cols + 63) / 64), block(256);
+  hipLaunchKernelGGL(split_planes_half_kernel, grid, block, 0, stream, rows, cols, X, ldx, pc, pr,
+                     pstride, row_e, col_e);
   return hipGetLastError();
 }
 

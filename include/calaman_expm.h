@@ -170,6 +170,10 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
  * relative error <= 1e-6 against FP64. */
 #define CLM_SPLIT_BF16X3 3
 #define CLM_SPLIT_BF16X6 6
+/* IEEE half planes, 2 terms, 3 products, every row of a left operand and column
+ * of a right operand scaled by a power of two into the half range first (~22
+ * bits per product, between X3 and X6 at the cost of X3) */
+#define CLM_SPLIT_FP16X3 13
 typedef struct clm_split_opts {
   int ps_split;        /* 0 = auto (X3 when s <= 3, else X6), or CLM_SPLIT_BF16X3 / X6 */
   int split_squarings; /* -1 = auto (1 when 1 <= s <= 8, else 0), else 0..s */

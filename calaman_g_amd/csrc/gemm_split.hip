@@ -407,21 +407,59 @@ __global__ void fill_exponents_kernel(int *a, int na, int *b, int nb) {
   if (b && i < nb) b[i] = kNoExponent;
 }
 
-constexpr int kPlaneA = 128 * kSplitRow;  // bf16 per A plane in LDS (128 rows)
-constexpr int kPlaneB = 128 * kSplitRow;
+// the 16x16x32 MFMA of a plane format: bf16 or (FMT = 1) IEEE half
+template <int FMT>
+struct PlaneMfma;
+template <>
+struct PlaneMfma<0> {
+  using vec = bf16x8;
+  __device__ static f4 mma(vec a, vec b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct PlaneMfma<1> {
+  using vec = f16x8;
+  __device__ static f4 mma(vec a, vec b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
 
-template <int NPL, bool kFast>
-__device__ __forceinline__ void planes_gload(const GemmArgs<double> &g, const unsigned short *PA,
-                                             const unsigned short *PB, long pstride, int m0, int n0,
-                                             int k0, int tid, uint4 (&ra)[NPL][2],
-                                             uint4 (&rb)[NPL][2]) {
+// ------------------------------------------------------------------------------------
+// The plane product with more waves and deeper K slices. A 128 x 128 tile of C
+// is shared by NW waves (NW = 8: a 2 x 4 grid of 64 x 32 wave tiles; NW = 4:
+// 2 x 2 of 64 x 64) and each LDS stage holds KS MFMA k-steps (32 KS values of
+// k). At n = 2048 there are exactly as many tiles as CUs, so the waves of one
+// workgroup are all a CU has to hide the load, LDS and MFMA latencies with:
+// eight waves give every SIMD two, and a 64-deep slice halves the barriers
+// (measured at n = 2048 against a 4-wave, 32-deep version: split 3 86 -> 68 us,
+// split 6 124 -> 115 us; a second register set for a two-slice prefetch
+// spilled to scratch and ran 2.7x slower). The k-steps run in ascending order and the products of a
+// k-step in a fixed order (smallest terms first).
+// ------------------------------------------------------------------------------------
+template <int NPL, int KS>
+struct PlaneTile {
+  static constexpr int BK = 32 * KS;
+  static constexpr int ROW = BK + 8;  // 16-byte pad per row: conflict-free b128 reads
+  static constexpr int PLANE = 128 * ROW;
+  static constexpr int STAGE = 2 * NPL * PLANE;  // A and B planes
+};
+
+template <int NPL, int KS, int NT, bool kFast>
+__device__ __forceinline__ void planes2_gload(const GemmArgs<double> &g, const unsigned short *PA,
+                                              const unsigned short *PB, long pstride, int m0,
+                                              int n0, int k0, int tid,
+                                              uint4 (&ra)[NPL][128 * 4 * KS / NT],
+                                              uint4 (&rb)[NPL][128 * 4 * KS / NT]) {
+  constexpr int CHUNKS = 4 * KS;  // 16-byte chunks per row per slice
+  constexpr int IT = 128 * CHUNKS / NT;
   const long K = g.K;
 #pragma unroll
   for (int t = 0; t < NPL; ++t)
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const int c = tid + it * 256;
-      const int r = c >> 2, kq = (c & 3) * 8;
+    for (int it = 0; it < IT; ++it) {
+      const int c = tid + it * NT;
+      const int r = c / CHUNKS, kq = (c % CHUNKS) * 8;
       const int gk = k0 + kq;
       if (kFast) {
         ra[t][it] = *reinterpret_cast<const uint4 *>(PA + t * pstride + (long)(m0 + r) * K + gk);
@@ -442,98 +480,86 @@ __device__ __forceinline__ void planes_gload(const GemmArgs<double> &g, const un
     }
 }
 
-template <int NPL>
-__device__ __forceinline__ void planes_sstore(unsigned short *S, int tid, const uint4 (&ra)[NPL][2],
-                                              const uint4 (&rb)[NPL][2]) {
+template <int NPL, int KS, int NT>
+__device__ __forceinline__ void planes2_sstore(unsigned short *S, int tid,
+                                               const uint4 (&ra)[NPL][128 * 4 * KS / NT],
+                                               const uint4 (&rb)[NPL][128 * 4 * KS / NT]) {
+  using PT = PlaneTile<NPL, KS>;
+  constexpr int CHUNKS = 4 * KS;
+  constexpr int IT = 128 * CHUNKS / NT;
 #pragma unroll
   for (int t = 0; t < NPL; ++t)
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const int c = tid + it * 256;
-      const int r = c >> 2, kq = (c & 3) * 8;
-      *reinterpret_cast<uint4 *>(&S[t * kPlaneA + r * kSplitRow + kq]) = ra[t][it];
-      *reinterpret_cast<uint4 *>(&S[NPL * kPlaneA + t * kPlaneB + r * kSplitRow + kq]) = rb[t][it];
+    for (int it = 0; it < IT; ++it) {
+      const int c = tid + it * NT;
+      const int r = c / CHUNKS, kq = (c % CHUNKS) * 8;
+      *reinterpret_cast<uint4 *>(&S[t * PT::PLANE + r * PT::ROW + kq]) = ra[t][it];
+      *reinterpret_cast<uint4 *>(&S[(NPL + t) * PT::PLANE + r * PT::ROW + kq]) = rb[t][it];
     }
 }
 
-// the 16x16x32 MFMA of a plane format: bf16 or (FMT = 1) IEEE half
-template <int FMT>
-struct PlaneMfma;
-template <>
-struct PlaneMfma<0> {
-  using vec = bf16x8;
-  __device__ static f4 mma(vec a, vec b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct PlaneMfma<1> {
-  using vec = f16x8;
-  __device__ static f4 mma(vec a, vec b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-  }
-};
-
-template <int SPLIT, int FMT>
-__device__ __forceinline__ void planes_compute(const unsigned short *S, int lane, int wm0, int wn0,
-                                               f4 (&acc)[4][4]) {
+template <int SPLIT, int FMT, int KS, int TM, int TN>
+__device__ __forceinline__ void planes2_compute(const unsigned short *S, int lane, int wm0, int wn0,
+                                                f4 (&acc)[TM][TN]) {
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  using PT = PlaneTile<NPL, KS>;
   using V = typename PlaneMfma<FMT>::vec;
-  const int koff = 8 * (lane >> 4);
-  V a[NPL][4], b[NPL][4];
 #pragma unroll
-  for (int t = 0; t < NPL; ++t) {
+  for (int ks = 0; ks < KS; ++ks) {
+    const int koff = ks * 32 + 8 * (lane >> 4);
+    V a[NPL][TM], b[NPL][TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      a[t][i] = *reinterpret_cast<const V *>(
-          &S[t * kPlaneA + (wm0 + i * 16 + (lane & 15)) * kSplitRow + koff]);
+    for (int t = 0; t < NPL; ++t) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      b[t][j] = *reinterpret_cast<const V *>(
-          &S[NPL * kPlaneA + t * kPlaneB + (wn0 + j * 16 + (lane & 15)) * kSplitRow + koff]);
-  }
+      for (int i = 0; i < TM; ++i)
+        a[t][i] = *reinterpret_cast<const V *>(
+            &S[t * PT::PLANE + (wm0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if constexpr (SPLIT == 6) {
-        acc[i][j] = PlaneMfma<FMT>::mma(a[2][i], b[0][j], acc[i][j]);
-        acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[1][j], acc[i][j]);
-        acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[2][j], acc[i][j]);
-      }
-      acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[0][j], acc[i][j]);
-      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[1][j], acc[i][j]);
-      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
+      for (int j = 0; j < TN; ++j)
+        b[t][j] = *reinterpret_cast<const V *>(
+            &S[(NPL + t) * PT::PLANE + (wn0 + j * 16 + (lane & 15)) * PT::ROW + koff]);
     }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        if constexpr (SPLIT == 6) {
+          acc[i][j] = PlaneMfma<FMT>::mma(a[2][i], b[0][j], acc[i][j]);
+          acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[1][j], acc[i][j]);
+          acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[2][j], acc[i][j]);
+        }
+        acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[0][j], acc[i][j]);
+        acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[1][j], acc[i][j]);
+        acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
+      }
+  }
 }
 
-// C = alpha * A * B (+ epilogue) from the planes of A (row-major, row stride K)
-// and of B (column-major, column stride K). Same tile, wave and MFMA shape as
-// dgemm_split_bf16; LDS double-buffered, one barrier per K slice of 32.
-template <int SPLIT, bool kFast, int FMT>
-__global__ void __launch_bounds__(256)
-    dgemm_split_planes(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
-                       const unsigned short *__restrict__ PB, const long pstride,
-                       const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
-                       const int *__restrict__ col_e) {
-  constexpr int BM = 128, BN = 128, WM = 64, WN = 64;
+template <int SPLIT, int FMT, int NW, int KS, bool kFast>
+__global__ void __launch_bounds__(NW * 64)
+    dgemm_split_planes2(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
+                        const unsigned short *__restrict__ PB, const long pstride,
+                        const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
+                        const int *__restrict__ col_e) {
+  constexpr int NT = NW * 64;
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  using PT = PlaneTile<NPL, KS>;
+  constexpr int WAVES_N = NW == 8 ? 4 : 2;
+  constexpr int WM = 64, WN = 128 / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int PLANE_A = BM * kSplitRow;
-  constexpr int PLANE_B = BN * kSplitRow;
-  constexpr int STAGE = NPL * (PLANE_A + PLANE_B);
+  constexpr int IT = 128 * 4 * KS / NT;
+  static_assert(IT >= 1 && (128 * 4 * KS) % NT == 0, "staging shape");
 
-  __shared__ __attribute__((aligned(16))) unsigned short L[2 * STAGE];
+  __shared__ __attribute__((aligned(16))) unsigned short L[2 * PT::STAGE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   int tm, tn;
   tile_coords_s(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int wm0 = (wave & 1) * WM;
-  const int wn0 = (wave >> 1) * WN;
-  const long K = g.K;
+  const int m0 = tm * 128, n0 = tn * 128;
+  const int wm0 = (wave / WAVES_N) * WM;
+  const int wn0 = (wave % WAVES_N) * WN;
 
   f4 acc[TM][TN];
 #pragma unroll
@@ -541,22 +567,20 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // 16-byte chunks: per plane, 128 rows x 4 chunks of 8 k = 512, two per lane
-  uint4 ra[NPL][2], rb[NPL][2];
-
-  const int KT = (g.K + kSplitBK - 1) / kSplitBK;
+  uint4 ra[NPL][IT], rb[NPL][IT];
+  const int KT = (g.K + PT::BK - 1) / PT::BK;
   if (KT > 0) {
-    planes_gload<NPL, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
-    planes_sstore<NPL>(L, tid, ra, rb);
+    planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
+    planes2_sstore<NPL, KS, NT>(L, tid, ra, rb);
     __syncthreads();
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
-      planes_gload<NPL, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * kSplitBK, tid, ra, rb);
-      planes_compute<SPLIT, FMT>(L + cur * STAGE, lane, wm0, wn0, acc);
-      planes_sstore<NPL>(L + (cur ^ 1) * STAGE, tid, ra, rb);
+      planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * PT::BK, tid, ra, rb);
+      planes2_compute<SPLIT, FMT, KS, TM, TN>(L + cur * PT::STAGE, lane, wm0, wn0, acc);
+      planes2_sstore<NPL, KS, NT>(L + (cur ^ 1) * PT::STAGE, tid, ra, rb);
       __syncthreads();
     }
-    planes_compute<SPLIT, FMT>(L + ((KT - 1) & 1) * STAGE, lane, wm0, wn0, acc);
+    planes2_compute<SPLIT, FMT, KS, TM, TN>(L + ((KT - 1) & 1) * PT::STAGE, lane, wm0, wn0, acc);
   }
 
 #pragma unroll
@@ -569,7 +593,6 @@ __global__ void __launch_bounds__(256)
         const int col = n0 + wn0 + j * 16 + (lane & 15);
         if (kFast || (row < g.M && col < g.N)) {
           double v = static_cast<double>(acc[i][j][r]);
-          // half planes carry power-of-two row / column scales: undo them exactly
           if constexpr (FMT == 1) v = ldexp(v, -(half_scale(row_e[row]) + half_scale(col_e[col])));
           epilogue_store_s(g, row, col, v);
         }
@@ -614,21 +637,24 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
                                     long pstride, const int *row_e, const int *col_e) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
-  const bool fast = g.M % 128 == 0 && g.N % 128 == 0 && g.K % kSplitBK == 0 &&
-                    aligned16s(PA) && aligned16s(PB) && (pstride % 8) == 0;
-  const dim3 grid(tiles_m * tiles_n), block(256);
-#define CLM_SPLIT_LAUNCH(S, F, FMT)                                                           \
-  hipLaunchKernelGGL((dgemm_split_planes<S, F, FMT>), grid, block, 0, stream, g, PA, PB, pstride, \
-                     tiles_m, tiles_n, row_e, col_e)
+  // 3-product formats: 8 waves, 64-deep slices (two LDS stages of two planes);
+  // 6 products: 8 waves, 32-deep slices (three planes)
+  const int bk = (split == CLM_SPLIT_BF16X6) ? 32 : 64;
+  const bool fast = g.M % 128 == 0 && g.N % 128 == 0 && g.K % bk == 0 && aligned16s(PA) &&
+                    aligned16s(PB) && (pstride % 8) == 0;
+  const dim3 grid(tiles_m * tiles_n), block(512);
+#define CLM_SPLIT_LAUNCH(S, FMT, KS, F)                                                       \
+  hipLaunchKernelGGL((dgemm_split_planes2<S, FMT, 8, KS, F>), grid, block, 0, stream, g, PA, PB, \
+                     pstride, tiles_m, tiles_n, row_e, col_e)
   if (split == CLM_SPLIT_FP16X3) {
-    if (fast) CLM_SPLIT_LAUNCH(3, true, 1);
-    else CLM_SPLIT_LAUNCH(3, false, 1);
+    if (fast) CLM_SPLIT_LAUNCH(3, 1, 2, true);
+    else CLM_SPLIT_LAUNCH(3, 1, 2, false);
   } else if (split == CLM_SPLIT_BF16X6) {
-    if (fast) CLM_SPLIT_LAUNCH(6, true, 0);
-    else CLM_SPLIT_LAUNCH(6, false, 0);
+    if (fast) CLM_SPLIT_LAUNCH(6, 0, 1, true);
+    else CLM_SPLIT_LAUNCH(6, 0, 1, false);
   } else {
-    if (fast) CLM_SPLIT_LAUNCH(3, true, 0);
-    else CLM_SPLIT_LAUNCH(3, false, 0);
+    if (fast) CLM_SPLIT_LAUNCH(3, 0, 2, true);
+    else CLM_SPLIT_LAUNCH(3, 0, 2, false);
   }
 #undef CLM_SPLIT_LAUNCH
   return hipGetLastError();

@@ -246,6 +246,58 @@ def batched_leg(dev, steps, warmup):
             "graph": True, "spot_check_bitwise_mismatches": mism}
 
 
+def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
+    """Forward + hand-written backward of L = Re tr(C^H exp(A)) over 512 complex64
+    generators A = -i t H, H = (G + G^H)/2 (n = 128), C unitary (QR of a Gaussian)."""
+    import numpy as np
+    import torch
+    from calaman_g_amd.expm import expm_batched_autograd, BatchPlan
+
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+    h = (g + np.conj(np.transpose(g, (0, 2, 1)))) / 2
+    a = torch.from_numpy((-1j * t * h).astype(np.complex64)).to(dev)
+    q, _ = np.linalg.qr(rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n)))
+    c = torch.from_numpy(q.astype(np.complex64)).to(dev)
+
+    def step():
+        x = a.clone().requires_grad_(True)
+        loss = (c.conj() * expm_batched_autograd(x, checkpoints=4)).sum().real
+        loss.backward()
+        return x.grad
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        grad = step()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    # algorithmic FLOP from the executed plans: forward num_gemms products, backward
+    # two per squaring, two per Horner step and power, plus recomputed squarings
+    from calaman_g_amd.expm import expm_plan
+    from calaman_g_amd import _lib
+    import calaman_g_amd as cg
+    norms = [float(np.abs(m).sum(0).max()) for m in (-1j * t * h)]
+    flop = 0.0
+    for nm in norms:
+        p = expm_plan(nm, torch.complex64)
+        ps = cg.ps_plan(p["m"])
+        sq = p["s"]
+        cstep = max(1, -(-sq // 4))
+        recompute = sum(k % cstep for k in range(sq))
+        fwd = p["num_gemms"]
+        bwd = 2 * sq + 2 * ps["r"] + 2 * ps["num_powers"] + recompute
+        flop += (fwd + bwd) * 8.0 * n ** 3
+    achieved = flop / sec / 1e12
+    return {"fwd_bwd_ms": round(sec * 1e3, 3), "batches_per_s": round(1.0 / sec, 2),
+            "matrices": b, "n": n, "dtype": "c64", "checkpoints": 4,
+            "flop_per_step": flop, "mfma_tflops": round(achieved, 2),
+            "mfma_peak_tflops": PEAK_FP32_TFLOPS, "mfma_frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "grad_finite": bool(torch.isfinite(grad).all().item())}
+
+
 def cpu_baseline(mats):
     import oracle
     threads = oracle.num_threads()
@@ -298,6 +350,8 @@ def main():
 
     batched = batched_leg(dev, max(3, args.steps // 2), 2)
     log(f"batched: {batched}")
+    adjoint = adjoint_leg(dev, max(3, args.steps // 4), 1)
+    log(f"adjoint: {adjoint}")
 
     cpu = None
     errs = None
@@ -342,7 +396,8 @@ def main():
                                  for t, p in zip(NORMS, splans)},
             "complex128_expm_per_s": round(len(NORMS) / zsec, 3),
             "complex128_ms_per_step": round(zsec * 1e3, 4),
-            "batched": batched},
+            "batched": batched,
+            "adjoint_fwd_bwd": adjoint},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1063,6 +1063,13 @@ struct clm_batch_plan {
   int ckpt = 0;
   size_t ws_bytes = 0;
   std::vector<clm::BatchDesc> host_desc;
+  // the (type, tile) groups run concurrently: group 0 on the caller's stream, the
+  // others on these side streams (default priority), forked from and joined
+  // back to the caller's stream with events -- capturable into a graph
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSide] = {};
   // the captured launch (clm_expm_batched_graph)
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -1072,6 +1079,43 @@ struct clm_batch_plan {
 
 namespace clm {
 namespace {
+
+void destroy_plan(clm_batch_plan *plan) {
+  if (!plan) return;
+  if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
+  if (plan->d_desc) (void)hipFree(plan->d_desc);
+  for (int i = 0; i < clm_batch_plan::kSide; ++i) {
+    if (plan->side[i]) (void)hipStreamDestroy(plan->side[i]);
+    if (plan->join[i]) (void)hipEventDestroy(plan->join[i]);
+  }
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
+  delete plan;
+}
+
+// Enqueues launch(g, stream) for every non-empty group: the first on st, the
+// others spread over the plan's side streams, which fork from st before and
+// join it after.
+template <typename F>
+clm_status_t for_each_group(const clm_batch_plan *plan, hipStream_t st, F launch) {
+  int groups[kGroups], ng = 0;
+  for (int g = 0; g < kGroups; ++g)
+    if (plan->group_count[g] > 0) groups[ng++] = g;
+  if (ng == 0) return CLM_STATUS_SUCCESS;
+  const int nside = std::min(ng - 1, clm_batch_plan::kSide);
+  if (nside > 0) {
+    CLM_TRY_HIP(hipEventRecord(plan->fork, st));
+    for (int i = 0; i < nside; ++i) CLM_TRY_HIP(hipStreamWaitEvent(plan->side[i], plan->fork, 0));
+  }
+  for (int k = 0; k < ng; ++k) {
+    hipStream_t s = (k == 0) ? st : plan->side[(k - 1) % clm_batch_plan::kSide];
+    CLM_TRY_HIP(launch(groups[k], s));
+  }
+  for (int i = 0; i < nside; ++i) {
+    CLM_TRY_HIP(hipEventRecord(plan->join[i], plan->side[i]));
+    CLM_TRY_HIP(hipStreamWaitEvent(st, plan->join[i], 0));
+  }
+  return CLM_STATUS_SUCCESS;
+}
 
 clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *forced_cfg,
                         clm_batch_plan **out, const clm_batch_ad_entry *ad = nullptr,
@@ -1155,6 +1199,14 @@ clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *fo
       delete plan;
       return CLM_STATUS_EXECUTION_FAILED;
     }
+  bool streams_ok = hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < clm_batch_plan::kSide && streams_ok; ++i)
+    streams_ok = hipStreamCreateWithFlags(&plan->side[i], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&plan->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!streams_ok) {
+    destroy_plan(plan);
+    return CLM_STATUS_EXECUTION_FAILED;
+  }
   plan->ws_bytes = off;
   plan->ad = ad != nullptr;
   plan->ckpt = ckpt;
@@ -1182,22 +1234,17 @@ clm_status_t launch_plan(clm_ctx *ctx, hipStream_t st, const clm_batch_plan *pla
   char *base = static_cast<char *>(work);
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, queue);
   CLM_TRY_HIP(hipGetLastError());
-  static constexpr int kTypes[4] = {CLM_DTYPE_F32, CLM_DTYPE_F64, CLM_DTYPE_C64, CLM_DTYPE_C128};
-  for (int g = 0; g < kGroups; ++g) {
+  return for_each_group(plan, st, [&](int g, hipStream_t s) -> hipError_t {
     const int count = plan->group_count[g];
-    if (count == 0) continue;
     const BatchDesc *desc = plan->d_desc + plan->group_start[g];
     const int cfg = g % 4;
-    hipError_t e;
-    switch (kTypes[g / 4]) {
-    case CLM_DTYPE_F32: e = launch_type<float>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
-    case CLM_DTYPE_F64: e = launch_type<double>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
-    case CLM_DTYPE_C64: e = launch_type<zc>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
-    default: e = launch_type<zd>(cfg, st, plan->device, desc, count, queue + g, base, results); break;
+    switch (g / 4) {
+    case 0: return launch_type<float>(cfg, s, plan->device, desc, count, queue + g, base, results);
+    case 1: return launch_type<double>(cfg, s, plan->device, desc, count, queue + g, base, results);
+    case 2: return launch_type<zc>(cfg, s, plan->device, desc, count, queue + g, base, results);
+    default: return launch_type<zd>(cfg, s, plan->device, desc, count, queue + g, base, results);
     }
-    CLM_TRY_HIP(e);
-  }
-  return CLM_STATUS_SUCCESS;
+  });
 }
 
 // the adjoint sweep of an adjoint plan; reads the tape the last forward launch of
@@ -1212,30 +1259,19 @@ clm_status_t launch_backward(clm_ctx *ctx, hipStream_t st, const clm_batch_plan 
   char *base = static_cast<char *>(work);
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, queue);
   CLM_TRY_HIP(hipGetLastError());
-  static constexpr int kTypes[4] = {CLM_DTYPE_F32, CLM_DTYPE_F64, CLM_DTYPE_C64, CLM_DTYPE_C128};
-  for (int g = 0; g < kGroups; ++g) {
+  return for_each_group(plan, st, [&](int g, hipStream_t s) -> hipError_t {
     const int count = plan->group_count[g];
-    if (count == 0) continue;
     const BatchDesc *desc = plan->d_desc + plan->group_start[g];
     const int cfg = g % 4;
-    hipError_t e;
-    switch (kTypes[g / 4]) {
-    case CLM_DTYPE_F32: e = launch_adjoint_type<float>(cfg, st, plan->device, desc, count, queue + g, base); break;
-    case CLM_DTYPE_F64: e = launch_adjoint_type<double>(cfg, st, plan->device, desc, count, queue + g, base); break;
-    case CLM_DTYPE_C64: e = launch_adjoint_type<zc>(cfg, st, plan->device, desc, count, queue + g, base); break;
-    default: e = launch_adjoint_type<zd>(cfg, st, plan->device, desc, count, queue + g, base); break;
+    switch (g / 4) {
+    case 0: return launch_adjoint_type<float>(cfg, s, plan->device, desc, count, queue + g, base);
+    case 1: return launch_adjoint_type<double>(cfg, s, plan->device, desc, count, queue + g, base);
+    case 2: return launch_adjoint_type<zc>(cfg, s, plan->device, desc, count, queue + g, base);
+    default: return launch_adjoint_type<zd>(cfg, s, plan->device, desc, count, queue + g, base);
     }
-    CLM_TRY_HIP(e);
-  }
-  return CLM_STATUS_SUCCESS;
+  });
 }
 
-void destroy_plan(clm_batch_plan *plan) {
-  if (!plan) return;
-  if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
-  if (plan->d_desc) (void)hipFree(plan->d_desc);
-  delete plan;
-}
 
 // one homogeneous tuning batch: `count` matrices of order n and type dtype, tile
 // configuration cfg; returns the median of `reps` timed launches in ms

@@ -74,7 +74,8 @@ class FloatComplex(ctypes.Structure):
 
 
 BATCH_BUCKETS = 8
-BATCH_TILE_32, BATCH_TILE_64, BATCH_TILE_128x64, BATCH_TILE_128 = 0, 1, 2, 3
+BATCH_TILE_32, BATCH_TILE_64, BATCH_TILE_128x64, BATCH_TILE_128, BATCH_TILE_128W16 = 0, 1, 2, 3, 4
+BATCH_TILES = 5
 
 
 class BatchEntry(ctypes.Structure):

@@ -59,10 +59,10 @@ namespace {
 
 using namespace tile;
 
-constexpr int kNT = 256;
 constexpr int kBuckets = CLM_BATCH_BUCKETS;
 constexpr int kMaxBatchN = 4096;
-constexpr int kGroups = 16;          // (element type, tile) pairs: 4 x 4
+constexpr int kTiles = CLM_BATCH_TILES;
+constexpr int kGroups = 4 * kTiles;  // (element type, tile) pairs
 constexpr size_t kHeadBytes = 256;  // the groups' queue counters at the head of the workspace
 
 template <typename T>
@@ -119,14 +119,14 @@ size_t tape_bytes(int n, int ckpt) {
 }
 
 // ---- LDS budget per tile configuration -------------------------------------------
-template <typename S, int BM, int BN, int BK>
+template <typename S, int BM, int BN, int BK, int NT = 256>
 constexpr size_t real_lds() {
-  using TL = RTile<S, BM, BN, kNT, BK>;
+  using TL = RTile<S, BM, BN, NT, BK>;
   return 2 * (TL::A_STAGE + TL::B_STAGE) * sizeof(S);
 }
-template <typename C, int BM, int BN>
+template <typename C, int BM, int BN, int NT = 256>
 constexpr size_t cplx_lds() {
-  using TL = CTile<C, BM, BN, kNT, kBK>;
+  using TL = CTile<C, BM, BN, NT, kBK>;
   return 2 * TL::STAGE * sizeof(typename CxReal<C>::type);
 }
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
@@ -142,15 +142,23 @@ constexpr int small_bk() {
 // at 64 x 64, 128 x 128 exists for real FP32 only (others take their largest)
 template <typename T>
 constexpr int effective_cfg(int cfg) {
+  if (cfg == CLM_BATCH_TILE_128W16) return cfg;
   if constexpr (std::is_same_v<T, zd>) return cfg > CLM_BATCH_TILE_64 ? CLM_BATCH_TILE_64 : cfg;
   else if constexpr (std::is_same_v<T, float>) return cfg;
   else return cfg > CLM_BATCH_TILE_128x64 ? CLM_BATCH_TILE_128x64 : cfg;
 }
 
+// lanes per workgroup: 16 waves for the 128 x 128 tile of 32 x 32 wave tiles
+// (a CU's worth of waves on one matrix), 4 for the others
+constexpr int threads_of(int cfg) { return cfg == CLM_BATCH_TILE_128W16 ? 1024 : 256; }
+
 template <typename T, int CFG>
 constexpr size_t lds_bytes() {
   constexpr int c = effective_cfg<T>(CFG);
-  if constexpr (is_cplx<T>()) {
+  if constexpr (c == CLM_BATCH_TILE_128W16) {
+    if constexpr (is_cplx<T>()) return cplx_lds<T, 128, 128, 1024>();
+    else return real_lds<T, 128, 128, 32, 1024>();
+  } else if constexpr (is_cplx<T>()) {
     if constexpr (c == CLM_BATCH_TILE_32) return cplx_lds<T, 32, 32>();
     else if constexpr (c == CLM_BATCH_TILE_64) return cplx_lds<T, 64, 64>();
     else return cplx_lds<T, 128, 64>();
@@ -179,12 +187,12 @@ __device__ __forceinline__ double uniform(double v) {
 // ---- workgroup-wide building blocks ---------------------------------------------------
 // C = A*B (+ epilogue of g) over the whole n x n output, tile by tile, by all
 // 256 lanes; ends with a barrier, so C is complete for every lane afterwards.
-template <typename S, int BM, int BN, int WM, int WN, int BK>
+template <typename S, int BM, int BN, int WM, int WN, int BK, int NT = 256>
 __device__ void wg_gemm_real(const GemmArgs<S> &g, char *lds) {
-  using TL = RTile<S, BM, BN, kNT, BK>;
+  using TL = RTile<S, BM, BN, NT, BK>;
   using V = typename Mfma<S>::vec;
   constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
+  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
   constexpr int TM = WM / 16, TN = WN / 16;
   S *As = reinterpret_cast<S *>(lds);
   S *Bs = As + 2 * TL::A_STAGE;
@@ -202,20 +210,20 @@ __device__ void wg_gemm_real(const GemmArgs<S> &g, char *lds) {
     V ra[TL::A_V], rb[TL::B_V];
     __syncthreads();  // the previous tile's last slice is consumed before restaging
     if (KT > 0) {
-      rgemm_gload<S, BM, BN, kNT, BK, false>(g, m0, n0, 0, tid, ra, rb);
-      rgemm_sstore<S, BM, BN, kNT, BK>(As, Bs, tid, ra, rb);
+      rgemm_gload<S, BM, BN, NT, BK, false>(g, m0, n0, 0, tid, ra, rb);
+      rgemm_sstore<S, BM, BN, NT, BK>(As, Bs, tid, ra, rb);
       __syncthreads();
       for (int kt = 0; kt < KT - 1; ++kt) {
         const int cur = kt & 1;
-        rgemm_gload<S, BM, BN, kNT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
-        rgemm_compute<S, BM, BN, WM, WN, kNT, BK>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE,
+        rgemm_gload<S, BM, BN, NT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
+        rgemm_compute<S, BM, BN, WM, WN, NT, BK>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE,
                                                   lane, wm0, wn0, acc);
-        rgemm_sstore<S, BM, BN, kNT, BK>(As + (cur ^ 1) * TL::A_STAGE,
+        rgemm_sstore<S, BM, BN, NT, BK>(As + (cur ^ 1) * TL::A_STAGE,
                                          Bs + (cur ^ 1) * TL::B_STAGE, tid, ra, rb);
         __syncthreads();
       }
       const int last = (KT - 1) & 1;
-      rgemm_compute<S, BM, BN, WM, WN, kNT, BK>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE,
+      rgemm_compute<S, BM, BN, WM, WN, NT, BK>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE,
                                                 lane, wm0, wn0, acc);
     }
 #pragma unroll
@@ -232,13 +240,13 @@ __device__ void wg_gemm_real(const GemmArgs<S> &g, char *lds) {
   __syncthreads();
 }
 
-template <typename C, int BM, int BN, int WM, int WN>
+template <typename C, int BM, int BN, int WM, int WN, int NT = 256>
 __device__ void wg_gemm_cplx(const GemmArgs<C> &g, char *lds) {
   constexpr int BK = kBK;
-  using TL = CTile<C, BM, BN, kNT, BK>;
+  using TL = CTile<C, BM, BN, NT, BK>;
   using S = typename CxReal<C>::type;
   constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
+  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
   constexpr int TM = WM / 16, TN = WN / 16;
   S *L = reinterpret_cast<S *>(lds);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -258,17 +266,17 @@ __device__ void wg_gemm_cplx(const GemmArgs<C> &g, char *lds) {
     C ra[TL::A_E], rb[TL::B_E];
     __syncthreads();
     if (KT > 0) {
-      cgemm_gload<C, BM, BN, kNT, BK, false>(g, m0, n0, 0, tid, ra, rb);
-      cgemm_sstore<C, BM, BN, kNT, BK>(L, tid, ra, rb);
+      cgemm_gload<C, BM, BN, NT, BK, false>(g, m0, n0, 0, tid, ra, rb);
+      cgemm_sstore<C, BM, BN, NT, BK>(L, tid, ra, rb);
       __syncthreads();
       for (int kt = 0; kt < KT - 1; ++kt) {
         const int cur = kt & 1;
-        cgemm_gload<C, BM, BN, kNT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
-        cgemm_compute<C, BM, BN, WM, WN, kNT, BK>(L + cur * TL::STAGE, lane, wm0, wn0, cr, ci);
-        cgemm_sstore<C, BM, BN, kNT, BK>(L + (cur ^ 1) * TL::STAGE, tid, ra, rb);
+        cgemm_gload<C, BM, BN, NT, BK, false>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
+        cgemm_compute<C, BM, BN, WM, WN, NT, BK>(L + cur * TL::STAGE, lane, wm0, wn0, cr, ci);
+        cgemm_sstore<C, BM, BN, NT, BK>(L + (cur ^ 1) * TL::STAGE, tid, ra, rb);
         __syncthreads();
       }
-      cgemm_compute<C, BM, BN, WM, WN, kNT, BK>(L + ((KT - 1) & 1) * TL::STAGE, lane, wm0, wn0,
+      cgemm_compute<C, BM, BN, WM, WN, NT, BK>(L + ((KT - 1) & 1) * TL::STAGE, lane, wm0, wn0,
                                                 cr, ci);
     }
 #pragma unroll
@@ -289,7 +297,10 @@ __device__ void wg_gemm_cplx(const GemmArgs<C> &g, char *lds) {
 template <typename T, int CFG>
 __device__ void wg_gemm(const GemmArgs<T> &g, char *lds) {
   constexpr int c = effective_cfg<T>(CFG);
-  if constexpr (is_cplx<T>()) {
+  if constexpr (c == CLM_BATCH_TILE_128W16) {
+    if constexpr (is_cplx<T>()) wg_gemm_cplx<T, 128, 128, 32, 32, 1024>(g, lds);
+    else wg_gemm_real<T, 128, 128, 32, 32, 32, 1024>(g, lds);
+  } else if constexpr (is_cplx<T>()) {
     if constexpr (c == CLM_BATCH_TILE_32) wg_gemm_cplx<T, 32, 32, 16, 16>(g, lds);
     else if constexpr (c == CLM_BATCH_TILE_64) wg_gemm_cplx<T, 64, 64, 32, 32>(g, lds);
     else wg_gemm_cplx<T, 128, 64, 64, 32>(g, lds);
@@ -313,12 +324,12 @@ __device__ __forceinline__ double conj_if(double x, bool) { return x; }
 __device__ __forceinline__ zc conj_if(zc x, bool c) { return c ? zc{x.re, -x.im} : x; }
 __device__ __forceinline__ zd conj_if(zd x, bool c) { return c ? zd{x.re, -x.im} : x; }
 
-template <typename T, int BM, int BN, int BK, bool TA, bool TB>
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, int NT>
 __device__ __forceinline__ void op_gload(const GemmArgs<T> &g, int m0, int n0, int k0, int tid,
-                                         T (&ra)[BM * BK / kNT], T (&rb)[BN * BK / kNT]) {
+                                         T (&ra)[BM * BK / NT], T (&rb)[BN * BK / NT]) {
 #pragma unroll
-  for (int it = 0; it < BM * BK / kNT; ++it) {
-    const int e = tid + it * kNT;
+  for (int it = 0; it < BM * BK / NT; ++it) {
+    const int e = tid + it * NT;
     const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
     const int gm = m0 + mm, gk = k0 + kk;
     T v = ops<T>::zero();
@@ -327,8 +338,8 @@ __device__ __forceinline__ void op_gload(const GemmArgs<T> &g, int m0, int n0, i
     ra[it] = v;
   }
 #pragma unroll
-  for (int it = 0; it < BN * BK / kNT; ++it) {
-    const int e = tid + it * kNT;
+  for (int it = 0; it < BN * BK / NT; ++it) {
+    const int e = tid + it * NT;
     const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
     const int gn = n0 + nn, gk = k0 + kk;
     T v = ops<T>::zero();
@@ -341,16 +352,16 @@ __device__ __forceinline__ void op_gload(const GemmArgs<T> &g, int m0, int n0, i
 // register -> LDS in the layouts of RTile / CTile: A at As[k][m] (row BM + 16),
 // B at Bs[n][k] (row BK + one vector); complex operands as a re plane followed by
 // an im plane of the same layout
-template <typename T, int BM, int BN, int BK, bool TA, bool TB>
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, int NT>
 __device__ __forceinline__ void op_sstore(real_t<T> *As, real_t<T> *Bs, int tid,
-                                          const T (&ra)[BM * BK / kNT],
-                                          const T (&rb)[BN * BK / kNT]) {
+                                          const T (&ra)[BM * BK / NT],
+                                          const T (&rb)[BN * BK / NT]) {
   using R = real_t<T>;
   constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
   constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
 #pragma unroll
-  for (int it = 0; it < BM * BK / kNT; ++it) {
-    const int e = tid + it * kNT;
+  for (int it = 0; it < BM * BK / NT; ++it) {
+    const int e = tid + it * NT;
     const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
     const int sidx = kk * LDA_S + mm;
     if constexpr (is_cplx<T>()) {
@@ -361,8 +372,8 @@ __device__ __forceinline__ void op_sstore(real_t<T> *As, real_t<T> *Bs, int tid,
     }
   }
 #pragma unroll
-  for (int it = 0; it < BN * BK / kNT; ++it) {
-    const int e = tid + it * kNT;
+  for (int it = 0; it < BN * BK / NT; ++it) {
+    const int e = tid + it * NT;
     const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
     const int sidx = nn * LDB_S + kk;
     if constexpr (is_cplx<T>()) {
@@ -376,12 +387,12 @@ __device__ __forceinline__ void op_sstore(real_t<T> *As, real_t<T> *Bs, int tid,
 
 // C = op(A) op(B) (+ epilogue) over the whole output by the workgroup; same
 // tiles and compute helpers as wg_gemm_real / wg_gemm_cplx
-template <typename T, int BM, int BN, int WM, int WN, int BK, bool TA, bool TB>
+template <typename T, int BM, int BN, int WM, int WN, int BK, bool TA, bool TB, int NT = 256>
 __device__ void wg_gemm_op(const GemmArgs<T> &g, char *lds) {
   using R = real_t<T>;
   constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BN / WN) == kNT / 64, "one wave per WM x WN sub-tile");
-  static_assert((BM * BK) % kNT == 0 && (BN * BK) % kNT == 0, "staging");
+  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
+  static_assert((BM * BK) % NT == 0 && (BN * BK) % NT == 0, "staging");
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
   constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
@@ -402,18 +413,18 @@ __device__ void wg_gemm_op(const GemmArgs<T> &g, char *lds) {
         cr[i][j] = Mfma<R>::zero();
         ci[i][j] = Mfma<R>::zero();
       }
-    T ra[BM * BK / kNT], rb[BN * BK / kNT];
+    T ra[BM * BK / NT], rb[BN * BK / NT];
     __syncthreads();
     for (int kt = 0; kt < KT; ++kt) {
       R *stage = L + (kt & 1) * STAGE;
       R *As = stage, *Bs = stage + NPL * A_PLANE;
-      op_gload<T, BM, BN, BK, TA, TB>(g, m0, n0, kt * BK, tid, ra, rb);
-      op_sstore<T, BM, BN, BK, TA, TB>(As, Bs, tid, ra, rb);
+      op_gload<T, BM, BN, BK, TA, TB, NT>(g, m0, n0, kt * BK, tid, ra, rb);
+      op_sstore<T, BM, BN, BK, TA, TB, NT>(As, Bs, tid, ra, rb);
       __syncthreads();
       if constexpr (is_cplx<T>()) {
-        cgemm_compute<T, BM, BN, WM, WN, kNT, BK>(stage, lane, wm0, wn0, cr, ci);
+        cgemm_compute<T, BM, BN, WM, WN, NT, BK>(stage, lane, wm0, wn0, cr, ci);
       } else {
-        rgemm_compute<T, BM, BN, WM, WN, kNT, BK>(As, Bs, lane, wm0, wn0, cr);
+        rgemm_compute<T, BM, BN, WM, WN, NT, BK>(As, Bs, lane, wm0, wn0, cr);
       }
       // the next slice goes to the other stage: no second barrier needed
     }
@@ -439,7 +450,11 @@ __device__ void wg_gemm_op(const GemmArgs<T> &g, char *lds) {
 template <typename T, int CFG, bool TA, bool TB>
 __device__ void wg_gemm_ops(const GemmArgs<T> &g, char *lds) {
   constexpr int c = effective_cfg<T>(CFG);
-  if constexpr (c == CLM_BATCH_TILE_32) {
+  if constexpr (c == CLM_BATCH_TILE_128W16) {
+    // the FP32 real tile stages 32-deep slices (one 16-byte vector per lane)
+    constexpr int bk = std::is_same_v<T, float> ? 32 : kBK;
+    wg_gemm_op<T, 128, 128, 32, 32, bk, TA, TB, 1024>(g, lds);
+  } else if constexpr (c == CLM_BATCH_TILE_32) {
     if constexpr (is_cplx<T>()) wg_gemm_op<T, 32, 32, 16, 16, kBK, TA, TB>(g, lds);
     else wg_gemm_op<T, 32, 32, 16, 16, small_bk<T>(), TA, TB>(g, lds);
   }
@@ -460,9 +475,9 @@ template <typename T>
 __device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
   using R = real_t<T>;
   R *red = reinterpret_cast<R *>(lds);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   R best = 0;
-  for (int j = wave; j < n; j += 4) {
+  for (int j = wave; j < n; j += nwv) {
     const T *col = A + static_cast<long>(j) * lda;
     R v[4];
 #pragma unroll
@@ -490,7 +505,7 @@ __device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
   if (lane == 0) red[wave] = best;
   __syncthreads();
   R r = 0;
-  for (int w = 0; w < 4; ++w) {
+  for (int w = 0; w < nwv; ++w) {
     const R v = red[w];
     if (v > r || isnan(v)) r = v;
   }
@@ -501,8 +516,8 @@ __device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
 // out = f * in, column by column (f a power of two: exact)
 template <typename T>
 __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, long ldo) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = wave; j < n; j += 4)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int j = wave; j < n; j += nwv)
     for (int i = lane; i < n; i += 64)
       out[i + static_cast<long>(j) * ldo] = ops<T>::scale(in[i + static_cast<long>(j) * ldi], f);
   __syncthreads();
@@ -513,8 +528,8 @@ __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, lo
 template <typename T>
 __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms, const T *X1,
                                long ldx1, const T *bank, long stride, long ldbank) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = wave; j < n; j += 4)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int j = wave; j < n; j += nwv)
     for (int i = lane; i < n; i += 64) {
       T acc = (i == j) ? coef[0] : ops<T>::zero();
       if (nterms > 1) acc = ops<T>::mad(coef[1], X1[i + static_cast<long>(j) * ldx1], acc);
@@ -528,8 +543,8 @@ __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms,
 // dst = src, column by column
 template <typename T>
 __device__ void wg_copy(int n, const T *src, long lds_, T *dst, long ldd) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = wave; j < n; j += 4)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int j = wave; j < n; j += nwv)
     for (int i = lane; i < n; i += 64)
       dst[i + static_cast<long>(j) * ldd] = src[i + static_cast<long>(j) * lds_];
   __syncthreads();
@@ -716,8 +731,8 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
 template <typename T>
 __device__ void wg_block_adjoint(int n, const T *bar, long ldbar, const T *c, int terms, T *Xb,
                                  const Tape<T> &tape) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = wave; j < n; j += 4)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int j = wave; j < n; j += nwv)
     for (int i = lane; i < n; i += 64) {
       const T b = bar[i + static_cast<long>(j) * ldbar];
       const long at = i + static_cast<long>(j) * n;
@@ -732,8 +747,8 @@ __device__ void wg_block_adjoint(int n, const T *bar, long ldbar, const T *c, in
 
 template <typename T>
 __device__ void wg_fill(int n, T *dst, long ld, T v) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = wave; j < n; j += 4)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int j = wave; j < n; j += nwv)
     for (int i = lane; i < n; i += 64) dst[i + static_cast<long>(j) * ld] = v;
   __syncthreads();
 }
@@ -822,7 +837,7 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
 }
 
 template <typename T, int CFG>
-__global__ void __launch_bounds__(kNT)
+__global__ void __launch_bounds__(threads_of(CFG))
     expm_adjoint_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
                         const T *coef) {
   __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T, CFG>()];
@@ -843,7 +858,7 @@ __global__ void __launch_bounds__(kNT)
 // 0, broadcast through LDS) until the queue is drained; every workgroup reaches
 // the exit once the counter passes `count`.
 template <typename T, int CFG>
-__global__ void __launch_bounds__(kNT)
+__global__ void __launch_bounds__(threads_of(CFG))
     expm_batched_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
                         clm_batch_result *results, const T *coef) {
   __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T, CFG>()];
@@ -941,7 +956,7 @@ size_t ws_bytes_of(int dtype, int n) {
 // group id of an (element type, tile) pair; dtype index F32 0, F64 1, C64 2, C128 3
 int group_of(int dtype, int cfg) {
   const int t = dtype == CLM_DTYPE_F32 ? 0 : dtype == CLM_DTYPE_F64 ? 1 : dtype == CLM_DTYPE_C64 ? 2 : 3;
-  return t * 4 + cfg;
+  return t * kTiles + cfg;
 }
 
 // workgroups resident at once on the device for one kernel instantiation
@@ -951,7 +966,7 @@ int resident_workgroups(int device) {
   int &r = cached[device & 63];
   if (r == 0) {
     int blocks = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_batched_kernel<T, CFG>, kNT, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_batched_kernel<T, CFG>, threads_of(CFG), 0) !=
             hipSuccess ||
         blocks < 1)
       blocks = 1;
@@ -969,7 +984,8 @@ hipError_t launch_group(hipStream_t st, int device, const BatchDesc *desc, int c
   const T *coef = inv_factorials<T>(device);
   if (!coef) return hipErrorInvalidValue;
   const int grid = std::min(count, resident_workgroups<T, CFG>(device));
-  hipLaunchKernelGGL((expm_batched_kernel<T, CFG>), dim3(grid), dim3(kNT), 0, st, desc, count,
+  hipLaunchKernelGGL((expm_batched_kernel<T, CFG>), dim3(grid), dim3(threads_of(CFG)), 0, st, desc,
+                     count,
                      queue, work, results, coef);
   return hipGetLastError();
 }
@@ -983,16 +999,17 @@ bool prepare_type(int cfg, int device) {
   case CLM_BATCH_TILE_32: return resident_workgroups<T, CLM_BATCH_TILE_32>(device) > 0;
   case CLM_BATCH_TILE_64: return resident_workgroups<T, CLM_BATCH_TILE_64>(device) > 0;
   case CLM_BATCH_TILE_128x64: return resident_workgroups<T, CLM_BATCH_TILE_128x64>(device) > 0;
-  default: return resident_workgroups<T, CLM_BATCH_TILE_128>(device) > 0;
+  case CLM_BATCH_TILE_128: return resident_workgroups<T, CLM_BATCH_TILE_128>(device) > 0;
+  default: return resident_workgroups<T, CLM_BATCH_TILE_128W16>(device) > 0;
   }
 }
 
 bool prepare_group(int g, int device) {
-  switch (g / 4) {
-  case 0: return prepare_type<float>(g % 4, device);
-  case 1: return prepare_type<double>(g % 4, device);
-  case 2: return prepare_type<zc>(g % 4, device);
-  default: return prepare_type<zd>(g % 4, device);
+  switch (g / kTiles) {
+  case 0: return prepare_type<float>(g % kTiles, device);
+  case 1: return prepare_type<double>(g % kTiles, device);
+  case 2: return prepare_type<zc>(g % kTiles, device);
+  default: return prepare_type<zd>(g % kTiles, device);
   }
 }
 
@@ -1005,7 +1022,7 @@ hipError_t launch_adjoint_group(hipStream_t st, int device, const BatchDesc *des
   int &resident = cached[device & 63];
   if (resident == 0) {
     int blocks = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_adjoint_kernel<T, CFG>, kNT, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, expm_adjoint_kernel<T, CFG>, threads_of(CFG), 0) !=
             hipSuccess ||
         blocks < 1)
       blocks = 1;
@@ -1014,7 +1031,8 @@ hipError_t launch_adjoint_group(hipStream_t st, int device, const BatchDesc *des
       cus = 256;
     resident = blocks * cus;
   }
-  hipLaunchKernelGGL((expm_adjoint_kernel<T, CFG>), dim3(std::min(count, resident)), dim3(kNT), 0,
+  hipLaunchKernelGGL((expm_adjoint_kernel<T, CFG>), dim3(std::min(count, resident)),
+                     dim3(threads_of(CFG)), 0,
                      st, desc, count, queue, work, coef);
   return hipGetLastError();
 }
@@ -1029,8 +1047,10 @@ hipError_t launch_adjoint_type(int cfg, hipStream_t st, int device, const BatchD
     return launch_adjoint_group<T, CLM_BATCH_TILE_64>(st, device, desc, count, queue, work);
   case CLM_BATCH_TILE_128x64:
     return launch_adjoint_group<T, CLM_BATCH_TILE_128x64>(st, device, desc, count, queue, work);
-  default:
+  case CLM_BATCH_TILE_128:
     return launch_adjoint_group<T, CLM_BATCH_TILE_128>(st, device, desc, count, queue, work);
+  default:
+    return launch_adjoint_group<T, CLM_BATCH_TILE_128W16>(st, device, desc, count, queue, work);
   }
 }
 
@@ -1044,8 +1064,10 @@ hipError_t launch_type(int cfg, hipStream_t st, int device, const BatchDesc *des
     return launch_group<T, CLM_BATCH_TILE_64>(st, device, desc, count, queue, work, results);
   case CLM_BATCH_TILE_128x64:
     return launch_group<T, CLM_BATCH_TILE_128x64>(st, device, desc, count, queue, work, results);
-  default:
+  case CLM_BATCH_TILE_128:
     return launch_group<T, CLM_BATCH_TILE_128>(st, device, desc, count, queue, work, results);
+  default:
+    return launch_group<T, CLM_BATCH_TILE_128W16>(st, device, desc, count, queue, work, results);
   }
 }
 
@@ -1237,8 +1259,8 @@ clm_status_t launch_plan(clm_ctx *ctx, hipStream_t st, const clm_batch_plan *pla
   return for_each_group(plan, st, [&](int g, hipStream_t s) -> hipError_t {
     const int count = plan->group_count[g];
     const BatchDesc *desc = plan->d_desc + plan->group_start[g];
-    const int cfg = g % 4;
-    switch (g / 4) {
+    const int cfg = g % kTiles;
+    switch (g / kTiles) {
     case 0: return launch_type<float>(cfg, s, plan->device, desc, count, queue + g, base, results);
     case 1: return launch_type<double>(cfg, s, plan->device, desc, count, queue + g, base, results);
     case 2: return launch_type<zc>(cfg, s, plan->device, desc, count, queue + g, base, results);
@@ -1262,8 +1284,8 @@ clm_status_t launch_backward(clm_ctx *ctx, hipStream_t st, const clm_batch_plan 
   return for_each_group(plan, st, [&](int g, hipStream_t s) -> hipError_t {
     const int count = plan->group_count[g];
     const BatchDesc *desc = plan->d_desc + plan->group_start[g];
-    const int cfg = g % 4;
-    switch (g / 4) {
+    const int cfg = g % kTiles;
+    switch (g / kTiles) {
     case 0: return launch_adjoint_type<float>(cfg, s, plan->device, desc, count, queue + g, base);
     case 1: return launch_adjoint_type<double>(cfg, s, plan->device, desc, count, queue + g, base);
     case 2: return launch_adjoint_type<zc>(cfg, s, plan->device, desc, count, queue + g, base);
@@ -1407,7 +1429,7 @@ clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg) {
 clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg) {
   if (!clm::valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
   for (int b = 0; b < clm::kBuckets; ++b)
-    if (cfg[b] < CLM_BATCH_TILE_32 || cfg[b] > CLM_BATCH_TILE_128) return CLM_STATUS_INVALID_VALUE;
+    if (cfg[b] < 0 || cfg[b] >= CLM_BATCH_TILES) return CLM_STATUS_INVALID_VALUE;
   std::lock_guard<std::mutex> lock(clm::g_tune_mutex);
   if (!clm::g_tune_init) clm::tune_defaults();
   for (int b = 0; b < clm::kBuckets; ++b)
@@ -1424,17 +1446,21 @@ clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dt
   CLM_TRY(clm_expm_batched_get_tuning(dtype, chosen));
   const bool single = clm::precision_of(dtype) == 0;
   const bool cplx = clm::is_complex_code(dtype);
-  // candidate tiles: 128 x 128 exists for real FP32 only, 128 x 64 for all but
-  // complex FP64
-  int max_cfg = CLM_BATCH_TILE_128x64;
-  if (single && !cplx) max_cfg = CLM_BATCH_TILE_128;
-  if (!single && cplx) max_cfg = CLM_BATCH_TILE_64;
+  // candidate tiles: those that are distinct for this type (128 x 128 of 4 waves
+  // exists for real FP32 only, 128 x 64 for all but complex FP64)
+  auto distinct = [&](int cfg) {
+    if (cfg == CLM_BATCH_TILE_128W16) return true;
+    if (cfg == CLM_BATCH_TILE_128) return single && !cplx;
+    if (cfg == CLM_BATCH_TILE_128x64) return single || !cplx;
+    return true;
+  };
   for (int b = 0; b < clm::kBuckets && 32 * b < nmax; ++b) {
     const int n = std::min(32 * b + 16, nmax);
     // enough matrices to keep every workgroup slot busy twice over
     const int count = 1024;
     float best = 0.f;
-    for (int cfg = CLM_BATCH_TILE_32; cfg <= max_cfg; ++cfg) {
+    for (int cfg = CLM_BATCH_TILE_32; cfg < CLM_BATCH_TILES; ++cfg) {
+      if (!distinct(cfg)) continue;
       float ms = 0.f;
       clm_status_t s;
       switch (dtype) {
@@ -1444,7 +1470,7 @@ clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dt
       default: s = clm::time_config<clm::zd>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
       }
       if (s != CLM_STATUS_SUCCESS) return s;
-      if (ms_out) ms_out[b * 4 + cfg] = ms;
+      if (ms_out) ms_out[b * CLM_BATCH_TILES + cfg] = ms;
       if (cfg == CLM_BATCH_TILE_32 || ms < best) {
         best = ms;
         chosen[b] = cfg;

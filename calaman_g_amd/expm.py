@@ -431,10 +431,11 @@ def batched_autotune(dtype: torch.dtype, nmax: int = 256,
                      stream: Optional[torch.cuda.Stream] = None) -> tuple:
     """Times every workgroup tile per order bucket up to nmax; returns (table, ms[bucket][tile])."""
     cfg = (ctypes.c_int * _lib.BATCH_BUCKETS)()
-    ms = (ctypes.c_float * (4 * _lib.BATCH_BUCKETS))()
+    nt = _lib.BATCH_TILES
+    ms = (ctypes.c_float * (nt * _lib.BATCH_BUCKETS))()
     check(lib().clm_expm_batched_autotune(None, ctypes.c_void_p(_stream_ptr(stream)),
                                           _DTYPE_CODE[dtype], int(nmax), cfg, ms), "autotune")
-    return list(cfg), [list(ms[4 * b:4 * b + 4]) for b in range(_lib.BATCH_BUCKETS)]
+    return list(cfg), [list(ms[nt * b:nt * b + nt]) for b in range(_lib.BATCH_BUCKETS)]
 
 
 def expm_batched(mats, stream: Optional[torch.cuda.Stream] = None):

@@ -223,7 +223,9 @@ typedef struct clm_batch_plan clm_batch_plan;
 #define CLM_BATCH_TILE_32 0     /* 32 x 32 */
 #define CLM_BATCH_TILE_64 1     /* 64 x 64 */
 #define CLM_BATCH_TILE_128x64 2 /* 128 x 64 (complex FP64: 64 x 64) */
-#define CLM_BATCH_TILE_128 3    /* 128 x 128 (real FP32 only; others: their largest) */
+#define CLM_BATCH_TILE_128 3    /* 128 x 128 of 4 waves (real FP32 only; others: their largest) */
+#define CLM_BATCH_TILE_128W16 4 /* 128 x 128 of 16 waves (every type) */
+#define CLM_BATCH_TILES 5
 
 /* Validates the entries (CLM_STATUS_INVALID_VALUE before any device call),
  * orders them by cost, takes each one's tile from the tuning table, and copies
@@ -244,7 +246,8 @@ clm_status_t clm_expm_batched_graph(clm_ctx *ctx, clm_stream_t stream, clm_batch
 /* The tuning table (CLM_BATCH_BUCKETS tile ids per element type) and the tuner
  * that fills it: for each bucket up to nmax it times a batch of 1024 matrices
  * of the bucket's middle order under every tile and keeps the fastest. ms_out
- * (optional, 4 * CLM_BATCH_BUCKETS floats) receives ms[bucket * 4 + tile].
+ * (optional, CLM_BATCH_TILES * CLM_BATCH_BUCKETS floats) receives
+ * ms[bucket * CLM_BATCH_TILES + tile] (0 for a tile that is not distinct for the type).
  * Plans created afterwards use the new table. */
 clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg);
 clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg);

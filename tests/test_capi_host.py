@@ -200,11 +200,11 @@ def test_batched_tuning_table_round_trip():
     cfg = (ctypes.c_int * _lib.BATCH_BUCKETS)()
     assert L.clm_expm_batched_get_tuning(ST.DTYPE_C64, cfg) == 0
     saved = list(cfg)
-    assert all(0 <= c <= 3 for c in saved)
+    assert all(0 <= c < _lib.BATCH_TILES for c in saved)
     new = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([1] * _lib.BATCH_BUCKETS))
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, new) == 0
     assert L.clm_expm_batched_get_tuning(ST.DTYPE_C64, cfg) == 0 and list(cfg) == [1] * 8
-    bad = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([4] * _lib.BATCH_BUCKETS))
+    bad = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([_lib.BATCH_TILES] * _lib.BATCH_BUCKETS))
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, bad) == ST.INVALID_VALUE
     assert L.clm_expm_batched_set_tuning(7, new) == ST.INVALID_VALUE
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, (ctypes.c_int * 8)(*saved)) == 0

@@ -99,7 +99,7 @@ def test_batched_mixed_types_bitwise_equals_single(dev):
     _check_batch(entries, inputs, BatchPlan.decode_results(res), dev)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 def test_batched_every_tile_gives_the_same_bits(dev, cfg):
     """the tile shape changes which lanes compute an element, not its arithmetic"""
     rng = np.random.default_rng(10 + cfg)
@@ -195,7 +195,8 @@ def test_batched_autotune_small(dev):
     saved = cg.batched_tuning(torch.float32)
     try:
         table, ms = cg.batched_autotune(torch.float32, nmax=64)
-        assert all(0 <= c <= 3 for c in table)
+        assert all(0 <= c < _lib.BATCH_TILES for c in table)
+        # real FP32 has every tile distinct: all were timed
         assert all(t > 0 for t in ms[0]) and all(t > 0 for t in ms[1])
         assert cg.batched_tuning(torch.float32) == table
         for b in (0, 1):

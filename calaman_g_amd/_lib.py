@@ -40,7 +40,8 @@ class ExpmPlan(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
                 ("method", ctypes.c_int), ("ps_block", ctypes.c_int), ("dtype", ctypes.c_int),
                 ("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int),
-                ("norm1", ctypes.c_double)]
+                ("norm1", ctypes.c_double), ("fp8_gemms", ctypes.c_int),
+                ("fp8_min_degree", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -52,6 +53,11 @@ SPLIT_BF16X6 = 6
 
 class SplitOpts(ctypes.Structure):
     _fields_ = [("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int)]
+
+
+class Fp8TailOpts(ctypes.Structure):
+    _fields_ = [("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int),
+                ("fp8_min_degree", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -127,6 +133,11 @@ _SIGNATURES = {
     "clm_expm_split_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_split_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
                               ctypes.POINTER(SplitOpts), ctypes.POINTER(ExpmPlan)]),
+    "clm_expm_fp8tail_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
+    "clm_expm_fp8tail_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
+                                ctypes.POINTER(Fp8TailOpts), ctypes.POINTER(ExpmPlan)]),
+    "clm_gemm_fp8_buffer_size_d": (_i, [_i, _i, _i, ctypes.POINTER(_sz)]),
+    "clm_gemm_fp8_d": (_i, [_vp, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i, _vp, _sz]),
     "clm_gemm_split_d": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
     "clm_expm_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_buffer_size_z": (_i, [_i, ctypes.POINTER(_sz)]),

@@ -129,7 +129,6 @@ constexpr size_t cplx_lds() {
   using TL = CTile<C, BM, BN, NT, kBK>;
   return 2 * TL::STAGE * sizeof(typename CxReal<C>::type);
 }
-constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 
 // K slice of the 32 x 32 tile: 16 for FP64 (2 doubles per 16-byte vector),
 // 32 for FP32 (4 floats), so that each of the 256 lanes stages one vector

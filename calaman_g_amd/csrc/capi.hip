@@ -428,6 +428,131 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   return CLM_STATUS_SUCCESS;
 }
 
+// ---- the FP8-tail variant's e4m3 workspace (after the split planes) -------------------
+//   [R  | n*n bytes | tiles^2 ints]  row-major e4m3 plane of the current left operand
+//   [CA | n*n bytes | tiles^2 ints]  column-major plane of A/2^s
+//   [CX | n*n bytes | tiles^2 ints]  column-major plane of X = (A/2^s)^b
+// with one power-of-two exponent per 128 x 128 tile (tiles = ceil(n / 128)).
+struct Fp8Plane {
+  unsigned char *q = nullptr;
+  int *e = nullptr;
+};
+struct Fp8Ws {
+  Fp8Plane R, CA, CX;
+  size_t carve(void *base, int n) {
+    Layout l(base);
+    const size_t tiles = (static_cast<size_t>(n) + 127) / 128;
+    for (Fp8Plane *f : {&R, &CA, &CX}) {
+      f->q = l.take<unsigned char>(static_cast<size_t>(n) * n);
+      f->e = l.take<int>(tiles * tiles);
+    }
+    return l.total();
+  }
+};
+
+size_t expm_fp8tail_buffer_size(int n) {
+  Fp8Ws f8;
+  return expm_split_buffer_size(n) + f8.carve(nullptr, n);
+}
+
+constexpr int kFp8MinDegree = 4;
+
+// Paterson-Stockmeyer of the FP8-tail variant. A product whose result holds
+// only Taylor terms of degree >= min_degree -- the bank's A^k = A^(k-1) A for
+// k >= min_degree, and the Horner product R_(j+1) X (X = A^b), whose lowest term
+// has degree (j+1) b -- runs on FP8 MFMA from e4m3 operands with one scale per
+// 128 x 128 tile; every other product on the split planes. Operand planes are
+// made once per operand and format, when a product first needs them.
+clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int degree,
+                                 const double *A, long lda, double *P, long ldp, double *bank,
+                                 long stride, double *scratch, long ld_scratch, int s_requested,
+                                 int split, int min_degree, const SplitWs &sp, const Fp8Ws &f8,
+                                 int *fp8_products) {
+  const clm_ps_plan p = ps_plan(degree, s_requested);
+  const int b = p.s;
+  int count = 0;
+  auto plain = [&](double *C, long ldc) {
+    GemmArgs<double> g;
+    g.M = g.N = g.K = n;
+    g.C = C;
+    g.ldc = ldc;
+    g.alpha = 1.0;
+    return g;
+  };
+  bool ca_split = false, ca_fp8 = false;
+  for (int k = 2; k <= p.num_powers + 1; ++k) {
+    const double *L = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+    const long ldl = (k == 2) ? lda : n;
+    const GemmArgs<double> g = plain(bank + static_cast<long>(k - 2) * stride, n);
+    if (k >= min_degree) {
+      if (k == 2) {
+        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, f8.R.q, f8.CA.e, f8.R.e));
+      } else {
+        if (!ca_fp8)
+          CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, nullptr, f8.CA.e, nullptr));
+        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, L, ldl, nullptr, f8.R.q, nullptr, f8.R.e));
+      }
+      ca_fp8 = true;
+      CLM_TRY_HIP(launch_gemm_fp8(st, g, f8.R.q, f8.CA.q, f8.R.e, f8.CA.e));
+      ++count;
+    } else {
+      if (k == 2) {
+        CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp.pstride));
+      } else {
+        if (!ca_split) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp.pstride));
+        CLM_TRY(split_planes(st, split, n, L, ldl, nullptr, &sp.PR, sp.pstride));
+      }
+      ca_split = true;
+      CLM_TRY(gemm_planes(st, g, split, sp.PR, sp.PCA, sp.pstride));
+    }
+  }
+  // X = A^b for the Horner steps (A itself when the block is 1 wide)
+  const bool x_is_a = !(p.r >= 1 && b >= 2);
+  const double *X = x_is_a ? A : bank + static_cast<long>(b - 2) * stride;
+  const long ldx = x_is_a ? lda : n;
+  bool cx_split = false, cx_fp8 = false;
+  double *buf[2] = {P, scratch};
+  const long ld[2] = {ldp, ld_scratch};
+  int cur = (p.r % 2 == 0) ? 0 : 1;
+  auto terms = [&](int j) {
+    const int remaining = degree - j * p.s + 1;
+    return remaining < p.s ? remaining : p.s;
+  };
+  CLM_TRY_HIP(launch_build_block(st, n, buf[cur], ld[cur], coef + static_cast<long>(p.r) * p.s,
+                                 terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
+  for (int j = p.r - 1; j >= 0; --j) {
+    const int next = 1 - cur;
+    GemmArgs<double> g = plain(buf[next], ld[next]);
+    g.coef = coef + static_cast<long>(j) * p.s;
+    g.nterms = terms(j);
+    g.X1 = A;
+    g.ldx1 = lda;
+    g.bank = bank;
+    g.bank_stride = stride;
+    g.ldbank = n;
+    if ((j + 1) * b >= min_degree) {
+      bool &have = x_is_a ? ca_fp8 : cx_fp8;
+      const Fp8Plane &right = x_is_a ? f8.CA : f8.CX;
+      if (!have)
+        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, X, ldx, right.q, nullptr, right.e, nullptr));
+      have = true;
+      CLM_TRY_HIP(launch_quantize_fp8(st, n, n, buf[cur], ld[cur], nullptr, f8.R.q, nullptr, f8.R.e));
+      CLM_TRY_HIP(launch_gemm_fp8(st, g, f8.R.q, right.q, f8.R.e, right.e));
+      ++count;
+    } else {
+      bool &have = x_is_a ? ca_split : cx_split;
+      const PlaneSet &right = x_is_a ? sp.PCA : sp.PCX;
+      if (!have) CLM_TRY(split_planes(st, split, n, X, ldx, &right, nullptr, sp.pstride));
+      have = true;
+      CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp.pstride));
+      CLM_TRY(gemm_planes(st, g, split, sp.PR, right, sp.pstride));
+    }
+    cur = next;
+  }
+  if (fp8_products) *fp8_products = count;
+  return CLM_STATUS_SUCCESS;
+}
+
 template <typename T>
 clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, real_t<T> *colsum,
                         real_t<T> *norm) {
@@ -439,15 +564,22 @@ clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, real_t<T> *
   return CLM_STATUS_SUCCESS;
 }
 
+// the precision variants of the FP64 driver
+enum Variant { kPlain = 0, kSplit = 1, kFp8Tail = 2 };
+
 template <typename T>
 clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda, T *dE, int lde,
                        void *work, size_t bytes, int *dInfo, clm_expm_plan *plan,
-                       const clm_split_opts *split = nullptr, bool use_split = false) {
+                       const clm_split_opts *split = nullptr, int variant = kPlain,
+                       int fp8_min_degree = kFp8MinDegree) {
   if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
       dE == nullptr)
     return CLM_STATUS_INVALID_VALUE;
-  if (bytes < (use_split ? expm_split_buffer_size(n) : expm_buffer_size<T>(n)))
-    return CLM_STATUS_ALLOC_FAILED;
+  const bool use_split = variant != kPlain;
+  const size_t need = variant == kFp8Tail ? expm_fp8tail_buffer_size(n)
+                      : use_split         ? expm_split_buffer_size(n)
+                                          : expm_buffer_size<T>(n);
+  if (bytes < need) return CLM_STATUS_ALLOC_FAILED;
   clm_ctx *c = nullptr;
   CLM_TRY(resolve_ctx(ctx, &c));
   const T *coef = inv_factorials<T>(c->device);
@@ -456,7 +588,11 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   ExpmWs<T> ws;
   const size_t fp64_bytes = ws.carve(work, n);
   SplitWs sp;
-  if (use_split) sp.carve(static_cast<char *>(work) + fp64_bytes, n);
+  Fp8Ws f8;
+  if (use_split) {
+    const size_t split_bytes = sp.carve(static_cast<char *>(work) + fp64_bytes, n);
+    if (variant == kFp8Tail) f8.carve(static_cast<char *>(work) + fp64_bytes + split_bytes, n);
+  }
 
   real_t<T> norm = 0;
   CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
@@ -465,6 +601,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   // the degree ladder at the element type's unit roundoff (2^-53 or 2^-24)
   clm_expm_plan p = use_split ? split_plan(norm, split) : taylor_plan(norm, is_single<T>());
   p.dtype = dtype_code<T>();
+  if (variant == kFp8Tail) p.fp8_min_degree = fp8_min_degree;
   if (plan) *plan = p;
 
   const T *src = dA;
@@ -483,7 +620,11 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   const long ld_other = (p.s % 2 == 0) ? n : lde;
 
   if constexpr (std::is_same_v<T, double>) {
-    if (use_split) {
+    if (variant == kFp8Tail) {
+      CLM_TRY(ps_evaluate_fp8tail(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+                                  static_cast<long>(ws.stride), other, ld_other, p.ps_block,
+                                  p.ps_split, fp8_min_degree, sp, f8, &p.fp8_gemms));
+    } else if (use_split) {
       CLM_TRY(ps_evaluate_split(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
                                 static_cast<long>(ws.stride), other, ld_other, p.ps_block,
                                 p.ps_split, sp));
@@ -495,6 +636,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
     CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
                            static_cast<long>(ws.stride), other, ld_other, p.ps_block));
   }
+  if (plan) plan->fp8_gemms = p.fp8_gemms;
 
   T *cur = target, *nxt = other;
   long ldc = ld_target, ldn = ld_other;
@@ -616,6 +758,55 @@ clm_status_t gemm_impl(clm_ctx *ctx, int m, int n, int k, T alpha, const T *dA, 
     g.ldcin = ldc;
   }
   CLM_TRY_HIP(launch_prec(c->stream, g, prec));
+  return CLM_STATUS_SUCCESS;
+}
+
+// the FP8 product's workspace: A's row-major e4m3 plane and tile exponents, then B's
+// column-major plane and exponents
+struct Fp8GemmWs {
+  unsigned char *pa = nullptr, *pb = nullptr;
+  int *ea = nullptr, *eb = nullptr;
+  size_t carve(void *base, int m, int n, int k) {
+    Layout l(base);
+    const size_t tm = (static_cast<size_t>(m) + 127) / 128, tn = (static_cast<size_t>(n) + 127) / 128,
+                 tk = (static_cast<size_t>(k) + 127) / 128;
+    pa = l.take<unsigned char>(static_cast<size_t>(m) * k);
+    ea = l.take<int>(tm * tk);
+    pb = l.take<unsigned char>(static_cast<size_t>(k) * n);
+    eb = l.take<int>(tk * tn);
+    return l.total();
+  }
+};
+
+clm_status_t gemm_fp8_impl(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA,
+                           int lda, const double *dB, int ldb, double beta, double *dC, int ldc,
+                           void *work, size_t bytes) {
+  if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) ||
+      ldc < (m > 1 ? m : 1))
+    return CLM_STATUS_INVALID_VALUE;
+  if (m == 0 || n == 0) return CLM_STATUS_SUCCESS;
+  if (dC == nullptr || (k > 0 && (dA == nullptr || dB == nullptr))) return CLM_STATUS_INVALID_VALUE;
+  Fp8GemmWs ws;
+  if (bytes < ws.carve(nullptr, m, n, k) || (k > 0 && work == nullptr))
+    return CLM_STATUS_ALLOC_FAILED;
+  ws.carve(work, m, n, k);
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  CLM_TRY_HIP(launch_quantize_fp8(c->stream, m, k, dA, lda, nullptr, ws.pa, nullptr, ws.ea));
+  CLM_TRY_HIP(launch_quantize_fp8(c->stream, k, n, dB, ldb, ws.pb, nullptr, ws.eb, nullptr));
+  GemmArgs<double> g;
+  g.M = m;
+  g.N = n;
+  g.K = k;
+  g.C = dC;
+  g.ldc = ldc;
+  g.alpha = alpha;
+  g.beta = beta;
+  if (beta != 0.0) {
+    g.Cin = dC;
+    g.ldcin = ldc;
+  }
+  CLM_TRY_HIP(launch_gemm_fp8(c->stream, g, ws.pa, ws.pb, ws.ea, ws.eb));
   return CLM_STATUS_SUCCESS;
 }
 
@@ -772,7 +963,31 @@ clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const do
                opts->split_squarings < -1))
     return CLM_STATUS_INVALID_VALUE;
   return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
-                                lwork_bytes, dInfo, plan, opts, true);
+                                lwork_bytes, dInfo, plan, opts, clm::kSplit);
+}
+
+clm_status_t clm_expm_fp8tail_buffer_size_d(int n, size_t *bytes) {
+  if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_fp8tail_buffer_size(n);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_fp8tail_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA,
+                                int lda, double *dE, int lde, void *dWork, size_t lwork_bytes,
+                                int *dInfo, const clm_fp8tail_opts *opts, clm_expm_plan *plan) {
+  clm_split_opts split{0, -1};
+  int min_degree = clm::kFp8MinDegree;
+  if (opts) {
+    if ((opts->ps_split != 0 && opts->ps_split != CLM_SPLIT_BF16X3 &&
+         opts->ps_split != CLM_SPLIT_BF16X6 && opts->ps_split != CLM_SPLIT_FP16X3) ||
+        opts->split_squarings < -1 || opts->fp8_min_degree < 0 || opts->fp8_min_degree == 1)
+      return CLM_STATUS_INVALID_VALUE;
+    split.ps_split = opts->ps_split;
+    split.split_squarings = opts->split_squarings;
+    if (opts->fp8_min_degree > 0) min_degree = opts->fp8_min_degree;
+  }
+  return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
+                                lwork_bytes, dInfo, plan, &split, clm::kFp8Tail, min_degree);
 }
 
 double clm_taylor_theta_d(int m) {
@@ -893,6 +1108,20 @@ clm_status_t clm_horner_z(clm_ctx *ctx, int n, const clm_double_complex *dCoeffs
 clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA, int lda,
                         const double *dB, int ldb, double beta, double *dC, int ldc) {
   return clm::gemm_impl<double>(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+}
+
+clm_status_t clm_gemm_fp8_buffer_size_d(int m, int n, int k, size_t *bytes) {
+  if (m < 0 || n < 0 || k < 0 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
+  clm::Fp8GemmWs ws;
+  *bytes = ws.carve(nullptr, m, n, k);
+  return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_gemm_fp8_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA,
+                            int lda, const double *dB, int ldb, double beta, double *dC, int ldc,
+                            void *dWork, size_t lwork_bytes) {
+  return clm::gemm_fp8_impl(ctx, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, dWork,
+                            lwork_bytes);
 }
 
 clm_status_t clm_gemm_split_d(clm_ctx *ctx, int split, int m, int n, int k, double alpha,

@@ -182,6 +182,18 @@ hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const d
 hipError_t launch_split_planes_half(hipStream_t stream, int rows, int cols, const double *X,
                                     long ldx, unsigned short *pc, unsigned short *pr, long pstride,
                                     const int *row_e, const int *col_e);
+// e4m3 planes with one power-of-two exponent per 128 x 128 tile: tile (ti, tj)
+// of X is scaled by 2^e, e = the tile's entry [ti + tj * ceil(rows / 128)] of
+// ec / er, so its largest magnitude lands in (224, 448]; pc column-major
+// (ld rows) with exponents ec, pr row-major (ld cols) with exponents er; any
+// of the four may be null
+hipError_t launch_quantize_fp8(hipStream_t stream, int rows, int cols, const double *X, long ldx,
+                               unsigned char *pc, unsigned char *pr, int *ec, int *er);
+// the product on FP8 MFMA from e4m3 planes: PA row-major planes of A (M x K),
+// PB column-major planes of B (K x N) with their tile exponents; FP32 sums per
+// 128-deep k-tile, FP64 epilogue; g.A / g.B are not read
+hipError_t launch_gemm_fp8(hipStream_t stream, const GemmArgs<double> &g, const unsigned char *PA,
+                           const unsigned char *PB, const int *eA, const int *eB);
 hipError_t launch_gemm_z(hipStream_t stream, const GemmArgs<zd> &g);
 template <typename T>
 hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);

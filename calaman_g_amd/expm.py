@@ -215,6 +215,30 @@ def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1) 
     return lib().clm_expm_split_plan_d(float(norm1), ctypes.byref(opts)).as_dict()
 
 
+def expm_fp8tail_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
+                    dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None, ps_split: int = 0,
+                    split_squarings: int = -1, fp8_min_degree: int = 0,
+                    stream: Optional[torch.cuda.Stream] = None,
+                    ctx: Optional[Context] = None) -> tuple:
+    """clm_expm_fp8tail_d: the split variant with the products that carry only Taylor
+    terms of degree >= fp8_min_degree (0 = 4) on FP8; returns (status, plan dict)."""
+    plan = ExpmPlan()
+    opts = _lib.Fp8TailOpts(int(ps_split), int(split_squarings), int(fp8_min_degree))
+    st = lib().clm_expm_fp8tail_d(ctx.handle if ctx else None,
+                                  ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA),
+                                  int(lda), _ptr(dE), int(lde), _ptr(dWork),
+                                  dWork.numel() * dWork.element_size(), _ptr(dInfo),
+                                  ctypes.byref(opts), ctypes.byref(plan))
+    return int(st), plan.as_dict()
+
+
+def expm_fp8tail_buffer_size(n: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(lib().clm_expm_fp8tail_buffer_size_d(int(n), ctypes.byref(out)),
+          "expm_fp8tail_buffer_size")
+    return int(out.value)
+
+
 def matrix_norm1_cm(n: int, dA: torch.Tensor, lda: int, dColsum: torch.Tensor,
                     stream: Optional[torch.cuda.Stream] = None) -> float:
     out = ctypes.c_float(0.0) if _is_single(dA.dtype) else ctypes.c_double(0.0)
@@ -269,6 +293,24 @@ def gemm_cm(ctx: Context, m: int, n: int, k: int, alpha, dA: torch.Tensor, lda: 
         a, b = float(alpha), float(beta)
     return int(getattr(lib(), f"clm_gemm_{sfx}")(ctx.handle, m, n, k, a, _ptr(dA), lda, _ptr(dB),
                                                   ldb, b, _ptr(dC), ldc))
+
+
+def gemm_fp8_buffer_size(m: int, n: int, k: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(lib().clm_gemm_fp8_buffer_size_d(int(m), int(n), int(k), ctypes.byref(out)),
+          "gemm_fp8_buffer_size")
+    return int(out.value)
+
+
+def gemm_fp8_cm(ctx: Optional[Context], m: int, n: int, k: int, alpha: float, dA: torch.Tensor,
+                lda: int, dB: torch.Tensor, ldb: int, beta: float, dC: torch.Tensor, ldc: int,
+                dWork: torch.Tensor) -> int:
+    """clm_gemm_fp8_d on column-major float64 buffers: C = alpha * Q(A) Q(B) + beta * C
+    with Q the e4m3 rounding at one power-of-two scale per 128 x 128 tile."""
+    return int(lib().clm_gemm_fp8_d(ctx.handle if ctx else None, int(m), int(n), int(k),
+                                    float(alpha), _ptr(dA), int(lda), _ptr(dB), int(ldb),
+                                    float(beta), _ptr(dC), int(ldc), _ptr(dWork),
+                                    dWork.numel() * dWork.element_size()))
 
 
 # ---- batched expm -----------------------------------------------------------------------------
@@ -484,6 +526,25 @@ def expm_split(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
     return (e, plan) if return_plan else e
 
 
+def expm_fp8tail(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
+                 fp8_min_degree: int = 0, stream: Optional[torch.cuda.Stream] = None,
+                 return_plan: bool = False):
+    """exp(a) for a float64 device matrix with the high-degree Taylor products on FP8
+    (e4m3, one scale per 128 x 128 tile); the error is reported, not bounded."""
+    _require_device(a, "a")
+    if a.dtype != _F64:
+        raise TypeError("expm_fp8tail takes float64")
+    n = a.shape[-1]
+    acm = to_cm(a)
+    ecm = torch.empty_like(acm)
+    work = torch.empty(expm_fp8tail_buffer_size(n), dtype=torch.uint8, device=a.device)
+    st, plan = expm_fp8tail_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings,
+                               fp8_min_degree, stream)
+    check(st, "expm_fp8tail")
+    e = ecm.t()
+    return (e, plan) if return_plan else e
+
+
 def matrix_norm1(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> float:
     _require_device(a, "a")
     n = a.shape[0]
@@ -553,7 +614,9 @@ def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torc
 
 
 __all__ = [
-    "CalamanError", "Context", "expm", "expm_cm", "expm_split", "expm_split_cm", "expm_split_plan", "expm_split_buffer_size", "expm_plan", "expm_buffer_size", "taylor_theta",
+    "CalamanError", "Context", "expm", "expm_cm", "expm_split", "expm_split_cm", "expm_split_plan", "expm_split_buffer_size",
+    "expm_fp8tail", "expm_fp8tail_cm", "expm_fp8tail_buffer_size", "gemm_fp8_cm",
+    "gemm_fp8_buffer_size", "expm_plan", "expm_buffer_size", "taylor_theta",
     "pade_theta", "pade_plan", "matrix_norm1", "matrix_norm1_cm", "lange", "lange_cm", "lascl",
     "lascl_cm", "paterson_stockmeyer", "paterson_stockmeyer_cm", "horner", "horner_cm", "gemm",
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",

@@ -103,6 +103,8 @@ typedef struct clm_expm_plan {
   int ps_split;  /* polynomial products: 0 = FP64 MFMA, 3 / 6 = BF16 split (clm_expm_split_d) */
   int split_squarings; /* how many of the last squarings ran BF16-split (6 products) */
   double norm1;  /* the 1-norm the plan was chosen from */
+  int fp8_gemms;      /* polynomial products that ran on FP8 (clm_expm_fp8tail_d) */
+  int fp8_min_degree; /* the lowest Taylor degree an FP8 product may carry (0 otherwise) */
 } clm_expm_plan;
 
 /* The plan clm_expm_d/z execute for a given 1-norm (the role of expm_plan<T>,
@@ -187,6 +189,28 @@ clm_status_t clm_expm_split_buffer_size_d(int n, size_t *bytes);
 clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
                               double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
                               const clm_split_opts *opts, clm_expm_plan *plan);
+
+/* ---- FP8 tail: the split variant with its high-degree products on FP8 -----------------
+ * No reference counterpart. The plan, the scaling and the squarings are those of
+ * clm_expm_split_d; in the Paterson-Stockmeyer polynomial every product whose
+ * result holds only Taylor terms of degree >= fp8_min_degree (the power A^k for
+ * k >= that degree, the Horner product R_(j+1) X whose lowest term has degree
+ * (j+1) b) runs on FP8 MFMA (OCP e4m3, one power-of-two scale per 128 x 128
+ * tile, FP32 sums per 128-deep k-tile, FP64 epilogue). An e4m3 operand keeps 4
+ * significant bits (relative rounding 2^-4), so the result's error is set by the
+ * size of the degree-fp8_min_degree terms at A / 2^s and the squarings that
+ * follow: it is reported, not bounded (plan->fp8_gemms counts the FP8 products). */
+typedef struct clm_fp8tail_opts {
+  int ps_split;        /* products below fp8_min_degree: as clm_split_opts.ps_split */
+  int split_squarings; /* as clm_split_opts.split_squarings */
+  int fp8_min_degree;  /* 0 = 4; otherwise >= 2 */
+} clm_fp8tail_opts;
+/* workspace: that of clm_expm_split_buffer_size_d plus three n x n e4m3 planes and
+ * their tile exponents */
+clm_status_t clm_expm_fp8tail_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_fp8tail_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA,
+                                int lda, double *dE, int lde, void *dWork, size_t lwork_bytes,
+                                int *dInfo, const clm_fp8tail_opts *opts, clm_expm_plan *plan);
 
 /* ---- batched expm: many orders and types in one persistent launch ------------------
  * No reference counterpart (the reference runs one expm per call). Every matrix
@@ -408,6 +432,14 @@ clm_status_t clm_gemm_d(clm_ctx *ctx, int m, int n, int k, double alpha, const d
 clm_status_t clm_gemm_split_d(clm_ctx *ctx, int split, int m, int n, int k, double alpha,
                               const double *dA, int lda, const double *dB, int ldb, double beta,
                               double *dC, int ldc);
+/* the product on FP8 MFMA (no reference counterpart): A and B are quantized to OCP
+ * e4m3 with one power-of-two scale per 128 x 128 tile (the largest magnitude of a
+ * tile lands in (224, 448]), multiplied with FP32 sums per 128-deep k-tile, and
+ * alpha * AB + beta * C is formed in FP64. The workspace holds the e4m3 copies. */
+clm_status_t clm_gemm_fp8_buffer_size_d(int m, int n, int k, size_t *bytes);
+clm_status_t clm_gemm_fp8_d(clm_ctx *ctx, int m, int n, int k, double alpha, const double *dA,
+                            int lda, const double *dB, int ldb, double beta, double *dC, int ldc,
+                            void *dWork, size_t lwork_bytes);
 clm_status_t clm_gemm_z(clm_ctx *ctx, int m, int n, int k, clm_double_complex alpha,
                         const clm_double_complex *dA, int lda, const clm_double_complex *dB,
                         int ldb, clm_double_complex beta, clm_double_complex *dC, int ldc);

@@ -369,3 +369,90 @@ def taylor_plan(norm1: float, single: bool = False):
 def inv_factorials():
     p = lib().orc_inv_factorials()
     return [p[k] for k in range(33)]
+
+
+# ---- FP8 (OCP e4m3) tile quantization and the FP8-tail evaluation ---------------------
+# No reference counterpart: the precision variant's own arithmetic, restated in
+# numpy. e4m3 (OCP 8-bit floating point, E4M3 encoding): 1 sign, 4 exponent bits
+# (bias 7), 3 fraction bits; normal magnitudes 2^-6 .. 448, subnormal quantum
+# 2^-9, no infinities. Pinned against torch.float8_e4m3fn conversion and the
+# format's code table in tests/test_oracle_fp8.py.
+E4M3_MAX = 448.0
+FP8_TILE = 128
+
+
+def e4m3_round(x):
+    """Round-to-nearest-even onto e4m3 of float64 values with |x| <= 448, after a
+    first rounding to float32 (the device converts FP64 -> FP32 -> e4m3)."""
+    x = np.asarray(x, dtype=np.float64).astype(np.float32).astype(np.float64)
+    if np.any(np.abs(x) > E4M3_MAX):
+        raise ValueError("e4m3_round: magnitude beyond 448")
+    _, ex = np.frexp(x)                        # |x| = f 2^ex, f in [0.5, 1)
+    e = np.maximum(ex - 1, -6)                 # binade exponent, subnormals at 2^-6
+    q = np.ldexp(1.0, e - 3)                   # quantum: 3 fraction bits
+    return np.round(x / q) * q                 # exact division; np.round is half-to-even
+
+
+def fp8_tile_exponent(mx: float) -> int:
+    """The largest e with mx * 2^e <= 448 (0 for a zero tile)."""
+    if not mx > 0.0 or not np.isfinite(mx):
+        return 0
+    f, k = np.frexp(mx)                        # mx = f 2^k; 448 = 0.875 2^9
+    return int(9 - k) if f <= 0.875 else int(8 - k)
+
+
+def fp8_quantize(X, tile: int = FP8_TILE):
+    """X with every tile x tile block rounded to e4m3 at its own power-of-two
+    scale and scaled back: Q(X) = 2^-e e4m3(X 2^e) per tile. Returns (Q(X), exps)
+    with exps[ti, tj] the tile exponents."""
+    X = np.asarray(X, dtype=np.float64)
+    rows, cols = X.shape
+    tr, tc = -(-rows // tile), -(-cols // tile)
+    out = np.empty_like(X)
+    exps = np.zeros((tr, tc), dtype=np.int64)
+    for ti in range(tr):
+        for tj in range(tc):
+            blk = X[ti * tile:(ti + 1) * tile, tj * tile:(tj + 1) * tile]
+            e = fp8_tile_exponent(float(np.abs(blk).max()) if blk.size else 0.0)
+            exps[ti, tj] = e
+            out[ti * tile:(ti + 1) * tile, tj * tile:(tj + 1) * tile] = \
+                np.ldexp(e4m3_round(np.ldexp(blk, e)), -e)
+    return out, exps
+
+
+def gemm_fp8(A, B):
+    """Q(A) Q(B) in float64: the product the FP8 MFMA path forms, up to its FP32
+    summation."""
+    return fp8_quantize(A)[0] @ fp8_quantize(B)[0]
+
+
+def expm_fp8tail_model(A, degree: int, s: int, min_degree: int = 4):
+    """The FP8-tail evaluation at a given plan (Taylor degree, squarings s) in
+    float64: Paterson-Stockmeyer over A / 2^s with the block size of ps_plan;
+    every product whose result holds only Taylor terms of degree >= min_degree
+    (the power A^k for k >= min_degree, the Horner product R_(j+1) X with
+    (j+1) b >= min_degree) takes e4m3-quantized operands (gemm_fp8), every
+    other product and the squarings are exact float64 products."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    As = np.ldexp(A, -s)
+    c = inv_factorials()
+    p = ps_plan(degree, 0)
+    b, r = p["s"], p["r"]
+    max_power = b if r >= 1 else degree
+    pw = {0: np.eye(n), 1: As}
+    for k in range(2, max_power + 1):
+        pw[k] = gemm_fp8(pw[k - 1], As) if k >= min_degree else pw[k - 1] @ As
+
+    def block(j):
+        terms = min(degree - j * b + 1, b)
+        return sum(c[j * b + t] * pw[t] for t in range(terms))
+
+    X = pw[b] if (r >= 1 and b >= 2) else As
+    R = block(r)
+    for j in range(r - 1, -1, -1):
+        prod = gemm_fp8(R, X) if (j + 1) * b >= min_degree else R @ X
+        R = prod + block(j)
+    for _ in range(s):
+        R = R @ R
+    return R

@@ -208,3 +208,32 @@ def test_batched_tuning_table_round_trip():
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, bad) == ST.INVALID_VALUE
     assert L.clm_expm_batched_set_tuning(7, new) == ST.INVALID_VALUE
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, (ctypes.c_int * 8)(*saved)) == 0
+
+
+def test_fp8_buffer_sizes_and_argument_checks():
+    """the FP8 workspaces: the split workspace plus three n x n byte planes and their
+    tile exponents; the product's two operand planes and exponents; bad options
+    and sizes rejected before any device call"""
+    out, split = ctypes.c_size_t(), ctypes.c_size_t()
+    for n in (1, 100, 128, 129, 4096):
+        assert L.clm_expm_fp8tail_buffer_size_d(n, ctypes.byref(out)) == 0
+        assert L.clm_expm_split_buffer_size_d(n, ctypes.byref(split)) == 0
+        tiles = -(-n // 128)
+        plane = ((n * n + 255) // 256) * 256 + ((tiles * tiles * 4 + 255) // 256) * 256
+        assert out.value == split.value + 3 * plane
+    assert L.clm_expm_fp8tail_buffer_size_d(0, ctypes.byref(out)) == ST.INVALID_VALUE
+    al = lambda v: ((v + 255) // 256) * 256
+    assert L.clm_gemm_fp8_buffer_size_d(300, 200, 130, ctypes.byref(out)) == 0
+    assert out.value == al(300 * 130) + al(3 * 2 * 4) + al(130 * 200) + al(2 * 2 * 4)
+    assert L.clm_gemm_fp8_buffer_size_d(-1, 2, 2, ctypes.byref(out)) == ST.INVALID_VALUE
+    a = np.ones(64)
+    pa = _p(a.ctypes.data)
+    assert L.clm_gemm_fp8_d(None, 4, 4, 4, 1.0, pa, 3, pa, 4, 0.0, pa, 4, pa, 512) == ST.INVALID_VALUE
+    assert L.clm_gemm_fp8_d(None, 0, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4, None, 0) == ST.SUCCESS
+    assert L.clm_gemm_fp8_d(None, 4, 4, 4, 1.0, pa, 4, pa, 4, 0.0, pa, 4, pa, 8) == ST.ALLOC_FAILED
+    for opts in ((0, -1, 1), (5, -1, 0), (0, -2, 0), (0, -1, -3)):
+        o = _lib.Fp8TailOpts(*opts)
+        assert L.clm_expm_fp8tail_d(None, None, 4, pa, 4, pa, 4, pa, 10 ** 6, None,
+                                    ctypes.byref(o), None) == ST.INVALID_VALUE
+    assert L.clm_expm_fp8tail_d(None, None, 4, pa, 4, pa, 4, pa, 512, None, None,
+                                None) == ST.ALLOC_FAILED

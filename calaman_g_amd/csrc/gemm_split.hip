@@ -599,6 +599,121 @@ __global__ void __launch_bounds__(NW * 64)
       }
 }
 
+// ------------------------------------------------------------------------------------
+// The plane product with k-grouped waves (the 3-product formats). Eight waves
+// share a 128 x 128 tile as two groups of four; each group covers the whole tile
+// as 2 x 2 wave tiles of 64 x 64 and takes one of the two 32-deep k-steps of
+// every 64-deep LDS slice (group 0 the even steps, group 1 the odd). A 64 x 64
+// wave tile reads 64 + 64 rows of each plane per k-step for 16 MFMA tiles,
+// against 64 + 32 rows for 8 tiles at 64 x 32 -- a third less LDS reading per
+// MFMA -- and a CU still runs two waves per SIMD. At the end the groups' FP32
+// sums meet in LDS: group 1 writes its accumulators, group 0 adds them (even-step
+// sum + odd-step sum) and runs the epilogue.
+// ------------------------------------------------------------------------------------
+template <int SPLIT, int FMT>
+__device__ __forceinline__ void planes_kg_compute(const unsigned short *S, int lane, int wm0,
+                                                  int wn0, int ks, f4 (&acc)[4][4]) {
+  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  using PT = PlaneTile<NPL, 2>;
+  using V = typename PlaneMfma<FMT>::vec;
+  const int koff = ks * 32 + 8 * (lane >> 4);
+  V a[NPL][4], b[NPL][4];
+#pragma unroll
+  for (int t = 0; t < NPL; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[t][i] = *reinterpret_cast<const V *>(
+          &S[t * PT::PLANE + (wm0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
+      b[t][i] = *reinterpret_cast<const V *>(
+          &S[(NPL + t) * PT::PLANE + (wn0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
+    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[0][j], acc[i][j]);
+      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[1][j], acc[i][j]);
+      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
+    }
+}
+
+template <int SPLIT, int FMT, bool kFast>
+__global__ void __launch_bounds__(512)
+    dgemm_split_planes_kg(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
+                          const unsigned short *__restrict__ PB, const long pstride,
+                          const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
+                          const int *__restrict__ col_e) {
+  static_assert(SPLIT == 3, "the k-grouped kernel runs the 3-product formats");
+  constexpr int NT = 512, KS = 2;
+  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+  using PT = PlaneTile<NPL, KS>;
+  constexpr int IT = 128 * 4 * KS / NT;
+  static_assert(IT >= 1 && (128 * 4 * KS) % NT == 0, "staging shape");
+  // the reduction buffer: 4 waves x 16 tiles x 4 rows x 64 lanes of FP32
+  static_assert(4 * 16 * 4 * 64 * 4 <= 2 * PT::STAGE * 2, "reduction fits in the stages");
+
+  __shared__ __attribute__((aligned(16))) unsigned short L[2 * PT::STAGE];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int grp = wave >> 2, wpos = wave & 3;
+  int tm, tn;
+  tile_coords_s(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * 128, n0 = tn * 128;
+  const int wm0 = (wpos >> 1) * 64, wn0 = (wpos & 1) * 64;
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  uint4 ra[NPL][IT], rb[NPL][IT];
+  const int KT = (g.K + PT::BK - 1) / PT::BK;
+  if (KT > 0) {
+    planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
+    planes2_sstore<NPL, KS, NT>(L, tid, ra, rb);
+    __syncthreads();
+    for (int kt = 0; kt < KT - 1; ++kt) {
+      const int cur = kt & 1;
+      planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * PT::BK, tid, ra, rb);
+      planes_kg_compute<SPLIT, FMT>(L + cur * PT::STAGE, lane, wm0, wn0, grp, acc);
+      planes2_sstore<NPL, KS, NT>(L + (cur ^ 1) * PT::STAGE, tid, ra, rb);
+      __syncthreads();
+    }
+    planes_kg_compute<SPLIT, FMT>(L + ((KT - 1) & 1) * PT::STAGE, lane, wm0, wn0, grp, acc);
+  }
+
+  __syncthreads();  // every wave is done with the stages
+  float *red = reinterpret_cast<float *>(L);
+  if (grp == 1) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[((wpos * 16 + i * 4 + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+  }
+  __syncthreads();
+  if (grp == 1) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float sum = acc[i][j][r] + red[((wpos * 16 + i * 4 + j) * 4 + r) * 64 + lane];
+        const int row = m0 + wm0 + i * 16 + d_row(lane, r);
+        const int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (kFast || (row < g.M && col < g.N)) {
+          double v = static_cast<double>(sum);
+          if constexpr (FMT == 1) v = ldexp(v, -(half_scale(row_e[row]) + half_scale(col_e[col])));
+          epilogue_store_s(g, row, col, v);
+        }
+      }
+}
+
 inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 template <int SPLIT>
@@ -646,17 +761,21 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
 #define CLM_SPLIT_LAUNCH(S, FMT, KS, F)                                                       \
   hipLaunchKernelGGL((dgemm_split_planes2<S, FMT, 8, KS, F>), grid, block, 0, stream, g, PA, PB, \
                      pstride, tiles_m, tiles_n, row_e, col_e)
+#define CLM_SPLIT_LAUNCH_KG(FMT, F)                                                          \
+  hipLaunchKernelGGL((dgemm_split_planes_kg<3, FMT, F>), grid, block, 0, stream, g, PA, PB,    \
+                     pstride, tiles_m, tiles_n, row_e, col_e)
   if (split == CLM_SPLIT_FP16X3) {
-    if (fast) CLM_SPLIT_LAUNCH(3, 1, 2, true);
-    else CLM_SPLIT_LAUNCH(3, 1, 2, false);
+    if (fast) CLM_SPLIT_LAUNCH_KG(1, true);
+    else CLM_SPLIT_LAUNCH_KG(1, false);
   } else if (split == CLM_SPLIT_BF16X6) {
     if (fast) CLM_SPLIT_LAUNCH(6, 0, 1, true);
     else CLM_SPLIT_LAUNCH(6, 0, 1, false);
   } else {
-    if (fast) CLM_SPLIT_LAUNCH(3, 0, 2, true);
-    else CLM_SPLIT_LAUNCH(3, 0, 2, false);
+    if (fast) CLM_SPLIT_LAUNCH_KG(0, true);
+    else CLM_SPLIT_LAUNCH_KG(0, false);
   }
 #undef CLM_SPLIT_LAUNCH
+#undef CLM_SPLIT_LAUNCH_KG
   return hipGetLastError();
 }
 

@@ -89,20 +89,15 @@ __host__ __device__ inline long stride_of(int n) {
                            sizeof(T));
 }
 
-// per-matrix workspace: As | bank (max_bank_blocks()) | S, each an aligned n x n block
-template <typename T>
-size_t matrix_ws_bytes(int n) {
-  return static_cast<size_t>(2 + max_bank_blocks()) * stride_of<T>(n) * sizeof(T);
-}
-
 // The adjoint's tape, per matrix, after a 256-byte head {m, s, c, status}:
 //   P[1..r_max]      the Horner iterates P_r (= B_r) .. P_1
 //   CK[0..K)         squaring checkpoints R_0, R_c, R_2c, ... (c = ceil(s / K))
 //   U, V             cotangent ping-pong of the backward sweep
 //   W0, W1           squarings recomputed from a checkpoint
 //   Xb               cotangent of the scaled matrix X0 = A / 2^s
+//   T1, T2           conjugate transposes of product operands
 //   Yb[0..nbank)     cotangents of the power bank X0^2 ..
-__host__ __device__ inline int max_horner_steps() {
+__host__ __device__ constexpr int max_horner_steps() {
   int mx = 0;
   for (int i = 0; i < kNumTaylor; ++i) {
     const int r = ps_plan(taylor_degree(i), 0).r;
@@ -110,8 +105,17 @@ __host__ __device__ inline int max_horner_steps() {
   }
   return mx;
 }
+// compile-time constants: the tape and workspace offsets below are computed per
+// element in the adjoint's inner loops
+constexpr int kMaxHornerSteps = max_horner_steps();
+constexpr int kMaxBankBlocks = max_bank_blocks();
+// per-matrix workspace: As | bank (kMaxBankBlocks) | S, each an aligned n x n block
+template <typename T>
+size_t matrix_ws_bytes(int n) {
+  return static_cast<size_t>(2 + kMaxBankBlocks) * stride_of<T>(n) * sizeof(T);
+}
 __host__ __device__ inline int tape_blocks(int ckpt) {
-  return max_horner_steps() + ckpt + 5 + max_bank_blocks();
+  return kMaxHornerSteps + ckpt + 7 + kMaxBankBlocks;
 }
 template <typename T>
 size_t tape_bytes(int n, int ckpt) {
@@ -311,157 +315,11 @@ __device__ void wg_gemm(const GemmArgs<T> &g, char *lds) {
   }
 }
 
-// ---- products with (conjugate-)transposed operands, for the adjoint ------------------
-// C = op(A) op(B) (+ epilogue of g), op = identity or conjugate transpose (plain
-// transpose for real types). Each operand is staged through scalar loads laid
-// out so that consecutive lanes read consecutive addresses of the stored matrix
-// (along k for a transposed A, along n for a transposed B), then written into
-// the same LDS layouts the NN kernels use, so the MFMA compute helpers are
-// shared.
+// complex conjugate for the adjoint (identity for real types)
 __device__ __forceinline__ float conj_if(float x, bool) { return x; }
 __device__ __forceinline__ double conj_if(double x, bool) { return x; }
 __device__ __forceinline__ zc conj_if(zc x, bool c) { return c ? zc{x.re, -x.im} : x; }
 __device__ __forceinline__ zd conj_if(zd x, bool c) { return c ? zd{x.re, -x.im} : x; }
-
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, int NT>
-__device__ __forceinline__ void op_gload(const GemmArgs<T> &g, int m0, int n0, int k0, int tid,
-                                         T (&ra)[BM * BK / NT], T (&rb)[BN * BK / NT]) {
-#pragma unroll
-  for (int it = 0; it < BM * BK / NT; ++it) {
-    const int e = tid + it * NT;
-    const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
-    const int gm = m0 + mm, gk = k0 + kk;
-    T v = ops<T>::zero();
-    if (gm < g.M && gk < g.K)
-      v = conj_if(TA ? g.A[gk + static_cast<long>(gm) * g.lda] : g.A[gm + static_cast<long>(gk) * g.lda], TA);
-    ra[it] = v;
-  }
-#pragma unroll
-  for (int it = 0; it < BN * BK / NT; ++it) {
-    const int e = tid + it * NT;
-    const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
-    const int gn = n0 + nn, gk = k0 + kk;
-    T v = ops<T>::zero();
-    if (gn < g.N && gk < g.K)
-      v = conj_if(TB ? g.B[gn + static_cast<long>(gk) * g.ldb] : g.B[gk + static_cast<long>(gn) * g.ldb], TB);
-    rb[it] = v;
-  }
-}
-
-// register -> LDS in the layouts of RTile / CTile: A at As[k][m] (row BM + 16),
-// B at Bs[n][k] (row BK + one vector); complex operands as a re plane followed by
-// an im plane of the same layout
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, int NT>
-__device__ __forceinline__ void op_sstore(real_t<T> *As, real_t<T> *Bs, int tid,
-                                          const T (&ra)[BM * BK / NT],
-                                          const T (&rb)[BN * BK / NT]) {
-  using R = real_t<T>;
-  constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
-  constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
-#pragma unroll
-  for (int it = 0; it < BM * BK / NT; ++it) {
-    const int e = tid + it * NT;
-    const int mm = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
-    const int sidx = kk * LDA_S + mm;
-    if constexpr (is_cplx<T>()) {
-      As[sidx] = ra[it].re;
-      As[A_PLANE + sidx] = ra[it].im;
-    } else {
-      As[sidx] = ra[it];
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < BN * BK / NT; ++it) {
-    const int e = tid + it * NT;
-    const int nn = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
-    const int sidx = nn * LDB_S + kk;
-    if constexpr (is_cplx<T>()) {
-      Bs[sidx] = rb[it].re;
-      Bs[B_PLANE + sidx] = rb[it].im;
-    } else {
-      Bs[sidx] = rb[it];
-    }
-  }
-}
-
-// C = op(A) op(B) (+ epilogue) over the whole output by the workgroup; same
-// tiles and compute helpers as wg_gemm_real / wg_gemm_cplx
-template <typename T, int BM, int BN, int WM, int WN, int BK, bool TA, bool TB, int NT = 256>
-__device__ void wg_gemm_op(const GemmArgs<T> &g, char *lds) {
-  using R = real_t<T>;
-  constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
-  static_assert((BM * BK) % NT == 0 && (BN * BK) % NT == 0, "staging");
-  constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int LDA_S = BM + 16, LDB_S = BK + Mfma<R>::EV;
-  constexpr int A_PLANE = BK * LDA_S, B_PLANE = BN * LDB_S;
-  constexpr int NPL = is_cplx<T>() ? 2 : 1;
-  constexpr int STAGE = NPL * (A_PLANE + B_PLANE);
-  R *L = reinterpret_cast<R *>(lds);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-  const int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
-  const int KT = (g.K + BK - 1) / BK;
-  for (int t = 0; t < tiles_m * tiles_n; ++t) {
-    const int m0 = (t % tiles_m) * BM, n0 = (t / tiles_m) * BN;
-    typename Mfma<R>::acc cr[TM][TN], ci[TM][TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        cr[i][j] = Mfma<R>::zero();
-        ci[i][j] = Mfma<R>::zero();
-      }
-    T ra[BM * BK / NT], rb[BN * BK / NT];
-    __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-      R *stage = L + (kt & 1) * STAGE;
-      R *As = stage, *Bs = stage + NPL * A_PLANE;
-      op_gload<T, BM, BN, BK, TA, TB, NT>(g, m0, n0, kt * BK, tid, ra, rb);
-      op_sstore<T, BM, BN, BK, TA, TB, NT>(As, Bs, tid, ra, rb);
-      __syncthreads();
-      if constexpr (is_cplx<T>()) {
-        cgemm_compute<T, BM, BN, WM, WN, NT, BK>(stage, lane, wm0, wn0, cr, ci);
-      } else {
-        rgemm_compute<T, BM, BN, WM, WN, NT, BK>(As, Bs, lane, wm0, wn0, cr);
-      }
-      // the next slice goes to the other stage: no second barrier needed
-    }
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm0 + i * 16 + Mfma<R>::row(lane, r);
-          const int col = n0 + wn0 + j * 16 + (lane & 15);
-          if (row < g.M && col < g.N) {
-            if constexpr (is_cplx<T>())
-              epilogue_store(g, row, col, T{R(cr[i][j][r]), R(ci[i][j][r])});
-            else
-              epilogue_store(g, row, col, R(cr[i][j][r]));
-          }
-        }
-  }
-  __syncthreads();
-}
-
-template <typename T, int CFG, bool TA, bool TB>
-__device__ void wg_gemm_ops(const GemmArgs<T> &g, char *lds) {
-  constexpr int c = effective_cfg<T>(CFG);
-  if constexpr (c == CLM_BATCH_TILE_128W16) {
-    // the FP32 real tile stages 32-deep slices (one 16-byte vector per lane)
-    constexpr int bk = std::is_same_v<T, float> ? 32 : kBK;
-    wg_gemm_op<T, 128, 128, 32, 32, bk, TA, TB, 1024>(g, lds);
-  } else if constexpr (c == CLM_BATCH_TILE_32) {
-    if constexpr (is_cplx<T>()) wg_gemm_op<T, 32, 32, 16, 16, kBK, TA, TB>(g, lds);
-    else wg_gemm_op<T, 32, 32, 16, 16, small_bk<T>(), TA, TB>(g, lds);
-  }
-  else if constexpr (c == CLM_BATCH_TILE_64) wg_gemm_op<T, 64, 64, 32, 32, kBK, TA, TB>(g, lds);
-  else if constexpr (c == CLM_BATCH_TILE_128x64 || is_cplx<T>())
-    wg_gemm_op<T, 128, 64, 64, 32, kBK, TA, TB>(g, lds);
-  else wg_gemm_op<T, 128, 128, 64, 64, kBK, TA, TB>(g, lds);
-}
 
 // The 1-norm in the fold order of the single-matrix reduction (256 virtual
 // lanes, each folding rows t, t+256, ... seeded by its first, then a halving
@@ -549,6 +407,35 @@ __device__ void wg_copy(int n, const T *src, long lds_, T *dst, long ldd) {
   __syncthreads();
 }
 
+// dst = src^H (n x n) through 32 x 32 LDS tiles (row 33), so the reads and the
+// writes both run down columns; ends with a barrier
+template <typename T>
+constexpr size_t transpose_lds() {
+  return 32 * 33 * sizeof(T);
+}
+template <typename T>
+__device__ void wg_conj_transpose(int n, const T *src, long lds_, T *dst, long ldd, char *lds) {
+  T *tile = reinterpret_cast<T *>(lds);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tiles = (n + 31) / 32;
+  for (int t = 0; t < tiles * tiles; ++t) {
+    const int ti = t % tiles, tj = t / tiles;  // source tile: rows 32 ti.., columns 32 tj..
+    __syncthreads();
+    for (int e = tid; e < 1024; e += nt) {
+      const int i = e & 31, j = e >> 5;
+      const int gi = ti * 32 + i, gj = tj * 32 + j;
+      if (gi < n && gj < n) tile[j * 33 + i] = src[gi + static_cast<long>(gj) * lds_];
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += nt) {
+      const int i = e & 31, j = e >> 5;  // dst(32 tj + i, 32 ti + j) = conj(src(32 ti + j, 32 tj + i))
+      const int gi = tj * 32 + i, gj = ti * 32 + j;
+      if (gi < n && gj < n) dst[gi + static_cast<long>(gj) * ldd] = conj_if(tile[i * 33 + j], true);
+    }
+  }
+  __syncthreads();
+}
+
 // the adjoint tape of one matrix (layout above tape_blocks)
 template <typename T>
 struct Tape {
@@ -558,13 +445,13 @@ struct Tape {
   int ckpt;
   __device__ T *horner(int j) const { return base + static_cast<long>(j - 1) * stride; }  // P_j, j >= 1
   __device__ T *checkpoint(int i) const {
-    return base + static_cast<long>(max_horner_steps() + i) * stride;
+    return base + static_cast<long>(kMaxHornerSteps + i) * stride;
   }
-  __device__ T *scratch(int i) const {  // U, V, W0, W1, Xb
-    return base + static_cast<long>(max_horner_steps() + ckpt + i) * stride;
+  __device__ T *scratch(int i) const {  // U, V, W0, W1, Xb, T1, T2
+    return base + static_cast<long>(kMaxHornerSteps + ckpt + i) * stride;
   }
   __device__ T *bank_bar(int i) const {  // cotangent of X0^(i+2)
-    return base + static_cast<long>(max_horner_steps() + ckpt + 5 + i) * stride;
+    return base + static_cast<long>(kMaxHornerSteps + ckpt + 7 + i) * stride;
   }
 };
 
@@ -612,7 +499,7 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   const long stride = stride_of<T>(n);
   T *As = reinterpret_cast<T *>(work + d.ws_off);
   T *bank = As + stride;
-  T *S = bank + static_cast<long>(max_bank_blocks()) * stride;
+  T *S = bank + static_cast<long>(kMaxBankBlocks) * stride;
 
   const bool ad = d.tape_off >= 0;
   const Tape<T> tape = tape_of<T>(d, work);
@@ -782,55 +669,118 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
   const clm_ps_plan pp = ps_plan(degree, 0);
   const int bs = uniform(pp.s), nhorner = uniform(pp.r), nbank = uniform(pp.num_powers);
   T *U = tape.scratch(0), *V = tape.scratch(1), *W0 = tape.scratch(2), *W1 = tape.scratch(3);
-  T *Xb = tape.scratch(4);
-
-  // squarings, last first
+  T *Xb = tape.scratch(4), *T1 = tape.scratch(5), *T2 = tape.scratch(6);
+  // Every product below is A B with both operands as stored (the NN workgroup
+  // product, with its prefetching loads); a conjugate-transposed operand is
+  // first written out by wg_conj_transpose: T1 for one that changes per step,
+  // T2 for the one a whole phase multiplies by. The three phases run as one
+  // sequence of steps with a single product call site (as the forward does), so
+  // the product code is instantiated once:
+  //   squarings, last first: for k = s-1 .. 0, R_k squared forward again from
+  //     its checkpoint, then bar <- bar R_k^H + R_k^H bar;
+  //   Horner steps j = 0 .. r-1, each after the block adjoint of B_j:
+  //     Xbar += P_(j+1)^H bar, bar <- bar X^H; then the block adjoint of B_r;
+  //   power bank, highest power first: Y_(k-1)bar += Y_k bar X0^H,
+  //     Xb += Y_(k-1)^H Y_k bar.
   const T *bar = static_cast<const T *>(d.G);
   long ldbar = d.ldg;
-  for (int k = nsq - 1; k >= 0; --k) {
-    const int base = k / cstep;
-    const T *Rk = tape.checkpoint(base);
-    for (int have = base * cstep, w = 0; have < k; ++have, w ^= 1) {
-      T *to = w ? W1 : W0;
-      wg_gemm<T, CFG>(square_args(n, Rk, long(n), Rk, long(n), to, long(n)), lds);
-      Rk = to;
-    }
-    T *out = (bar == U) ? V : U;
-    wg_gemm_ops<T, CFG, false, true>(square_args(n, bar, ldbar, Rk, long(n), out, long(n)), lds);
-    wg_gemm_ops<T, CFG, true, false>(accumulate(square_args(n, Rk, long(n), bar, ldbar, out, long(n))),
-                                     lds);
-    bar = out;
-    ldbar = n;
-  }
-
-  // Horner steps and the blocks they consume
-  wg_fill(n, Xb, long(n), ops<T>::zero());
-  for (int t = 0; t < nbank; ++t) wg_fill(n, tape.bank_bar(t), long(n), ops<T>::zero());
   const T *X = (bs >= 2) ? bank + static_cast<long>(bs - 2) * stride : X0;
   const long ldx = (bs >= 2) ? long(n) : ldx0;
   T *Xbar = (bs >= 2) ? tape.bank_bar(bs - 2) : Xb;
-  for (int j = 0; j <= nhorner; ++j) {
-    wg_block_adjoint(n, bar, ldbar, coef + static_cast<long>(j) * bs,
-                     min(degree - j * bs + 1, bs), Xb, tape);
-    if (j == nhorner) break;
-    wg_gemm_ops<T, CFG, true, false>(
-        accumulate(square_args(n, tape.horner(j + 1), long(n), bar, ldbar, Xbar, long(n))), lds);
-    T *out = (bar == U) ? V : U;
-    wg_gemm_ops<T, CFG, false, true>(square_args(n, bar, ldbar, X, ldx, out, long(n)), lds);
-    bar = out;
-    ldbar = n;
-  }
-
-  // power bank, highest power first
-  for (int k = nbank + 1; k >= 2; --k) {
-    T *ykb = tape.bank_bar(k - 2);
-    T *ykm1b = (k - 1 >= 2) ? tape.bank_bar(k - 3) : Xb;
-    const T *ykm1 = (k - 1 >= 2) ? bank + static_cast<long>(k - 3) * stride : X0;
-    const long ldy = (k - 1 >= 2) ? long(n) : ldx0;
-    wg_gemm_ops<T, CFG, false, true>(accumulate(square_args(n, ykb, long(n), X0, ldx0, ykm1b, long(n))),
-                                     lds);
-    wg_gemm_ops<T, CFG, true, false>(accumulate(square_args(n, ykm1, ldy, ykb, long(n), Xb, long(n))),
-                                     lds);
+  int phase = nsq > 0 ? 0 : 1;
+  int k = nsq - 1, have = -1, w = 0, sub = 0;  // squarings
+  int j = 0;                                   // Horner steps
+  int kb = nbank + 1;                          // power bank
+  bool started = false;
+  const T *Rk = nullptr;
+  T *out = nullptr;
+  for (;;) {
+    const T *tsrc = nullptr;  // transpose tsrc (ld tld) into tdst before the product
+    long tld = 0;
+    T *tdst = T1;
+    GemmArgs<T> g;
+    if (phase == 0) {
+      if (have < 0) {
+        const int base = k / cstep;
+        Rk = tape.checkpoint(base);
+        have = base * cstep;
+        w = 0;
+        sub = 0;
+      }
+      if (have < k) {
+        T *to = w ? W1 : W0;
+        g = square_args(n, Rk, long(n), Rk, long(n), to, long(n));
+        Rk = to;
+        ++have;
+        w ^= 1;
+      } else if (sub == 0) {
+        out = (bar == U) ? V : U;
+        tsrc = Rk;
+        tld = n;
+        g = square_args(n, bar, ldbar, T1, long(n), out, long(n));
+        sub = 1;
+      } else {
+        g = accumulate(square_args(n, T1, long(n), bar, ldbar, out, long(n)));
+        bar = out;
+        ldbar = n;
+        --k;
+        have = -1;
+        if (k < 0) phase = 1;
+      }
+    } else if (phase == 1) {
+      if (!started) {
+        wg_fill(n, Xb, long(n), ops<T>::zero());
+        for (int t = 0; t < nbank; ++t) wg_fill(n, tape.bank_bar(t), long(n), ops<T>::zero());
+        if (nhorner > 0) wg_conj_transpose(n, X, ldx, T2, long(n), lds);  // X^H
+        started = true;
+        sub = 0;
+      }
+      if (sub == 0) {
+        wg_block_adjoint(n, bar, ldbar, coef + static_cast<long>(j) * bs,
+                         min(degree - j * bs + 1, bs), Xb, tape);
+        if (j == nhorner) {
+          phase = 2;
+          started = false;
+          continue;
+        }
+        tsrc = tape.horner(j + 1);
+        tld = n;
+        g = accumulate(square_args(n, T1, long(n), bar, ldbar, Xbar, long(n)));
+        sub = 1;
+      } else {
+        out = (bar == U) ? V : U;
+        g = square_args(n, bar, ldbar, T2, long(n), out, long(n));
+        bar = out;
+        ldbar = n;
+        ++j;
+        sub = 0;
+      }
+    } else {
+      if (!started) {
+        if (nbank > 0) {
+          tsrc = X0;  // X0^H into T2, before the first bank product
+          tld = ldx0;
+          tdst = T2;
+        }
+        started = true;
+        sub = 0;
+      }
+      if (kb < 2) break;
+      T *ykb = tape.bank_bar(kb - 2);
+      if (sub == 0) {
+        T *ykm1b = (kb - 1 >= 2) ? tape.bank_bar(kb - 3) : Xb;
+        g = accumulate(square_args(n, ykb, long(n), T2, long(n), ykm1b, long(n)));
+        sub = 1;
+      } else {
+        tsrc = (kb - 1 >= 2) ? bank + static_cast<long>(kb - 3) * stride : X0;
+        tld = (kb - 1 >= 2) ? long(n) : ldx0;
+        g = accumulate(square_args(n, T1, long(n), ykb, long(n), Xb, long(n)));
+        sub = 0;
+        --kb;
+      }
+    }
+    if (tsrc) wg_conj_transpose(n, tsrc, tld, tdst, long(n), lds);
+    wg_gemm<T, CFG>(g, lds);
   }
   wg_scale_copy(n, ldexp(1.0, -nsq), Xb, long(n), dA, long(d.ldda));
 }
@@ -839,7 +789,9 @@ template <typename T, int CFG>
 __global__ void __launch_bounds__(threads_of(CFG))
     expm_adjoint_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
                         const T *coef) {
-  __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T, CFG>()];
+  constexpr size_t kLds = lds_bytes<T, CFG>() > transpose_lds<T>() ? lds_bytes<T, CFG>()
+                                                                     : transpose_lds<T>();
+  __shared__ __attribute__((aligned(16))) char lds[kLds];
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);

@@ -21,7 +21,7 @@
 
 namespace clm {
 
-__host__ __device__ inline int ps_cost(int degree, int s) {
+__host__ __device__ constexpr int ps_cost(int degree, int s) {
   if (s < 1 || degree < 1) return 0;
   const int r = degree / s;
   const int max_power = (r >= 1) ? s : degree;
@@ -29,7 +29,7 @@ __host__ __device__ inline int ps_cost(int degree, int s) {
   return num_powers + r;
 }
 
-__host__ __device__ inline int ps_block_size(int degree) {
+__host__ __device__ constexpr int ps_block_size(int degree) {
   if (degree < 1) return 1;
   int best_s = 1, best = ps_cost(degree, 1);
   for (int s = 2; s <= degree + 1; ++s) {
@@ -42,7 +42,7 @@ __host__ __device__ inline int ps_block_size(int degree) {
   return best_s;
 }
 
-__host__ __device__ inline clm_ps_plan ps_plan(int degree, int s_requested) {
+__host__ __device__ constexpr clm_ps_plan ps_plan(int degree, int s_requested) {
   clm_ps_plan p{1, 0, 0, 0, 0};
   if (degree < 0) return p;
   int s = (s_requested > 0) ? s_requested : ps_block_size(degree);
@@ -59,7 +59,7 @@ __host__ __device__ inline clm_ps_plan ps_plan(int degree, int s_requested) {
 
 constexpr int kNumTaylor = 10;
 
-__host__ __device__ inline int taylor_degree(int i) {
+__host__ __device__ constexpr int taylor_degree(int i) {
   switch (i) {
   case 0: return 1;
   case 1: return 2;
@@ -105,7 +105,7 @@ __host__ __device__ inline double taylor_theta_at(int i, bool single) {
 }
 
 // the largest power bank any ladder degree needs
-__host__ __device__ inline int max_bank_blocks() {
+__host__ __device__ constexpr int max_bank_blocks() {
   int mx = 0;
   for (int i = 0; i < kNumTaylor; ++i) {
     const int np = ps_plan(taylor_degree(i), 0).num_powers;

@@ -266,6 +266,10 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
         loss.backward()
         return x.grad
 
+    import calaman_g_amd as cg
+    t_tune = time.perf_counter()
+    table, _ = cg.batched_autotune(torch.complex64, nmax=n, adjoint=True)
+    tune_s = time.perf_counter() - t_tune
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -295,7 +299,8 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
             "matrices": b, "n": n, "dtype": "c64", "checkpoints": 4,
             "flop_per_step": flop, "mfma_tflops": round(achieved, 2),
             "mfma_peak_tflops": PEAK_FP32_TFLOPS, "mfma_frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "grad_finite": bool(torch.isfinite(grad).all().item())}
+            "grad_finite": bool(torch.isfinite(grad).all().item()),
+            "adjoint_tiles_c64": table, "autotune_s": round(tune_s, 2)}
 
 
 FP8_N = 4096

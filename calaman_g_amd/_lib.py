@@ -200,6 +200,9 @@ _SIGNATURES = {
     "clm_expm_batched_get_tuning": (_i, [_i, ctypes.POINTER(_i)]),
     "clm_expm_batched_set_tuning": (_i, [_i, ctypes.POINTER(_i)]),
     "clm_expm_batched_autotune": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_f)]),
+    "clm_expm_batched_get_tuning_ad": (_i, [_i, ctypes.POINTER(_i)]),
+    "clm_expm_batched_set_tuning_ad": (_i, [_i, ctypes.POINTER(_i)]),
+    "clm_expm_batched_autotune_ad": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_f)]),
     "clm_expm_batched_ad_plan_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchAdEntry),
                                              _i]),
     "clm_expm_batched_backward": (_i, [_vp, _vp, _vp, _vp, _sz]),

@@ -856,21 +856,24 @@ __global__ void fill_tuning_matrices(T *A, int n, int count, unsigned seed) {
 }
 
 // ---- tuning table ---------------------------------------------------------------------
-// cfg[precision: 0 FP32, 1 FP64][0 real, 1 complex][bucket], bucket b covers
-// n in (32 b, 32 (b + 1)], the last one everything above.
+// cfg[path: 0 forward plans, 1 adjoint plans][precision: 0 FP32, 1 FP64]
+// [0 real, 1 complex][bucket], bucket b covers n in (32 b, 32 (b + 1)], the
+// last one everything above. Adjoint plans have a table of their own: their
+// launches are dominated by the backward sweep, whose best tile differs.
 std::mutex g_tune_mutex;
-int g_tune[2][2][kBuckets];
+int g_tune[2][2][2][kBuckets];
 bool g_tune_init = false;
 
 void tune_defaults() {
-  for (int p = 0; p < 2; ++p)
-    for (int c = 0; c < 2; ++c)
-      for (int b = 0; b < kBuckets; ++b) {
-        const int nmax = 32 * (b + 1);
-        int cfg = nmax <= 32 ? CLM_BATCH_TILE_32 : nmax <= 96 ? CLM_BATCH_TILE_64
-                                                               : CLM_BATCH_TILE_128x64;
-        g_tune[p][c][b] = cfg;
-      }
+  for (int a = 0; a < 2; ++a)
+    for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < kBuckets; ++b) {
+          const int nmax = 32 * (b + 1);
+          int cfg = nmax <= 32 ? CLM_BATCH_TILE_32 : nmax <= 96 ? CLM_BATCH_TILE_64
+                                                                 : CLM_BATCH_TILE_128x64;
+          g_tune[a][p][c][b] = cfg;
+        }
   g_tune_init = true;
 }
 
@@ -1121,7 +1124,8 @@ clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *fo
     for (int i = 0; i < count; ++i) {
       const clm_batch_entry &e = entries[i];
       cfg[i] = forced_cfg ? *forced_cfg
-                          : g_tune[precision_of(e.dtype)][is_complex_code(e.dtype)][bucket_of(e.n)];
+                          : g_tune[ad ? 1 : 0][precision_of(e.dtype)][is_complex_code(e.dtype)]
+                                  [bucket_of(e.n)];
     }
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -1250,17 +1254,29 @@ clm_status_t launch_backward(clm_ctx *ctx, hipStream_t st, const clm_batch_plan 
 // configuration cfg; returns the median of `reps` timed launches in ms
 template <typename T>
 clm_status_t time_config(clm_ctx *ctx, hipStream_t st, int dtype, int n, int count, int cfg,
-                         int reps, float *ms) {
+                         int reps, float *ms, bool adjoint = false) {
   const size_t mat = static_cast<size_t>(n) * n * sizeof(T);
+  const int nmat = adjoint ? 4 : 2;  // A, E (+ G, dA)
   char *buf = nullptr;
-  if (hipMalloc(&buf, 2 * mat * count) != hipSuccess) return CLM_STATUS_ALLOC_FAILED;
+  if (hipMalloc(&buf, nmat * mat * count) != hipSuccess) return CLM_STATUS_ALLOC_FAILED;
   std::vector<clm_batch_entry> entries(count);
-  for (int i = 0; i < count; ++i)
+  std::vector<clm_batch_ad_entry> ad(adjoint ? count : 0);
+  for (int i = 0; i < count; ++i) {
     entries[i] = clm_batch_entry{n, dtype, buf + mat * i, n, buf + mat * (count + i), n};
+    if (adjoint)
+      ad[i] = clm_batch_ad_entry{n, dtype, entries[i].A, n, entries[i].E, n,
+                                 buf + mat * (2 * count + i), n, buf + mat * (3 * count + i), n};
+  }
+  // A and (for the adjoint) the cotangents G: the first 2 count matrices are A
+  // and E, so fill A, then G as a second batch of tuning matrices
   hipLaunchKernelGGL(fill_tuning_matrices<T>, dim3(1024), dim3(256), 0, st,
                      reinterpret_cast<T *>(buf), n, count, 12345u);
+  if (adjoint)
+    hipLaunchKernelGGL(fill_tuning_matrices<T>, dim3(1024), dim3(256), 0, st,
+                       reinterpret_cast<T *>(buf + mat * 2 * count), n, count, 777u);
   clm_batch_plan *plan = nullptr;
-  clm_status_t status = build_plan(count, entries.data(), &cfg, &plan);
+  clm_status_t status = build_plan(count, entries.data(), &cfg, &plan,
+                                   adjoint ? ad.data() : nullptr, adjoint ? 4 : 0);
   void *work = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<float> times;
@@ -1272,6 +1288,8 @@ clm_status_t time_config(clm_ctx *ctx, hipStream_t st, int dtype, int n, int cou
   for (int r = -1; status == CLM_STATUS_SUCCESS && r < reps; ++r) {
     if (hipEventRecord(e0, st) != hipSuccess) status = CLM_STATUS_EXECUTION_FAILED;
     if (status == CLM_STATUS_SUCCESS) status = launch_plan(ctx, st, plan, work, plan->ws_bytes, nullptr);
+    if (status == CLM_STATUS_SUCCESS && adjoint)
+      status = launch_backward(ctx, st, plan, work, plan->ws_bytes);
     if (status == CLM_STATUS_SUCCESS &&
         (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
       status = CLM_STATUS_EXECUTION_FAILED;
@@ -1368,35 +1386,38 @@ clm_status_t clm_expm_batched_graph(clm_ctx *ctx, clm_stream_t stream, clm_batch
   return CLM_STATUS_SUCCESS;
 }
 
-clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg) {
-  if (!clm::valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
-  std::lock_guard<std::mutex> lock(clm::g_tune_mutex);
-  if (!clm::g_tune_init) clm::tune_defaults();
-  for (int b = 0; b < clm::kBuckets; ++b)
-    cfg[b] = clm::g_tune[clm::precision_of(dtype)][clm::is_complex_code(dtype)][b];
+}  // extern "C"
+
+namespace clm {
+namespace {
+clm_status_t get_tuning(int path, int dtype, int *cfg) {
+  if (!valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (!g_tune_init) tune_defaults();
+  for (int b = 0; b < kBuckets; ++b) cfg[b] = g_tune[path][precision_of(dtype)][is_complex_code(dtype)][b];
   return CLM_STATUS_SUCCESS;
 }
 
-clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg) {
-  if (!clm::valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
-  for (int b = 0; b < clm::kBuckets; ++b)
+clm_status_t set_tuning(int path, int dtype, const int *cfg) {
+  if (!valid_dtype(dtype) || cfg == nullptr) return CLM_STATUS_INVALID_VALUE;
+  for (int b = 0; b < kBuckets; ++b)
     if (cfg[b] < 0 || cfg[b] >= CLM_BATCH_TILES) return CLM_STATUS_INVALID_VALUE;
-  std::lock_guard<std::mutex> lock(clm::g_tune_mutex);
-  if (!clm::g_tune_init) clm::tune_defaults();
-  for (int b = 0; b < clm::kBuckets; ++b)
-    clm::g_tune[clm::precision_of(dtype)][clm::is_complex_code(dtype)][b] = cfg[b];
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (!g_tune_init) tune_defaults();
+  for (int b = 0; b < kBuckets; ++b) g_tune[path][precision_of(dtype)][is_complex_code(dtype)][b] = cfg[b];
   return CLM_STATUS_SUCCESS;
 }
 
-clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
-                                       int *cfg_out, float *ms_out) {
-  if (!clm::valid_dtype(dtype) || nmax < 1 || nmax > clm::kMaxBatchN)
-    return CLM_STATUS_INVALID_VALUE;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  int chosen[clm::kBuckets];
-  CLM_TRY(clm_expm_batched_get_tuning(dtype, chosen));
-  const bool single = clm::precision_of(dtype) == 0;
-  const bool cplx = clm::is_complex_code(dtype);
+// times every distinct tile on one homogeneous batch per bucket up to nmax
+// (n = 32 b + 16) and installs the fastest: forward launches of 1024 matrices,
+// or (adjoint) forward + backward launches of 512
+clm_status_t autotune(clm_ctx *ctx, hipStream_t st, int path, int dtype, int nmax, int *cfg_out,
+                      float *ms_out) {
+  if (!valid_dtype(dtype) || nmax < 1 || nmax > kMaxBatchN) return CLM_STATUS_INVALID_VALUE;
+  int chosen[kBuckets];
+  CLM_TRY(get_tuning(path, dtype, chosen));
+  const bool single = precision_of(dtype) == 0;
+  const bool cplx = is_complex_code(dtype);
   // candidate tiles: those that are distinct for this type (128 x 128 of 4 waves
   // exists for real FP32 only, 128 x 64 for all but complex FP64)
   auto distinct = [&](int cfg) {
@@ -1405,20 +1426,21 @@ clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dt
     if (cfg == CLM_BATCH_TILE_128x64) return single || !cplx;
     return true;
   };
-  for (int b = 0; b < clm::kBuckets && 32 * b < nmax; ++b) {
+  const bool adj = path == 1;
+  for (int b = 0; b < kBuckets && 32 * b < nmax; ++b) {
     const int n = std::min(32 * b + 16, nmax);
-    // enough matrices to keep every workgroup slot busy twice over
-    const int count = 1024;
+    // enough matrices to keep every workgroup slot busy
+    const int count = adj ? 512 : 1024;
     float best = 0.f;
     for (int cfg = CLM_BATCH_TILE_32; cfg < CLM_BATCH_TILES; ++cfg) {
       if (!distinct(cfg)) continue;
       float ms = 0.f;
       clm_status_t s;
       switch (dtype) {
-      case CLM_DTYPE_F32: s = clm::time_config<float>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
-      case CLM_DTYPE_F64: s = clm::time_config<double>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
-      case CLM_DTYPE_C64: s = clm::time_config<clm::zc>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
-      default: s = clm::time_config<clm::zd>(ctx, st, dtype, n, count, cfg, 3, &ms); break;
+      case CLM_DTYPE_F32: s = time_config<float>(ctx, st, dtype, n, count, cfg, 3, &ms, adj); break;
+      case CLM_DTYPE_F64: s = time_config<double>(ctx, st, dtype, n, count, cfg, 3, &ms, adj); break;
+      case CLM_DTYPE_C64: s = time_config<zc>(ctx, st, dtype, n, count, cfg, 3, &ms, adj); break;
+      default: s = time_config<zd>(ctx, st, dtype, n, count, cfg, 3, &ms, adj); break;
       }
       if (s != CLM_STATUS_SUCCESS) return s;
       if (ms_out) ms_out[b * CLM_BATCH_TILES + cfg] = ms;
@@ -1429,8 +1451,32 @@ clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dt
     }
   }
   if (cfg_out)
-    for (int b = 0; b < clm::kBuckets; ++b) cfg_out[b] = chosen[b];
-  return clm_expm_batched_set_tuning(dtype, chosen);
+    for (int b = 0; b < kBuckets; ++b) cfg_out[b] = chosen[b];
+  return set_tuning(path, dtype, chosen);
+}
+}  // namespace
+}  // namespace clm
+
+extern "C" {
+
+clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg) { return clm::get_tuning(0, dtype, cfg); }
+clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg) {
+  return clm::set_tuning(0, dtype, cfg);
+}
+clm_status_t clm_expm_batched_get_tuning_ad(int dtype, int *cfg) {
+  return clm::get_tuning(1, dtype, cfg);
+}
+clm_status_t clm_expm_batched_set_tuning_ad(int dtype, const int *cfg) {
+  return clm::set_tuning(1, dtype, cfg);
+}
+
+clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
+                                       int *cfg_out, float *ms_out) {
+  return clm::autotune(ctx, static_cast<hipStream_t>(stream), 0, dtype, nmax, cfg_out, ms_out);
+}
+clm_status_t clm_expm_batched_autotune_ad(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
+                                          int *cfg_out, float *ms_out) {
+  return clm::autotune(ctx, static_cast<hipStream_t>(stream), 1, dtype, nmax, cfg_out, ms_out);
 }
 
 }  // extern "C"

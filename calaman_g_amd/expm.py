@@ -458,25 +458,31 @@ def expm_batched_autograd(a: torch.Tensor, checkpoints: int = 4) -> torch.Tensor
     return ExpmBatchedFunction.apply(a, checkpoints)
 
 
-def batched_tuning(dtype: torch.dtype) -> list:
+def batched_tuning(dtype: torch.dtype, adjoint: bool = False) -> list:
+    """The tile table of forward plans (adjoint=False) or of adjoint plans."""
     arr = (ctypes.c_int * _lib.BATCH_BUCKETS)()
-    check(lib().clm_expm_batched_get_tuning(_DTYPE_CODE[dtype], arr), "get_tuning")
+    f = lib().clm_expm_batched_get_tuning_ad if adjoint else lib().clm_expm_batched_get_tuning
+    check(f(_DTYPE_CODE[dtype], arr), "get_tuning")
     return list(arr)
 
 
-def set_batched_tuning(dtype: torch.dtype, cfg) -> None:
+def set_batched_tuning(dtype: torch.dtype, cfg, adjoint: bool = False) -> None:
     arr = (ctypes.c_int * _lib.BATCH_BUCKETS)(*cfg)
-    check(lib().clm_expm_batched_set_tuning(_DTYPE_CODE[dtype], arr), "set_tuning")
+    f = lib().clm_expm_batched_set_tuning_ad if adjoint else lib().clm_expm_batched_set_tuning
+    check(f(_DTYPE_CODE[dtype], arr), "set_tuning")
 
 
 def batched_autotune(dtype: torch.dtype, nmax: int = 256,
-                     stream: Optional[torch.cuda.Stream] = None) -> tuple:
-    """Times every workgroup tile per order bucket up to nmax; returns (table, ms[bucket][tile])."""
+                     stream: Optional[torch.cuda.Stream] = None, adjoint: bool = False) -> tuple:
+    """Times every workgroup tile per order bucket up to nmax (forward launches, or
+    forward + backward launches for adjoint=True) and installs the fastest;
+    returns (table, ms[bucket][tile])."""
     cfg = (ctypes.c_int * _lib.BATCH_BUCKETS)()
     nt = _lib.BATCH_TILES
     ms = (ctypes.c_float * (nt * _lib.BATCH_BUCKETS))()
-    check(lib().clm_expm_batched_autotune(None, ctypes.c_void_p(_stream_ptr(stream)),
-                                          _DTYPE_CODE[dtype], int(nmax), cfg, ms), "autotune")
+    f = lib().clm_expm_batched_autotune_ad if adjoint else lib().clm_expm_batched_autotune
+    check(f(None, ctypes.c_void_p(_stream_ptr(stream)), _DTYPE_CODE[dtype], int(nmax), cfg, ms),
+          "autotune")
     return list(cfg), [list(ms[nt * b:nt * b + nt]) for b in range(_lib.BATCH_BUCKETS)]
 
 

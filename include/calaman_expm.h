@@ -277,6 +277,13 @@ clm_status_t clm_expm_batched_get_tuning(int dtype, int *cfg);
 clm_status_t clm_expm_batched_set_tuning(int dtype, const int *cfg);
 clm_status_t clm_expm_batched_autotune(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
                                        int *cfg_out, float *ms_out);
+/* Adjoint plans (clm_expm_batched_ad_plan_create) take their tiles from a table
+ * of their own, since their launches are dominated by the backward sweep; its
+ * tuner times forward + backward launches of 512 matrices per bucket. */
+clm_status_t clm_expm_batched_get_tuning_ad(int dtype, int *cfg);
+clm_status_t clm_expm_batched_set_tuning_ad(int dtype, const int *cfg);
+clm_status_t clm_expm_batched_autotune_ad(clm_ctx *ctx, clm_stream_t stream, int dtype, int nmax,
+                                          int *cfg_out, float *ms_out);
 
 /* ---- batched adjoint (Frechet derivative backward) -----------------------------------
  * No reference counterpart. For L(E) with cotangent G = dL/dE (the conjugate

@@ -210,6 +210,22 @@ def test_batched_tuning_table_round_trip():
     assert L.clm_expm_batched_set_tuning(ST.DTYPE_C64, (ctypes.c_int * 8)(*saved)) == 0
 
 
+def test_adjoint_tuning_table_is_separate():
+    fwd, ad = (ctypes.c_int * _lib.BATCH_BUCKETS)(), (ctypes.c_int * _lib.BATCH_BUCKETS)()
+    assert L.clm_expm_batched_get_tuning(ST.DTYPE_F64, fwd) == 0
+    assert L.clm_expm_batched_get_tuning_ad(ST.DTYPE_F64, ad) == 0
+    saved_f, saved_a = list(fwd), list(ad)
+    new = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([_lib.BATCH_TILE_128W16] * _lib.BATCH_BUCKETS))
+    assert L.clm_expm_batched_set_tuning_ad(ST.DTYPE_F64, new) == 0
+    assert L.clm_expm_batched_get_tuning_ad(ST.DTYPE_F64, ad) == 0
+    assert list(ad) == [_lib.BATCH_TILE_128W16] * 8
+    assert L.clm_expm_batched_get_tuning(ST.DTYPE_F64, fwd) == 0 and list(fwd) == saved_f
+    bad = (ctypes.c_int * _lib.BATCH_BUCKETS)(*([-1] * _lib.BATCH_BUCKETS))
+    assert L.clm_expm_batched_set_tuning_ad(ST.DTYPE_F64, bad) == ST.INVALID_VALUE
+    assert L.clm_expm_batched_autotune_ad(None, None, 9, 64, None, None) == ST.INVALID_VALUE
+    assert L.clm_expm_batched_set_tuning_ad(ST.DTYPE_F64, (ctypes.c_int * 8)(*saved_a)) == 0
+
+
 def test_fp8_buffer_sizes_and_argument_checks():
     """the FP8 workspaces: the split workspace plus three n x n byte planes and their
     tile exponents; the product's two operand planes and exponents; bad options

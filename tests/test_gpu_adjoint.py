@@ -172,3 +172,45 @@ def test_autograd_unitary_generator_batch(dev):
     for i in range(b):
         ref = oracle.frechet_adjoint(a[i].astype(np.complex128), q[i])
         assert _rel(x.grad[i].cpu().numpy(), ref) <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.float64])
+def test_adjoint_every_tile_gives_the_same_bits(dev, dtype):
+    """every backward product is the NN workgroup product after an exact
+    conjugate transpose: the tile decides which lanes compute an element, not its
+    arithmetic, so all five tiles give the same gradient bits"""
+    rng = np.random.default_rng(8)
+    specs = [(7, 0.5), (45, 6.0), (130, 12.0)]
+    mats = [_mat(rng, n, dtype, t) for n, t in specs]
+    grads = [_mat(rng, n, dtype, 3.0) for n, _ in specs]
+    saved = cg.batched_tuning(dtype, adjoint=True)
+    try:
+        ref = None
+        for cfg in range(5):
+            cg.set_batched_tuning(dtype, [cfg] * 8, adjoint=True)
+            _, ds, _ = _run(dev, mats, grads)
+            if ref is None:
+                ref = ds
+            for x, y in zip(ds, ref):
+                assert np.array_equal(x, y), cfg
+    finally:
+        cg.set_batched_tuning(dtype, saved, adjoint=True)
+
+
+def test_adjoint_autotune_small(dev):
+    saved = cg.batched_tuning(torch.complex64, adjoint=True)
+    fwd = cg.batched_tuning(torch.complex64)
+    try:
+        table, ms = cg.batched_autotune(torch.complex64, nmax=40, adjoint=True)
+        assert all(0 <= c < 5 for c in table)
+        assert cg.batched_tuning(torch.complex64, adjoint=True) == table
+        assert cg.batched_tuning(torch.complex64) == fwd  # the forward table is untouched
+        for b in (0, 1):
+            assert min(t for t in ms[b] if t > 0) == ms[b][table[b]]
+        rng = np.random.default_rng(9)
+        a, g = _mat(rng, 33, torch.complex64, 2.0), _mat(rng, 33, torch.complex64, 1.0)
+        _, ds, _ = _run(dev, [a], [g])
+        ref = oracle.frechet_adjoint(a.astype(np.complex128), g.astype(np.complex128))
+        assert _rel(ds[0], ref) <= 1e-4
+    finally:
+        cg.set_batched_tuning(torch.complex64, saved, adjoint=True)

@@ -108,6 +108,38 @@ def time_expms(mats, dev, steps, warmup, split=False):
     return max(wall, ev) / steps, [dict(p) for p in plans], results
 
 
+def per_norm_split(mats, dev, reps=5):
+    """FP64 and split-BF16 expm time per matrix (one norm at a time): the speedup of
+    the precision variant depends on s, so the per-norm ratios are reported beside
+    the whole-step one."""
+    import numpy as np
+    import torch
+    import calaman_g_amd as cg
+    from calaman_g_amd.expm import expm_cm, expm_split_cm
+
+    n = mats[0].shape[0]
+    work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
+    out = {}
+    for t, a in zip(NORMS, mats):
+        da = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+        de = torch.empty_like(da)
+        row = {}
+        for key, fn in (("fp64_ms", expm_cm), ("split_ms", expm_split_cm)):
+            fn(n, da, n, de, n, work)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn(n, da, n, de, n, work)
+            e1.record()
+            torch.cuda.synchronize()
+            row[key] = round(e0.elapsed_time(e1) / reps, 4)
+        row["speedup"] = round(row["fp64_ms"] / row["split_ms"], 3)
+        out[str(t)] = row
+    return out
+
+
 def gemm_roofline(dev, n=N, reps=20):
     """Mean duration of the FP64 GEMM kernel at n, HIP events on its stream."""
     import torch
@@ -426,6 +458,9 @@ def main():
     ssec, splans, sresults = time_expms(real, dev, args.steps, args.warmup, split=True)
     log(f"split-bf16: {ssec * 1e3:.3f} ms/step, {len(NORMS) / ssec:.1f} expm/s, plans {splans}")
 
+    split_per_norm = per_norm_split(real, dev)
+    log(f"split per norm: {split_per_norm}")
+
     cplx = make_inputs(N, cplx=True)
     zsec, zplans, _ = time_expms(cplx, dev, max(2, args.steps // 4), 1)
     log(f"complex128: {zsec * 1e3:.3f} ms/step, {len(NORMS) / zsec:.1f} expm/s")
@@ -476,6 +511,7 @@ def main():
             "split_bf16_speedup_vs_fp64": round(sec / ssec, 3),
             "split_bf16_rel_err_vs_fp64_oracle": serrs,
             "split_bf16_max_rel_err": (max(serrs) if serrs else None),
+            "split_bf16_per_norm": split_per_norm,
             "split_bf16_plans": {str(t): {"taylor_degree": p["m"], "s": p["s"],
                                           "ps_split": p["ps_split"],
                                           "split_squarings": p["split_squarings"]}

@@ -95,8 +95,8 @@ __host__ __device__ inline long stride_of(int n) {
 //   U, V             cotangent ping-pong of the backward sweep
 //   W0, W1           squarings recomputed from a checkpoint
 //   Xb               cotangent of the scaled matrix X0 = A / 2^s
-//   T1, T2           conjugate transposes of product operands
 //   Yb[0..nbank)     cotangents of the power bank X0^2 ..
+// (the backward keeps every cotangent conjugate-transposed)
 __host__ __device__ constexpr int max_horner_steps() {
   int mx = 0;
   for (int i = 0; i < kNumTaylor; ++i) {
@@ -115,7 +115,7 @@ size_t matrix_ws_bytes(int n) {
   return static_cast<size_t>(2 + kMaxBankBlocks) * stride_of<T>(n) * sizeof(T);
 }
 __host__ __device__ inline int tape_blocks(int ckpt) {
-  return kMaxHornerSteps + ckpt + 7 + kMaxBankBlocks;
+  return kMaxHornerSteps + ckpt + 5 + kMaxBankBlocks;
 }
 template <typename T>
 size_t tape_bytes(int n, int ckpt) {
@@ -370,13 +370,41 @@ __device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
   return r;
 }
 
-// out = f * in, column by column (f a power of two: exact)
+// Element-wise passes over an n x n matrix: the workgroup walks the elements in
+// column-major order, each lane taking kEw of them per round (consecutive lanes
+// on consecutive rows), and every round issues all of its loads before any of
+// its stores -- a lane has kEw loads in flight instead of one, which is what a
+// single workgroup's pass over a matrix is bound by (load latency, not
+// bandwidth). Each element's arithmetic is unchanged.
+constexpr int kEw = 8;
+
+// (row, column) of linear element e of an n x n matrix
+__device__ __forceinline__ void ew_rc(int e, int n, int &i, int &j) {
+  j = e / n;
+  i = e - j * n;
+}
+
+// out = f * in (f a power of two: exact)
 template <typename T>
 __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, long ldo) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int j = wave; j < n; j += nwv)
-    for (int i = lane; i < n; i += 64)
-      out[i + static_cast<long>(j) * ldo] = ops<T>::scale(in[i + static_cast<long>(j) * ldi], f);
+  const int total = n * n, nt = blockDim.x;
+  for (int base = threadIdx.x; base < total; base += nt * kEw) {
+    T v[kEw];
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int e = base + u * nt;
+      int i, j;
+      ew_rc(e, n, i, j);
+      if (e < total) v[u] = in[i + static_cast<long>(j) * ldi];
+    }
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int e = base + u * nt;
+      int i, j;
+      ew_rc(e, n, i, j);
+      if (e < total) out[i + static_cast<long>(j) * ldo] = ops<T>::scale(v[u], f);
+    }
+  }
   __syncthreads();
 }
 
@@ -385,55 +413,87 @@ __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, lo
 template <typename T>
 __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms, const T *X1,
                                long ldx1, const T *bank, long stride, long ldbank) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int j = wave; j < n; j += nwv)
-    for (int i = lane; i < n; i += 64) {
-      T acc = (i == j) ? coef[0] : ops<T>::zero();
-      if (nterms > 1) acc = ops<T>::mad(coef[1], X1[i + static_cast<long>(j) * ldx1], acc);
-      for (int t = 2; t < nterms; ++t)
-        acc = ops<T>::mad(coef[t], bank[static_cast<long>(t - 2) * stride + i + j * ldbank], acc);
-      B[i + static_cast<long>(j) * ldb] = acc;
+  const int total = n * n, nt = blockDim.x;
+  for (int base = threadIdx.x; base < total; base += nt * kEw) {
+    T acc[kEw];
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int e = base + u * nt;
+      int i, j;
+      ew_rc(e, n, i, j);
+      acc[u] = (i == j) ? coef[0] : ops<T>::zero();
+      if (e < total && nterms > 1)
+        acc[u] = ops<T>::mad(coef[1], X1[i + static_cast<long>(j) * ldx1], acc[u]);
     }
+    for (int t = 2; t < nterms; ++t) {
+      const T *bt = bank + static_cast<long>(t - 2) * stride;
+#pragma unroll
+      for (int u = 0; u < kEw; ++u) {
+        const int e = base + u * nt;
+        int i, j;
+        ew_rc(e, n, i, j);
+        if (e < total) acc[u] = ops<T>::mad(coef[t], bt[i + j * ldbank], acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int e = base + u * nt;
+      int i, j;
+      ew_rc(e, n, i, j);
+      if (e < total) B[i + static_cast<long>(j) * ldb] = acc[u];
+    }
+  }
   __syncthreads();
 }
 
-// dst = src, column by column
+// dst = src
 template <typename T>
 __device__ void wg_copy(int n, const T *src, long lds_, T *dst, long ldd) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int j = wave; j < n; j += nwv)
-    for (int i = lane; i < n; i += 64)
-      dst[i + static_cast<long>(j) * ldd] = src[i + static_cast<long>(j) * lds_];
-  __syncthreads();
+  wg_scale_copy(n, 1.0, src, lds_, dst, ldd);
 }
 
-// dst = src^H (n x n) through 32 x 32 LDS tiles (row 33), so the reads and the
-// writes both run down columns; ends with a barrier
+// dst = f src^H (n x n, f a power of two) through 32 x 32 LDS tiles (row 33), so
+// the reads and the writes both run down columns; NT tiles per pair of barriers
+// (as many as the kernel's LDS holds), every lane's loads of a batch in flight
+// together; ends with a barrier
 template <typename T>
 constexpr size_t transpose_lds() {
   return 32 * 33 * sizeof(T);
 }
-template <typename T>
-__device__ void wg_conj_transpose(int n, const T *src, long lds_, T *dst, long ldd, char *lds) {
+template <typename T, int NTILE>
+__device__ void wg_conj_transpose(int n, const T *src, long lds_, T *dst, long ldd, char *lds,
+                                  double f = 1.0) {
   T *tile = reinterpret_cast<T *>(lds);
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int tiles = (n + 31) / 32;
-  for (int t = 0; t < tiles * tiles; ++t) {
-    const int ti = t % tiles, tj = t / tiles;  // source tile: rows 32 ti.., columns 32 tj..
+  const int tiles = (n + 31) / 32, count = tiles * tiles;
+  for (int t0 = 0; t0 < count; t0 += NTILE) {
     __syncthreads();
-    for (int e = tid; e < 1024; e += nt) {
-      const int i = e & 31, j = e >> 5;
+    for (int e = tid; e < 1024 * NTILE; e += nt) {
+      const int t = t0 + (e >> 10);
+      const int i = e & 31, j = (e >> 5) & 31;
+      const int ti = t % tiles, tj = t / tiles;  // source tile: rows 32 ti.., columns 32 tj..
       const int gi = ti * 32 + i, gj = tj * 32 + j;
-      if (gi < n && gj < n) tile[j * 33 + i] = src[gi + static_cast<long>(gj) * lds_];
+      if (t < count && gi < n && gj < n)
+        tile[(e >> 10) * 1056 + j * 33 + i] = src[gi + static_cast<long>(gj) * lds_];
     }
     __syncthreads();
-    for (int e = tid; e < 1024; e += nt) {
-      const int i = e & 31, j = e >> 5;  // dst(32 tj + i, 32 ti + j) = conj(src(32 ti + j, 32 tj + i))
+    for (int e = tid; e < 1024 * NTILE; e += nt) {
+      const int t = t0 + (e >> 10);
+      const int i = e & 31, j = (e >> 5) & 31;  // dst(32 tj + i, 32 ti + j) = conj(src(32 ti + j, 32 tj + i))
+      const int ti = t % tiles, tj = t / tiles;
       const int gi = tj * 32 + i, gj = ti * 32 + j;
-      if (gi < n && gj < n) dst[gi + static_cast<long>(gj) * ldd] = conj_if(tile[i * 33 + j], true);
+      if (t < count && gi < n && gj < n)
+        dst[gi + static_cast<long>(gj) * ldd] =
+            ops<T>::scale(conj_if(tile[(e >> 10) * 1056 + i * 33 + j], true), f);
     }
   }
   __syncthreads();
+}
+
+// the adjoint kernel's LDS: the product's stages, at least one transpose tile
+template <typename T, int CFG>
+constexpr size_t kAdjointLds() {
+  return lds_bytes<T, CFG>() > transpose_lds<T>() ? lds_bytes<T, CFG>() : transpose_lds<T>();
 }
 
 // the adjoint tape of one matrix (layout above tape_blocks)
@@ -447,11 +507,11 @@ struct Tape {
   __device__ T *checkpoint(int i) const {
     return base + static_cast<long>(kMaxHornerSteps + i) * stride;
   }
-  __device__ T *scratch(int i) const {  // U, V, W0, W1, Xb, T1, T2
+  __device__ T *scratch(int i) const {  // U, V, W0, W1, Xb
     return base + static_cast<long>(kMaxHornerSteps + ckpt + i) * stride;
   }
   __device__ T *bank_bar(int i) const {  // cotangent of X0^(i+2)
-    return base + static_cast<long>(kMaxHornerSteps + ckpt + 7 + i) * stride;
+    return base + static_cast<long>(kMaxHornerSteps + ckpt + 5 + i) * stride;
   }
 };
 
@@ -612,30 +672,46 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
 // The R_k come from the checkpoints (squared forward again between them), the
 // Horner iterates and the power bank from the forward's tape and workspace.
 
-// dst_t += conj(c_t) * bar for the block terms t = 1 .. terms-1 (t = 1 into Xb,
-// t >= 2 into the bank cotangents)
+// The cotangents are kept conjugate-transposed (see adjoint_one): for the block
+// terms t = 1 .. terms-1, dst_t^H += c_t bar^H, i.e. dst_t += c_t * bt with bt
+// the transposed cotangent held (t = 1 into Xb, t >= 2 into the bank cotangents)
 template <typename T>
-__device__ void wg_block_adjoint(int n, const T *bar, long ldbar, const T *c, int terms, T *Xb,
+__device__ void wg_block_adjoint(int n, const T *bt, const T *c, int terms, T *Xb,
                                  const Tape<T> &tape) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int j = wave; j < n; j += nwv)
-    for (int i = lane; i < n; i += 64) {
-      const T b = bar[i + static_cast<long>(j) * ldbar];
-      const long at = i + static_cast<long>(j) * n;
-      if (terms > 1) Xb[at] = ops<T>::mad(conj_if(c[1], true), b, Xb[at]);
-      for (int t = 2; t < terms; ++t) {
-        T *y = tape.bank_bar(t - 2);
-        y[at] = ops<T>::mad(conj_if(c[t], true), b, y[at]);
+  const int total = n * n, nt = blockDim.x;
+  for (int base = threadIdx.x; base < total; base += nt * kEw) {
+    T b[kEw];
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int e = base + u * nt;
+      if (e < total) b[u] = bt[e];
+    }
+    for (int t = 1; t < terms; ++t) {
+      T *y = (t == 1) ? Xb : tape.bank_bar(t - 2);
+      T v[kEw];
+#pragma unroll
+      for (int u = 0; u < kEw; ++u) {
+        const int e = base + u * nt;
+        if (e < total) v[u] = y[e];
+      }
+#pragma unroll
+      for (int u = 0; u < kEw; ++u) {
+        const int e = base + u * nt;
+        if (e < total) y[e] = ops<T>::mad(c[t], b[u], v[u]);
       }
     }
+  }
   __syncthreads();
 }
 
 template <typename T>
 __device__ void wg_fill(int n, T *dst, long ld, T v) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int j = wave; j < n; j += nwv)
-    for (int i = lane; i < n; i += 64) dst[i + static_cast<long>(j) * ld] = v;
+  const int total = n * n, nt = blockDim.x;
+  for (int e = threadIdx.x; e < total; e += nt) {
+    int i, j;
+    ew_rc(e, n, i, j);
+    dst[i + static_cast<long>(j) * ld] = v;
+  }
   __syncthreads();
 }
 
@@ -669,21 +745,25 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
   const clm_ps_plan pp = ps_plan(degree, 0);
   const int bs = uniform(pp.s), nhorner = uniform(pp.r), nbank = uniform(pp.num_powers);
   T *U = tape.scratch(0), *V = tape.scratch(1), *W0 = tape.scratch(2), *W1 = tape.scratch(3);
-  T *Xb = tape.scratch(4), *T1 = tape.scratch(5), *T2 = tape.scratch(6);
-  // Every product below is A B with both operands as stored (the NN workgroup
-  // product, with its prefetching loads); a conjugate-transposed operand is
-  // first written out by wg_conj_transpose: T1 for one that changes per step,
-  // T2 for the one a whole phase multiplies by. The three phases run as one
+  T *Xb = tape.scratch(4);
+  // The sweep runs on the conjugate transposes of the cotangents, B = bar^H:
+  // every rule above turns into products of stored operands, with no operand
+  // transposed, e.g. bar R^H + R^H bar = (R B + B R)^H and X0bar += Y^H Ybar
+  // becomes X0bar^H += Ybar^H Y. G is transposed once on the way in and X0bar
+  // once on the way out (into dA, with the 2^-s). The three phases run as one
   // sequence of steps with a single product call site (as the forward does), so
   // the product code is instantiated once:
   //   squarings, last first: for k = s-1 .. 0, R_k squared forward again from
-  //     its checkpoint, then bar <- bar R_k^H + R_k^H bar;
+  //     its checkpoint, then B <- R_k B + B R_k;
   //   Horner steps j = 0 .. r-1, each after the block adjoint of B_j:
-  //     Xbar += P_(j+1)^H bar, bar <- bar X^H; then the block adjoint of B_r;
-  //   power bank, highest power first: Y_(k-1)bar += Y_k bar X0^H,
-  //     Xb += Y_(k-1)^H Y_k bar.
-  const T *bar = static_cast<const T *>(d.G);
-  long ldbar = d.ldg;
+  //     Xbar^H += B P_(j+1), B <- X B; then the block adjoint of B_r;
+  //   power bank, highest power first: Y_(k-1)bar^H += X0 Y_k bar^H,
+  //     X0bar^H += Y_k bar^H Y_(k-1).
+  constexpr int kTiles = kAdjointLds<T, CFG>() / transpose_lds<T>() < 8
+                             ? int(kAdjointLds<T, CFG>() / transpose_lds<T>())
+                             : 8;
+  wg_conj_transpose<T, kTiles>(n, static_cast<const T *>(d.G), d.ldg, U, long(n), lds);
+  const T *bar = U;
   const T *X = (bs >= 2) ? bank + static_cast<long>(bs - 2) * stride : X0;
   const long ldx = (bs >= 2) ? long(n) : ldx0;
   T *Xbar = (bs >= 2) ? tape.bank_bar(bs - 2) : Xb;
@@ -695,9 +775,6 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
   const T *Rk = nullptr;
   T *out = nullptr;
   for (;;) {
-    const T *tsrc = nullptr;  // transpose tsrc (ld tld) into tdst before the product
-    long tld = 0;
-    T *tdst = T1;
     GemmArgs<T> g;
     if (phase == 0) {
       if (have < 0) {
@@ -715,14 +792,11 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
         w ^= 1;
       } else if (sub == 0) {
         out = (bar == U) ? V : U;
-        tsrc = Rk;
-        tld = n;
-        g = square_args(n, bar, ldbar, T1, long(n), out, long(n));
+        g = square_args(n, Rk, long(n), bar, long(n), out, long(n));
         sub = 1;
       } else {
-        g = accumulate(square_args(n, T1, long(n), bar, ldbar, out, long(n)));
+        g = accumulate(square_args(n, bar, long(n), Rk, long(n), out, long(n)));
         bar = out;
-        ldbar = n;
         --k;
         have = -1;
         if (k < 0) phase = 1;
@@ -731,67 +805,50 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
       if (!started) {
         wg_fill(n, Xb, long(n), ops<T>::zero());
         for (int t = 0; t < nbank; ++t) wg_fill(n, tape.bank_bar(t), long(n), ops<T>::zero());
-        if (nhorner > 0) wg_conj_transpose(n, X, ldx, T2, long(n), lds);  // X^H
         started = true;
         sub = 0;
       }
       if (sub == 0) {
-        wg_block_adjoint(n, bar, ldbar, coef + static_cast<long>(j) * bs,
-                         min(degree - j * bs + 1, bs), Xb, tape);
+        wg_block_adjoint(n, bar, coef + static_cast<long>(j) * bs, min(degree - j * bs + 1, bs),
+                         Xb, tape);
         if (j == nhorner) {
           phase = 2;
-          started = false;
           continue;
         }
-        tsrc = tape.horner(j + 1);
-        tld = n;
-        g = accumulate(square_args(n, T1, long(n), bar, ldbar, Xbar, long(n)));
+        g = accumulate(square_args(n, bar, long(n), tape.horner(j + 1), long(n), Xbar, long(n)));
         sub = 1;
       } else {
         out = (bar == U) ? V : U;
-        g = square_args(n, bar, ldbar, T2, long(n), out, long(n));
+        g = square_args(n, X, ldx, bar, long(n), out, long(n));
         bar = out;
-        ldbar = n;
         ++j;
         sub = 0;
       }
     } else {
-      if (!started) {
-        if (nbank > 0) {
-          tsrc = X0;  // X0^H into T2, before the first bank product
-          tld = ldx0;
-          tdst = T2;
-        }
-        started = true;
-        sub = 0;
-      }
       if (kb < 2) break;
       T *ykb = tape.bank_bar(kb - 2);
       if (sub == 0) {
         T *ykm1b = (kb - 1 >= 2) ? tape.bank_bar(kb - 3) : Xb;
-        g = accumulate(square_args(n, ykb, long(n), T2, long(n), ykm1b, long(n)));
+        g = accumulate(square_args(n, X0, ldx0, ykb, long(n), ykm1b, long(n)));
         sub = 1;
       } else {
-        tsrc = (kb - 1 >= 2) ? bank + static_cast<long>(kb - 3) * stride : X0;
-        tld = (kb - 1 >= 2) ? long(n) : ldx0;
-        g = accumulate(square_args(n, T1, long(n), ykb, long(n), Xb, long(n)));
+        const T *ykm1 = (kb - 1 >= 2) ? bank + static_cast<long>(kb - 3) * stride : X0;
+        const long ldy = (kb - 1 >= 2) ? long(n) : ldx0;
+        g = accumulate(square_args(n, ykb, long(n), ykm1, ldy, Xb, long(n)));
         sub = 0;
         --kb;
       }
     }
-    if (tsrc) wg_conj_transpose(n, tsrc, tld, tdst, long(n), lds);
     wg_gemm<T, CFG>(g, lds);
   }
-  wg_scale_copy(n, ldexp(1.0, -nsq), Xb, long(n), dA, long(d.ldda));
+  wg_conj_transpose<T, kTiles>(n, Xb, long(n), dA, long(d.ldda), lds, ldexp(1.0, -nsq));
 }
 
 template <typename T, int CFG>
 __global__ void __launch_bounds__(threads_of(CFG))
     expm_adjoint_kernel(const BatchDesc *__restrict__ desc, int count, int *queue, char *work,
                         const T *coef) {
-  constexpr size_t kLds = lds_bytes<T, CFG>() > transpose_lds<T>() ? lds_bytes<T, CFG>()
-                                                                     : transpose_lds<T>();
-  __shared__ __attribute__((aligned(16))) char lds[kLds];
+  __shared__ __attribute__((aligned(16))) char lds[kAdjointLds<T, CFG>()];
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);

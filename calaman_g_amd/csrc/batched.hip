@@ -1091,6 +1091,10 @@ struct clm_batch_plan {
   int count = 0;
   int group_count[clm::kGroups] = {};  // matrices per (element type, tile) group
   int group_start[clm::kGroups] = {};
+  double group_cost[clm::kGroups] = {};  // sum of n^3 (x4 complex, x2 FP64) of the members
+  int launch_order[clm::kGroups] = {};   // non-empty groups, costliest first
+  int launch_lane[clm::kGroups] = {};    // 0: the caller's stream, i > 0: side[i - 1]
+  int num_launch = 0;
   clm::BatchDesc *d_desc = nullptr;    // grouped, each group in decreasing cost
   bool ad = false;                     // records the adjoint tape; backward allowed
   int ckpt = 0;
@@ -1130,9 +1134,7 @@ void destroy_plan(clm_batch_plan *plan) {
 // join it after.
 template <typename F>
 clm_status_t for_each_group(const clm_batch_plan *plan, hipStream_t st, F launch) {
-  int groups[kGroups], ng = 0;
-  for (int g = 0; g < kGroups; ++g)
-    if (plan->group_count[g] > 0) groups[ng++] = g;
+  const int ng = plan->num_launch;
   if (ng == 0) return CLM_STATUS_SUCCESS;
   const int nside = std::min(ng - 1, clm_batch_plan::kSide);
   if (nside > 0) {
@@ -1140,8 +1142,9 @@ clm_status_t for_each_group(const clm_batch_plan *plan, hipStream_t st, F launch
     for (int i = 0; i < nside; ++i) CLM_TRY_HIP(hipStreamWaitEvent(plan->side[i], plan->fork, 0));
   }
   for (int k = 0; k < ng; ++k) {
-    hipStream_t s = (k == 0) ? st : plan->side[(k - 1) % clm_batch_plan::kSide];
-    CLM_TRY_HIP(launch(groups[k], s));
+    const int lane = plan->launch_lane[k];
+    hipStream_t s = (lane == 0) ? st : plan->side[lane - 1];
+    CLM_TRY_HIP(launch(plan->launch_order[k], s));
   }
   for (int i = 0; i < nside; ++i) {
     CLM_TRY_HIP(hipEventRecord(plan->join[i], plan->side[i]));
@@ -1225,9 +1228,31 @@ clm_status_t build_plan(int count, const clm_batch_entry *entries, const int *fo
       off += Layout::align(tape_bytes_of(e.dtype, e.n, ckpt));
     }
     plan->group_count[group_of(e.dtype, cfg[i])] += 1;
+    plan->group_cost[group_of(e.dtype, cfg[i])] +=
+        cost(i) * ((e.dtype == CLM_DTYPE_F64 || e.dtype == CLM_DTYPE_C128) ? 2.0 : 1.0);
   }
   for (int g = 1; g < kGroups; ++g)
     plan->group_start[g] = plan->group_start[g - 1] + plan->group_count[g - 1];
+  // Longest-processing-time lanes: the groups in decreasing cost, each on the
+  // stream whose groups so far cost least, so the heaviest groups start together
+  // at the launch instead of queueing behind each other on one stream.
+  {
+    int ng = 0;
+    for (int g = 0; g < kGroups; ++g)
+      if (plan->group_count[g] > 0) plan->launch_order[ng++] = g;
+    std::stable_sort(plan->launch_order, plan->launch_order + ng, [&](int a, int b) {
+      return plan->group_cost[a] > plan->group_cost[b];
+    });
+    double load[1 + clm_batch_plan::kSide] = {};
+    for (int k = 0; k < ng; ++k) {
+      int best = 0;
+      for (int l = 1; l < 1 + clm_batch_plan::kSide && l < ng; ++l)
+        if (load[l] < load[best]) best = l;
+      plan->launch_lane[k] = best;
+      load[best] += plan->group_cost[plan->launch_order[k]];
+    }
+    plan->num_launch = ng;
+  }
   for (int g = 0; g < kGroups; ++g)
     if (plan->group_count[g] > 0 && !prepare_group(g, dev)) {
       delete plan;

@@ -1,23 +1,16 @@
 // gemm_fp8.hip -- FP64-in / FP64-out matrix products on FP8 MFMA with a power-of-two
-// scale per 128 x 128 tile (the low-order-term products of the FP8-tail variant).
+// scale per 128 x 128 tile (the high-degree-term products of the FP8-tail variant).
 //
 // Quantization: every 128 x 128 tile of an FP64 matrix X gets the exponent
 // e = floor(log2(448 / max|tile|)) (448 is the largest OCP e4m3 value; a value
 // beyond it would convert to NaN), and q = e4m3(x 2^e) with round-to-nearest-
 // even. Left operands are stored row-major (k contiguous), right operands
-// column-major (k contiguous): the 16x16x32 MFMA wants 8 consecutive k of one
-// row / column in each lane.
+// column-major (k contiguous): the MFMA wants 32 consecutive k of one row /
+// column in each lane.
 //
-// Product: C(I, J) = sum over k-tiles K of [Aq(I, K) Bq(K, J)] 2^-(e_A(I, K) + e_B(K, J)).
-// A 128 x 128 tile of C per workgroup of 8 wave64 (2 x 4 waves of 64 x 32); each
-// LDS stage holds one 128-deep k-tile of both operands, so one slice has one
-// scale pair: its four v_mfma_f32_16x16x32_fp8_fp8 k-steps accumulate a partial
-// sum that is then added, scaled, to the running FP32 sum. The epilogue is the
-// FP64 path's (alpha, beta*Cin, the fused Paterson-Stockmeyer block).
-//
-// MFMA operand layout (16x16x32 fp8, measured on this part): lane l holds A row
-// l%16, k = 8(l/16) .. +7 as 8 bytes; B column l%16 likewise; D lane l holds
-// rows 4(l/16) + r of column l%16.
+// Product: C(I, J) = sum over k-tiles K of [Aq(I, K) Bq(K, J)] 2^-(e_A(I, K) + e_B(K, J)),
+// the scales applied by the MFMA itself (see dgemm_fp8_w), FP32 accumulation, and
+// the FP64 path's epilogue (alpha, beta*Cin, the fused Paterson-Stockmeyer block).
 #include "clm_internal.hpp"
 #include "gemm_tile.hpp"
 
@@ -26,41 +19,52 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 128;        // tile edge: quantization tile = product tile = k-slice
+constexpr int kT = 128;        // quantization tile edge = k-tile depth
 constexpr int kRowB = kT + 16;  // LDS row (bytes): 16-byte pad
-constexpr int kStage = 2 * kT * kRowB;
 
 __device__ __forceinline__ unsigned fp8_pair(float a, float b) {
   return static_cast<unsigned>(__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false)) & 0xffffu;
 }
 
-// one 128 x 128 tile per workgroup: the tile's exponent (into ec and / or er),
-// then its bytes in the column-major plane pc (ld rows) and / or the row-major
-// plane pr (ld cols)
-__global__ void __launch_bounds__(256)
+// One 128 x 128 tile per workgroup of 512 lanes, read from HBM once: lane t holds
+// rows 2 (t % 64) and 2 (t % 64) + 1 of the tile's columns t / 64 + 8 q, q < 16, in
+// registers; the tile's max (wave shuffles, then LDS) gives its exponent (into ec
+// and / or er); the pairs are converted to e4m3 two at a time and written as
+// 16-bit words down the columns of the column-major plane pc (ld rows), and
+// through an LDS byte tile along the rows of the row-major plane pr (ld cols).
+template <bool kFast>
+__global__ void __launch_bounds__(512)
     quantize_fp8_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
                         unsigned char *__restrict__ pc, unsigned char *__restrict__ pr,
                         int *__restrict__ ec, int *__restrict__ er) {
-  __shared__ double red[256];
-  __shared__ unsigned char tile[kT][kT + 4];
+  __shared__ double red[8];
+  __shared__ unsigned char tile[kT][kT + 4];  // [column][row]
   __shared__ int s_e;
   const int ti = blockIdx.x, tj = blockIdx.y;
   const int i0 = ti * kT, j0 = tj * kT;
-  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ii = 2 * lane;  // the lane's row pair within the tile
+  double2 v[16];
   double mx = 0.0;
-  for (int jj = ty; jj < kT; jj += 4)
-    for (int ii = tx; ii < kT; ii += 64) {
-      const int i = i0 + ii, j = j0 + jj;
-      if (i < rows && j < cols) mx = fmax(mx, fabs(X[i + (long)j * ldx]));
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int jj = wave + 8 * q;
+    const int i = i0 + ii, j = j0 + jj;
+    if (kFast) {
+      v[q] = *reinterpret_cast<const double2 *>(X + i + (long)j * ldx);
+    } else {
+      v[q].x = (i < rows && j < cols) ? X[i + (long)j * ldx] : 0.0;
+      v[q].y = (i + 1 < rows && j < cols) ? X[i + 1 + (long)j * ldx] : 0.0;
     }
-  red[tid] = mx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
-    __syncthreads();
+    mx = fmax(mx, fmax(fabs(v[q].x), fabs(v[q].y)));
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
   if (tid == 0) {
-    const double m = red[0];
+    double m = red[0];
+    for (int w = 1; w < 8; ++w) m = fmax(m, red[w]);
     int e = 0;
     if (m > 0.0 && isfinite(m)) {
       e = ilogb(448.0 / m);
@@ -72,140 +76,183 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const int e = s_e;
-  for (int jj = ty; jj < kT; jj += 4)
-    for (int ii = 2 * tx; ii < kT; ii += 128) {
-      const int i = i0 + ii, j = j0 + jj;
-      float a = 0.f, b = 0.f;
-      if (i < rows && j < cols) a = static_cast<float>(ldexp(X[i + (long)j * ldx], e));
-      if (i + 1 < rows && j < cols) b = static_cast<float>(ldexp(X[i + 1 + (long)j * ldx], e));
-      const unsigned q = fp8_pair(a, b);
-      if (pc && j < cols) {
-        if (i < rows) pc[i + (long)j * rows] = static_cast<unsigned char>(q & 0xff);
-        if (i + 1 < rows) pc[i + 1 + (long)j * rows] = static_cast<unsigned char>(q >> 8);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int jj = wave + 8 * q;
+    const int i = i0 + ii, j = j0 + jj;
+    const unsigned w = fp8_pair(static_cast<float>(ldexp(v[q].x, e)),
+                                static_cast<float>(ldexp(v[q].y, e)));
+    if (pc) {
+      if (kFast) {
+        *reinterpret_cast<unsigned short *>(pc + i + (long)j * rows) = static_cast<unsigned short>(w);
+      } else if (j < cols) {
+        if (i < rows) pc[i + (long)j * rows] = static_cast<unsigned char>(w & 0xff);
+        if (i + 1 < rows) pc[i + 1 + (long)j * rows] = static_cast<unsigned char>(w >> 8);
       }
-      tile[jj][ii] = static_cast<unsigned char>(q & 0xff);
-      tile[jj][ii + 1] = static_cast<unsigned char>(q >> 8);
     }
+    *reinterpret_cast<unsigned short *>(&tile[jj][ii]) = static_cast<unsigned short>(w);
+  }
   if (!pr) return;
   __syncthreads();
-  for (int ii = ty; ii < kT; ii += 4)
-    for (int jj = tx; jj < kT; jj += 64) {
-      const int i = i0 + ii, j = j0 + jj;
-      if (i < rows && j < cols) pr[(long)i * cols + j] = tile[jj][ii];
+  // rows of pr: lane t writes 4 consecutive columns of row t / 32 + 16 p
+  for (int idx = tid; idx < kT * kT / 4; idx += 512) {
+    const int ir = idx >> 5, jq = (idx & 31) * 4;
+    const int i = i0 + ir, j = j0 + jq;
+    if (kFast) {
+      const unsigned word = tile[jq][ir] | (tile[jq + 1][ir] << 8) | (tile[jq + 2][ir] << 16) |
+                            (static_cast<unsigned>(tile[jq + 3][ir]) << 24);
+      *reinterpret_cast<unsigned *>(pr + (long)i * cols + j) = word;
+    } else if (i < rows) {
+      for (int c = 0; c < 4; ++c)
+        if (j + c < cols) pr[(long)i * cols + j + c] = tile[jq + c][ir];
     }
+  }
 }
 
-// global -> registers: one 128 x 128 byte slice of each operand, 2 x 16 bytes per lane
+// ------------------------------------------------------------------------------------
+// The product kernel: a 256 x 128 tile of C per workgroup of 8 wave64 (4 x 2 waves of
+// 64 x 64), one 128-deep k-tile per LDS stage (A 256 x 128 and B 128 x 128 bytes, rows
+// padded to 144 bytes), double-buffered. Each wave issues one scaled
+// v_mfma_scale_f32_16x16x128_f8f6f4 per 16 x 16 sub-tile and k-tile: the operands'
+// tile exponents enter as the instruction's E8M0 block scales (2^(127 - e) each), so
+// every product lands in the FP32 accumulator already scaled. A wave's 64 rows lie in
+// one 128-row quantization tile of A and the workgroup's 128 columns in one of B, so
+// the scales are uniform over a wave.
+// Operand layout (measured on the part with an in-bounds probe): lane l holds row
+// (A) / column (B) l % 16, k = 32 (l / 16) + b in byte b of its 32 bytes; format code
+// 0 is e4m3; D rows 4 (l / 16) + r of column l % 16.
+// ------------------------------------------------------------------------------------
+typedef int i8v __attribute__((ext_vector_type(8)));
+// staging registers as a native vector: HIP's uint4 (a struct with a union) kept the
+// arrays of it from being split into registers, and the compiler moved them to LDS
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+constexpr int kBM = 256, kBN = 128;
+constexpr int kStageA = kBM * kRowB, kStageB = kBN * kRowB;
+constexpr int kStageW = kStageA + kStageB;  // bytes per stage
+
 template <bool kFast>
-__device__ __forceinline__ void fp8_gload(const GemmArgs<double> &g, const unsigned char *PA,
-                                          const unsigned char *PB, int m0, int n0, int k0, int tid,
-                                          uint4 (&ra)[2], uint4 (&rb)[2]) {
+__device__ __forceinline__ void fp8w_gload(const GemmArgs<double> &g, const unsigned char *PA,
+                                           const unsigned char *PB, int m0, int n0, int k0,
+                                           int tid, u4v (&ra)[4], u4v (&rb)[2]) {
   const long K = g.K;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int c = tid + it * 512;
+    const int r = c >> 3, kq = (c & 7) * 16;
+    if (kFast) {
+      ra[it] = *reinterpret_cast<const u4v *>(PA + (long)(m0 + r) * K + k0 + kq);
+    } else {
+      u4v v = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        if (k0 + kq + e < K && m0 + r < g.M)
+          v[e >> 2] |= static_cast<unsigned>(PA[(long)(m0 + r) * K + k0 + kq + e]) << (8 * (e & 3));
+      ra[it] = v;
+    }
+  }
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int c = tid + it * 512;
     const int r = c >> 3, kq = (c & 7) * 16;
     if (kFast) {
-      ra[it] = *reinterpret_cast<const uint4 *>(PA + (long)(m0 + r) * K + k0 + kq);
-      rb[it] = *reinterpret_cast<const uint4 *>(PB + (long)(n0 + r) * K + k0 + kq);
+      rb[it] = *reinterpret_cast<const u4v *>(PB + (long)(n0 + r) * K + k0 + kq);
     } else {
-      unsigned char va[16], vb[16];
+      u4v v = {0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const bool kin = k0 + kq + e < K;
-        va[e] = (kin && m0 + r < g.M) ? PA[(long)(m0 + r) * K + k0 + kq + e] : 0;
-        vb[e] = (kin && n0 + r < g.N) ? PB[(long)(n0 + r) * K + k0 + kq + e] : 0;
-      }
-      ra[it] = *reinterpret_cast<const uint4 *>(va);
-      rb[it] = *reinterpret_cast<const uint4 *>(vb);
+      for (int e = 0; e < 16; ++e)
+        if (k0 + kq + e < K && n0 + r < g.N)
+          v[e >> 2] |= static_cast<unsigned>(PB[(long)(n0 + r) * K + k0 + kq + e]) << (8 * (e & 3));
+      rb[it] = v;
     }
   }
 }
 
-__device__ __forceinline__ void fp8_sstore(unsigned char *S, int tid, const uint4 (&ra)[2],
-                                           const uint4 (&rb)[2]) {
+__device__ __forceinline__ void fp8w_sstore(unsigned char *S, int tid, const u4v (&ra)[4],
+                                            const u4v (&rb)[2]) {
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int c = tid + it * 512;
+    *reinterpret_cast<u4v *>(&S[(c >> 3) * kRowB + (c & 7) * 16]) = ra[it];
+  }
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int c = tid + it * 512;
-    const int r = c >> 3, kq = (c & 7) * 16;
-    *reinterpret_cast<uint4 *>(&S[r * kRowB + kq]) = ra[it];
-    *reinterpret_cast<uint4 *>(&S[kT * kRowB + r * kRowB + kq]) = rb[it];
+    *reinterpret_cast<u4v *>(&S[kStageA + (c >> 3) * kRowB + (c & 7) * 16]) = rb[it];
   }
 }
 
-// the 128-deep slice: four k-steps into a zeroed partial sum, then added to acc
-// with the slice's scale
-__device__ __forceinline__ void fp8_compute(const unsigned char *S, int lane, int wm0, int wn0,
-                                            float scale, f4 (&acc)[4][2]) {
-  f4 part[4][2];
+__device__ __forceinline__ i8v lds_i8v(const unsigned char *p) {
+  const u4v lo = *reinterpret_cast<const u4v *>(p);
+  const u4v hi = *reinterpret_cast<const u4v *>(p + 16);
+  i8v v;
+  v[0] = lo[0]; v[1] = lo[1]; v[2] = lo[2]; v[3] = lo[3];
+  v[4] = hi[0]; v[5] = hi[1]; v[6] = hi[2]; v[7] = hi[3];
+  return v;
+}
+
+// one 128-deep k-tile: 4 x 4 scaled MFMAs per wave
+__device__ __forceinline__ void fp8w_compute(const unsigned char *S, int lane, int wm0, int wn0,
+                                             int sa, int sb, f4 (&acc)[4][4]) {
+  const int koff = 32 * (lane >> 4);
+  i8v a[4], b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = lds_i8v(&S[(wm0 + i * 16 + (lane & 15)) * kRowB + koff]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    b[j] = lds_i8v(&S[kStageA + (wn0 + j * 16 + (lane & 15)) * kRowB + koff]);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) part[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
-    const int koff = ks * 32 + 8 * (lane >> 4);
-    long a[4], b[2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      a[i] = *reinterpret_cast<const long *>(&S[(wm0 + i * 16 + (lane & 15)) * kRowB + koff]);
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      b[j] = *reinterpret_cast<const long *>(
-          &S[kT * kRowB + (wn0 + j * 16 + (lane & 15)) * kRowB + koff]);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j], part[i][j], 0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = fmaf(part[i][j][r], scale, acc[i][j][r]);
+    for (int j = 0; j < 4; ++j)
+      acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0,
+                                                                   sa, 0, sb);
+}
+
+// E8M0 scale of a tile exponent e: 2^-e, clamped to the format's range
+__device__ __forceinline__ int e8m0_of(int e) {
+  const int v = 127 - e;
+  return v < 0 ? 0 : (v > 254 ? 254 : v);
 }
 
 template <bool kFast>
 __global__ void __launch_bounds__(512)
-    dgemm_fp8_tiles(const GemmArgs<double> g, const unsigned char *__restrict__ PA,
-                    const unsigned char *__restrict__ PB, const int *__restrict__ eA,
-                    const int *__restrict__ eB, const int tiles_m, const int tiles_n,
-                    const int tiles_k) {
-  __shared__ __attribute__((aligned(16))) unsigned char L[2 * kStage];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    dgemm_fp8_w(const GemmArgs<double> g, const unsigned char *__restrict__ PA,
+                const unsigned char *__restrict__ PB, const int *__restrict__ eA,
+                const int *__restrict__ eB, const int tiles_m, const int tiles_n,
+                const int qtiles_m, const int tiles_k) {
+  __shared__ __attribute__((aligned(16))) unsigned char L[2 * kStageW];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
-  const int m0 = tm * kT, n0 = tn * kT;
-  const int wm0 = (wave >> 2) * 64, wn0 = (wave & 3) * 32;
-  f4 acc[4][2];
+  const int m0 = tm * kBM, n0 = tn * kBN;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  // the quantization tiles this wave's rows and the workgroup's columns lie in
+  // (a row tile past the matrix -- rows that are all padding -- reads the last one)
+  const int qm = min((m0 + wm0) / 128, qtiles_m - 1);
+  f4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  // exponents: A tiles (tm, tk) of an tiles_m x tiles_k grid, B tiles (tk, tn)
-  auto scale_of = [&](int tk) {
-    return ldexpf(1.0f, -(eA[tm + tk * tiles_m] + eB[tk + tn * tiles_k]));
-  };
-  uint4 ra[2], rb[2];
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  u4v ra[4], rb[2];
   const int KT = tiles_k;
   if (KT > 0) {
-    fp8_gload<kFast>(g, PA, PB, m0, n0, 0, tid, ra, rb);
-    fp8_sstore(L, tid, ra, rb);
+    fp8w_gload<kFast>(g, PA, PB, m0, n0, 0, tid, ra, rb);
+    fp8w_sstore(L, tid, ra, rb);
     __syncthreads();
-    for (int kt = 0; kt < KT - 1; ++kt) {
+    for (int kt = 0; kt < KT; ++kt) {
       const int cur = kt & 1;
-      fp8_gload<kFast>(g, PA, PB, m0, n0, (kt + 1) * kT, tid, ra, rb);
-      fp8_compute(L + cur * kStage, lane, wm0, wn0, scale_of(kt), acc);
-      fp8_sstore(L + (cur ^ 1) * kStage, tid, ra, rb);
+      if (kt + 1 < KT) fp8w_gload<kFast>(g, PA, PB, m0, n0, (kt + 1) * kT, tid, ra, rb);
+      const int sa = e8m0_of(eA[qm + kt * qtiles_m]), sb = e8m0_of(eB[kt + tn * tiles_k]);
+      fp8w_compute(L + cur * kStageW, lane, wm0, wn0, sa, sb, acc);
+      if (kt + 1 < KT) fp8w_sstore(L + (cur ^ 1) * kStageW, tid, ra, rb);
       __syncthreads();
     }
-    fp8_compute(L + ((KT - 1) & 1) * kStage, lane, wm0, wn0, scale_of(KT - 1), acc);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
@@ -223,25 +270,34 @@ hipError_t launch_quantize_fp8(hipStream_t stream, int rows, int cols, const dou
                                unsigned char *pc, unsigned char *pr, int *ec, int *er) {
   if (rows <= 0 || cols <= 0) return hipSuccess;
   const dim3 grid((rows + kT - 1) / kT, (cols + kT - 1) / kT);
-  hipLaunchKernelGGL(quantize_fp8_kernel, grid, dim3(256), 0, stream, rows, cols, X, ldx, pc, pr,
-                     ec, er);
+  const bool fast = rows % kT == 0 && cols % kT == 0 && ldx % 2 == 0 &&
+                    (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(pc) & 1) == 0 &&
+                    (reinterpret_cast<uintptr_t>(pr) & 3) == 0;
+  if (fast)
+    hipLaunchKernelGGL(quantize_fp8_kernel<true>, grid, dim3(512), 0, stream, rows, cols, X, ldx,
+                       pc, pr, ec, er);
+  else
+    hipLaunchKernelGGL(quantize_fp8_kernel<false>, grid, dim3(512), 0, stream, rows, cols, X, ldx,
+                       pc, pr, ec, er);
   return hipGetLastError();
 }
 
 hipError_t launch_gemm_fp8(hipStream_t stream, const GemmArgs<double> &g, const unsigned char *PA,
                            const unsigned char *PB, const int *eA, const int *eB) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  const int tiles_m = (g.M + kT - 1) / kT, tiles_n = (g.N + kT - 1) / kT;
+  const int tiles_m = (g.M + kBM - 1) / kBM, tiles_n = (g.N + kBN - 1) / kBN;
+  const int qtiles_m = (g.M + kT - 1) / kT;
   const int tiles_k = (g.K + kT - 1) / kT;
-  const bool fast = g.M % kT == 0 && g.N % kT == 0 && g.K % kT == 0 && aligned16_8(PA) &&
+  const bool fast = g.M % kBM == 0 && g.N % kBN == 0 && g.K % kT == 0 && aligned16_8(PA) &&
                     aligned16_8(PB);
   const dim3 grid(tiles_m * tiles_n), block(512);
   if (fast)
-    hipLaunchKernelGGL(dgemm_fp8_tiles<true>, grid, block, 0, stream, g, PA, PB, eA, eB, tiles_m,
-                       tiles_n, tiles_k);
+    hipLaunchKernelGGL(dgemm_fp8_w<true>, grid, block, 0, stream, g, PA, PB, eA, eB, tiles_m,
+                       tiles_n, qtiles_m, tiles_k);
   else
-    hipLaunchKernelGGL(dgemm_fp8_tiles<false>, grid, block, 0, stream, g, PA, PB, eA, eB, tiles_m,
-                       tiles_n, tiles_k);
+    hipLaunchKernelGGL(dgemm_fp8_w<false>, grid, block, 0, stream, g, PA, PB, eA, eB, tiles_m,
+                       tiles_n, qtiles_m, tiles_k);
   return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ import torch
 
 import calaman_g_amd as cg
 import oracle
+from calaman_g_amd import _lib
 from calaman_g_amd.expm import BatchPlan, expm_batched_autograd
 
 pytestmark = pytest.mark.gpu
@@ -178,7 +179,7 @@ def test_autograd_unitary_generator_batch(dev):
 def test_adjoint_every_tile_gives_the_same_bits(dev, dtype):
     """every backward product is the NN workgroup product after an exact
     conjugate transpose: the tile decides which lanes compute an element, not its
-    arithmetic, so all five tiles give the same gradient bits"""
+    arithmetic, so every tile gives the same gradient bits"""
     rng = np.random.default_rng(8)
     specs = [(7, 0.5), (45, 6.0), (130, 12.0)]
     mats = [_mat(rng, n, dtype, t) for n, t in specs]
@@ -186,7 +187,7 @@ def test_adjoint_every_tile_gives_the_same_bits(dev, dtype):
     saved = cg.batched_tuning(dtype, adjoint=True)
     try:
         ref = None
-        for cfg in range(5):
+        for cfg in range(_lib.BATCH_TILES):
             cg.set_batched_tuning(dtype, [cfg] * 8, adjoint=True)
             _, ds, _ = _run(dev, mats, grads)
             if ref is None:
@@ -202,7 +203,7 @@ def test_adjoint_autotune_small(dev):
     fwd = cg.batched_tuning(torch.complex64)
     try:
         table, ms = cg.batched_autotune(torch.complex64, nmax=40, adjoint=True)
-        assert all(0 <= c < 5 for c in table)
+        assert all(0 <= c < _lib.BATCH_TILES for c in table)
         assert cg.batched_tuning(torch.complex64, adjoint=True) == table
         assert cg.batched_tuning(torch.complex64) == fwd  # the forward table is untouched
         for b in (0, 1):

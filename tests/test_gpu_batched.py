@@ -99,7 +99,7 @@ def test_batched_mixed_types_bitwise_equals_single(dev):
     _check_batch(entries, inputs, BatchPlan.decode_results(res), dev)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("cfg", list(range(_lib.BATCH_TILES)))
 def test_batched_every_tile_gives_the_same_bits(dev, cfg):
     """the tile shape changes which lanes compute an element, not its arithmetic"""
     rng = np.random.default_rng(10 + cfg)

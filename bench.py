@@ -149,7 +149,10 @@ def gemm_roofline(dev, n=N, reps=20):
     a = torch.randn(n, n, dtype=torch.float64, device=dev)
     b = torch.randn(n, n, dtype=torch.float64, device=dev)
     c = torch.empty_like(a)
-    for _ in range(3):
+    # the GPU raises its clocks over the first ~25 ms of sustained MFMA load
+    # (measured: 307 -> 261 us per launch over the first 100 launches, flat after),
+    # so the kernel is timed after 100 untimed launches
+    for _ in range(100):
         gemm_cm(ctx, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -432,7 +435,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the copy + GEMM launches profiled for HBM traffic")

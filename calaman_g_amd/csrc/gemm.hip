@@ -73,10 +73,15 @@ __global__ void __launch_bounds__(NT)
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
       rgemm_gload<S, BM, BN, NT, BK, kFast>(g, m0, n0, (kt + 1) * BK, tid, ra, rb);
-      rgemm_compute<S, BM, BN, WM, WN, NT, BK>(As + cur * TL::A_STAGE, Bs + cur * TL::B_STAGE,
-                                               lane, wm0, wn0, acc);
+      // the next slice goes into the other stage half-way through this one, so its
+      // LDS writes overlap the other waves' MFMAs instead of idling the CU before
+      // the barrier (the k-steps still run in ascending order)
+      rgemm_compute<S, BM, BN, WM, WN, NT, BK, 0, BK / 2>(As + cur * TL::A_STAGE,
+                                                         Bs + cur * TL::B_STAGE, lane, wm0, wn0, acc);
       rgemm_sstore<S, BM, BN, NT, BK>(As + (cur ^ 1) * TL::A_STAGE, Bs + (cur ^ 1) * TL::B_STAGE,
                                       tid, ra, rb);
+      rgemm_compute<S, BM, BN, WM, WN, NT, BK, BK / 2, BK>(As + cur * TL::A_STAGE,
+                                                          Bs + cur * TL::B_STAGE, lane, wm0, wn0, acc);
       __syncthreads();
     }
     const int last = (KT - 1) & 1;

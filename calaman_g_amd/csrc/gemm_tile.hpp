@@ -154,13 +154,14 @@ __device__ __forceinline__ void rgemm_sstore(
   }
 }
 
-template <typename S, int BM, int BN, int WM, int WN, int NT, int BK>
+// the k-steps [K0, K1) of a slice (default: all of it), in ascending order
+template <typename S, int BM, int BN, int WM, int WN, int NT, int BK, int K0 = 0, int K1 = BK>
 __device__ __forceinline__ void rgemm_compute(const S *As, const S *Bs, int lane, int wm0, int wn0,
                                               typename Mfma<S>::acc (&acc)[WM / 16][WN / 16]) {
   using TL = RTile<S, BM, BN, NT, BK>;
   constexpr int TM = WM / 16, TN = WN / 16;
 #pragma unroll
-  for (int kk = 0; kk < BK; kk += 4) {
+  for (int kk = K0; kk < K1; kk += 4) {
     const int kr = kk + (lane >> 4);
     S a[TM], b[TN];
 #pragma unroll

@@ -101,7 +101,7 @@ typedef struct clm_expm_plan {
   int ps_block;  /* Paterson-Stockmeyer block size (0 for Pade) */
   int dtype;     /* CLM_DTYPE_* (0 when only predicted from a norm) */
   int ps_split;  /* polynomial products: 0 = FP64 MFMA, 3 / 6 = BF16 split (clm_expm_split_d) */
-  int split_squarings; /* how many of the last squarings ran BF16-split (6 products) */
+  int split_squarings; /* how many of the last squarings ran split (FP16X3 planes) */
   double norm1;  /* the 1-norm the plan was chosen from */
   int fp8_gemms;      /* polynomial products that ran on FP8 (clm_expm_fp8tail_d) */
   int fp8_min_degree; /* the lowest Taylor degree an FP8 product may carry (0 otherwise) */
@@ -168,7 +168,7 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
  * The 1-norm, the plan and the 2^-s scaling stay FP64; the degree comes from the
  * Taylor thresholds at u = 2^-24; the squarings run FP64 except the last
  * `split_squarings`, whose errors the remaining squarings amplify least (none
- * when s > 8). Target:
+ * when s > 8), in IEEE-half planes. Target:
  * relative error <= 1e-6 against FP64. */
 #define CLM_SPLIT_BF16X3 3
 #define CLM_SPLIT_BF16X6 6
@@ -177,8 +177,11 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
  * bits per product, between X3 and X6 at the cost of X3) */
 #define CLM_SPLIT_FP16X3 13
 typedef struct clm_split_opts {
-  int ps_split;        /* 0 = auto (X3 when s <= 3, else X6), or CLM_SPLIT_BF16X3 / X6 */
-  int split_squarings; /* -1 = auto (1 when 1 <= s <= 8, else 0), else 0..s */
+  int ps_split;        /* 0 = auto (for s <= 5: BF16X3 while ||A/2^s||_1 <= 1, else FP16X3;
+                          BF16X6 above), or CLM_SPLIT_BF16X3 / BF16X6 / FP16X3 */
+  int split_squarings; /* -1 = auto: 0 when s = 0 or s > 8; 2 with an FP16X3 polynomial
+                          and s = 4, 5; otherwise 1. Else 0..s. Split squarings use
+                          FP16X3 planes whatever the polynomial's format */
 } clm_split_opts;
 /* the plan clm_expm_split_d executes for a 1-norm under opts (NULL = auto) */
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts);

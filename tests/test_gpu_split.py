@@ -18,7 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 def rel(x, y):
-    return np.linalg.norm(x - y) / np.linalg.norm(y)
+    """relative Frobenius error, scaled first (exp of a large norm reaches 1e159,
+    whose squares overflow FP64)"""
+    scale = np.abs(y).max()
+    return np.linalg.norm((x - y) / scale) / np.linalg.norm(y / scale)
 
 
 def cm(a, dev):
@@ -72,8 +75,18 @@ def _run_split(a, dev, ps_split=0, sq=-1):
     return de.cpu().numpy().T, plan
 
 
-@pytest.mark.parametrize("n", [64, 256, 1024])
-@pytest.mark.parametrize("target", [0.5, 4.0, 8.0, 100.0])
+# Sizes and norms where 1e-6 is attainable in single-precision-level arithmetic:
+# the split products round their inputs at ~2^-24 relative, and exp's relative
+# condition grows with ||A||, so for small matrices of large norm the error is set
+# by that rounding, not by the schedule (measured: n = 16 at ||A||_1 = 1000
+# 1.5e-6, n = 256 at 3000 1.7e-6; n = 2048 at 3000 5e-7).
+_SPLIT_CASES = [(n, t) for n in (64, 256, 1024)
+                for t in (0.5, 3.0, 8.0, 12.0, 40.0, 100.0, 400.0, 1000.0)] + \
+               [(1024, 3000.0)] + \
+               [(n, t) for n in (16, 32) for t in (0.5, 3.0, 8.0, 12.0, 40.0, 100.0)]
+
+
+@pytest.mark.parametrize("n,target", _SPLIT_CASES)
 def test_expm_split_meets_1e6(dev, n, target):
     rng = np.random.default_rng(n + int(target))
     a = rng.standard_normal((n, n))
@@ -81,9 +94,13 @@ def test_expm_split_meets_1e6(dev, n, target):
     got, plan = _run_split(a, dev)
     err = rel(got, oracle.expm(a))
     assert err <= 1e-6, (plan, err)
-    assert plan["ps_split"] in (3, 6)
-    # the last squaring splits; with s > 8 every squaring stays FP64
-    assert plan["split_squarings"] == (1 if 1 <= plan["s"] <= 8 else 0)
+    s = plan["s"]
+    small = plan["norm1"] / 2.0 ** s <= 1.0
+    assert plan["ps_split"] == ((3 if small else 13) if s <= 5 else 6)
+    # the last squarings split (2 for s = 4, 5, else 1); with s > 8 every squaring
+    # stays FP64
+    want_t = 0 if (s == 0 or s > 8) else (2 if (s in (4, 5) and not small) else 1)
+    assert plan["split_squarings"] == want_t
     # the plan is the one the host predicts
     assert plan == {**cg.expm_split_plan(plan["norm1"]), "dtype": 1}
 
@@ -117,3 +134,19 @@ def test_expm_split_rejects_bad_options(dev):
     work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
     st, _ = expm_split_cm(n, da, n, torch.empty_like(da), n, work, None, ps_split=4)
     assert st == _lib.INVALID_VALUE
+
+
+def test_expm_split_large_result_stays_finite(dev):
+    """a power whose entries pass the FP32 range: split squarings scale rows and
+    columns by powers of two first, so nothing overflows (BF16 planes of such a
+    power -- FP32 first -- would be infinite)"""
+    n = 32
+    rng = np.random.default_rng(77)
+    a = rng.standard_normal((n, n))
+    a *= 500.0 / np.abs(a).sum(0).max()
+    ref = oracle.expm(a)
+    assert np.abs(ref).max() > 1e39  # beyond FP32
+    got, plan = _run_split(a, dev)
+    assert 1 <= plan["s"] <= 8 and plan["split_squarings"] >= 1
+    assert np.all(np.isfinite(got))
+    assert rel(got, ref) <= 1e-6

@@ -371,12 +371,16 @@ __device__ real_t<T> wg_norm1(int n, const T *A, long lda, char *lds) {
 }
 
 // Element-wise passes over an n x n matrix: the workgroup walks the elements in
-// column-major order, each lane taking kEw of them per round (consecutive lanes
+// column-major order, each lane taking EW of them per round (consecutive lanes
 // on consecutive rows), and every round issues all of its loads before any of
-// its stores -- a lane has kEw loads in flight instead of one, which is what a
+// its stores -- a lane has EW loads in flight instead of one, which is what a
 // single workgroup's pass over a matrix is bound by (load latency, not
 // bandwidth). Each element's arithmetic is unchanged.
 constexpr int kEw = 8;
+// the round for a configuration: kEw for 4-wave workgroups; 2 for the 16-wave tile,
+// which has four times the lanes (the same loads in flight per CU) and a quarter
+// of the registers per lane (8 would spill)
+constexpr int ew_of(int cfg) { return threads_of(cfg) >= 1024 ? 2 : kEw; }
 
 // (row, column) of linear element e of an n x n matrix
 __device__ __forceinline__ void ew_rc(int e, int n, int &i, int &j) {
@@ -385,20 +389,20 @@ __device__ __forceinline__ void ew_rc(int e, int n, int &i, int &j) {
 }
 
 // out = f * in (f a power of two: exact)
-template <typename T>
+template <int EW = kEw, typename T>
 __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, long ldo) {
   const int total = n * n, nt = blockDim.x;
-  for (int base = threadIdx.x; base < total; base += nt * kEw) {
-    T v[kEw];
+  for (int base = threadIdx.x; base < total; base += nt * EW) {
+    T v[EW];
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
+    for (int u = 0; u < EW; ++u) {
       const int e = base + u * nt;
       int i, j;
       ew_rc(e, n, i, j);
       if (e < total) v[u] = in[i + static_cast<long>(j) * ldi];
     }
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
+    for (int u = 0; u < EW; ++u) {
       const int e = base + u * nt;
       int i, j;
       ew_rc(e, n, i, j);
@@ -410,14 +414,14 @@ __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, lo
 
 // B = c0 I + c1 X1 + sum_t c_t bank[t-2], the per-element order of the
 // single-matrix block build
-template <typename T>
+template <int EW = kEw, typename T>
 __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms, const T *X1,
                                long ldx1, const T *bank, long stride, long ldbank) {
   const int total = n * n, nt = blockDim.x;
-  for (int base = threadIdx.x; base < total; base += nt * kEw) {
-    T acc[kEw];
+  for (int base = threadIdx.x; base < total; base += nt * EW) {
+    T acc[EW];
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
+    for (int u = 0; u < EW; ++u) {
       const int e = base + u * nt;
       int i, j;
       ew_rc(e, n, i, j);
@@ -428,7 +432,7 @@ __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms,
     for (int t = 2; t < nterms; ++t) {
       const T *bt = bank + static_cast<long>(t - 2) * stride;
 #pragma unroll
-      for (int u = 0; u < kEw; ++u) {
+      for (int u = 0; u < EW; ++u) {
         const int e = base + u * nt;
         int i, j;
         ew_rc(e, n, i, j);
@@ -436,7 +440,7 @@ __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms,
       }
     }
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
+    for (int u = 0; u < EW; ++u) {
       const int e = base + u * nt;
       int i, j;
       ew_rc(e, n, i, j);
@@ -447,9 +451,9 @@ __device__ void wg_build_block(int n, T *B, long ldb, const T *coef, int nterms,
 }
 
 // dst = src
-template <typename T>
+template <int EW = kEw, typename T>
 __device__ void wg_copy(int n, const T *src, long lds_, T *dst, long ldd) {
-  wg_scale_copy(n, 1.0, src, lds_, dst, ldd);
+  wg_scale_copy<EW>(n, 1.0, src, lds_, dst, ldd);
 }
 
 // dst = f src^H (n x n, f a power of two) through 32 x 32 LDS tiles (row 33), so
@@ -588,7 +592,7 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   const T *src = A;
   long ld_src = d.lda;
   if (nsq > 0) {
-    wg_scale_copy(n, ldexp(1.0, -nsq), A, d.lda, As, n);
+    wg_scale_copy<ew_of(CFG)>(n, ldexp(1.0, -nsq), A, d.lda, As, n);
     src = As;
     ld_src = n;
   }
@@ -611,10 +615,11 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
       // with a tape, the Horner iterates P_r .. P_1 stay in their own blocks
       T *blk = (ad && nhorner >= 1) ? tape.horner(nhorner) : (cur == 0 ? target : other);
       const long ldb = (ad && nhorner >= 1) ? long(n) : (cur == 0 ? ld_target : ld_other);
-      wg_build_block(n, blk, ldb, coef + static_cast<long>(nhorner) * bs, terms_r, src, ld_src,
-                     bank, stride, static_cast<long>(n));
+      wg_build_block<ew_of(CFG)>(n, blk, ldb, coef + static_cast<long>(nhorner) * bs, terms_r,
+                                 src, ld_src, bank, stride, static_cast<long>(n));
     }
-    if (ad && nsq > 0 && q == nbank + nhorner) wg_copy(n, res, ld_res, tape.checkpoint(0), long(n));
+    if (ad && nsq > 0 && q == nbank + nhorner)
+      wg_copy<ew_of(CFG)>(n, res, ld_res, tape.checkpoint(0), long(n));
     if (q == total) break;
     GemmArgs<T> g;
     if (q < nbank) {
@@ -654,7 +659,8 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
     wg_gemm<T, CFG>(g, lds);
     if (ad && q >= nbank + nhorner) {
       const int k1 = q - (nbank + nhorner) + 1;  // this product made R_k1
-      if (k1 < nsq && k1 % cstep == 0) wg_copy(n, res, ld_res, tape.checkpoint(k1 / cstep), long(n));
+      if (k1 < nsq && k1 % cstep == 0)
+        wg_copy<ew_of(CFG)>(n, res, ld_res, tape.checkpoint(k1 / cstep), long(n));
     }
   }
 }
@@ -675,27 +681,27 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
 // The cotangents are kept conjugate-transposed (see adjoint_one): for the block
 // terms t = 1 .. terms-1, dst_t^H += c_t bar^H, i.e. dst_t += c_t * bt with bt
 // the transposed cotangent held (t = 1 into Xb, t >= 2 into the bank cotangents)
-template <typename T>
+template <int EW = kEw, typename T>
 __device__ void wg_block_adjoint(int n, const T *bt, const T *c, int terms, T *Xb,
                                  const Tape<T> &tape) {
   const int total = n * n, nt = blockDim.x;
-  for (int base = threadIdx.x; base < total; base += nt * kEw) {
-    T b[kEw];
+  for (int base = threadIdx.x; base < total; base += nt * EW) {
+    T b[EW];
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
+    for (int u = 0; u < EW; ++u) {
       const int e = base + u * nt;
       if (e < total) b[u] = bt[e];
     }
     for (int t = 1; t < terms; ++t) {
       T *y = (t == 1) ? Xb : tape.bank_bar(t - 2);
-      T v[kEw];
+      T v[EW];
 #pragma unroll
-      for (int u = 0; u < kEw; ++u) {
+      for (int u = 0; u < EW; ++u) {
         const int e = base + u * nt;
         if (e < total) v[u] = y[e];
       }
 #pragma unroll
-      for (int u = 0; u < kEw; ++u) {
+      for (int u = 0; u < EW; ++u) {
         const int e = base + u * nt;
         if (e < total) y[e] = ops<T>::mad(c[t], b[u], v[u]);
       }
@@ -809,7 +815,8 @@ __device__ void adjoint_one(const BatchDesc &d, char *work, const T *coef, char 
         sub = 0;
       }
       if (sub == 0) {
-        wg_block_adjoint(n, bar, coef + static_cast<long>(j) * bs, min(degree - j * bs + 1, bs),
+        wg_block_adjoint<ew_of(CFG)>(n, bar, coef + static_cast<long>(j) * bs,
+                                     min(degree - j * bs + 1, bs),
                          Xb, tape);
         if (j == nhorner) {
           phase = 2;

@@ -12,19 +12,24 @@
 // FP32, so the error is the FP32 accumulation plus the dropped terms: about
 // 2^-16 per product for SPLIT = 3 and the FP32 level (~2^-24 sqrt(k)) for 6.
 //
-// Operands are read from HBM as FP64 and split in registers on the way into
-// LDS, so no split planes ever travel through HBM. The epilogue is the FP64
-// path's (alpha, beta*Cin, fused Paterson-Stockmeyer block), in FP64.
+// Two ways in. The expm path splits each operand once into k-contiguous planes
+// in HBM (split_planes_kernel: BF16; max_exponent_kernel + split_planes_half_kernel:
+// IEEE half with per-row / per-column power-of-two scaling) and multiplies the
+// planes (dgemm_split_planes_kg for the 3-product formats, dgemm_split_planes2 for
+// the 6-product one). The stand-alone product (clm_gemm_split_d, dgemm_split_bf16)
+// reads FP64 and splits in registers on the way into LDS. Every epilogue is the
+// FP64 path's (alpha, beta*Cin, fused Paterson-Stockmeyer block), in FP64.
 //
-// Layout: a 128 x 128 tile of C per workgroup of 4 wave64, each wave 64 x 64 as
-// 4 x 4 MFMA tiles of 16 x 16; K advances in slices of 32 (one MFMA k-step).
-// LDS holds every plane k-contiguous -- As[m][k], Bs[n][k], rows of 40 bf16 (80 B)
-// -- because the MFMA wants 8 consecutive k of one row in a lane; the 80-byte row
-// makes the 16 lanes of a quarter-wave hit 16 disjoint 4-bank groups. A arrives
-// m-contiguous from column-major memory and is transposed on the LDS write.
+// dgemm_split_bf16 layout: a 128 x 128 tile of C per workgroup of 4 wave64, each
+// wave 64 x 64 as 4 x 4 MFMA tiles of 16 x 16; K advances in slices of 32 (one
+// MFMA k-step). LDS holds every plane k-contiguous -- As[m][k], Bs[n][k], rows of
+// 40 bf16 (80 B) -- because the MFMA wants 8 consecutive k of one row in a lane;
+// the 80-byte row makes the 16 lanes of a quarter-wave hit 16 disjoint 4-bank
+// groups. A arrives m-contiguous from column-major memory and is transposed on
+// the LDS write.
 //
-// MFMA operand layout (16x16x32 bf16): A lane l holds A[l%16][8(l/16) .. 8(l/16)+7],
-// B lane l holds B[8(l/16) .. +7][l%16], D lane l holds rows CLM_BF16_D_ROW(l, r).
+// MFMA operand layout (16x16x32 bf16 / f16): A lane l holds A[l%16][8(l/16) .. +7],
+// B lane l holds B[8(l/16) .. +7][l%16], D lane l holds rows 4 (l/16) + r.
 #include "clm_internal.hpp"
 
 namespace clm {

@@ -267,47 +267,175 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------------------------
-// Split once, multiply many times. split_planes writes the bfloat16 planes of an
-// FP64 matrix X (rows x cols, column-major, ldx) in the layouts the MFMA operands
-// want -- k contiguous:
+// Split once, multiply many times. The plane kernels write the 16-bit planes of
+// an FP64 matrix X (rows x cols, column-major, ldx) in the layouts the MFMA
+// operands want -- k contiguous:
 //   pc: column-major planes, pc[t*pstride + i + j*rows]  (X as the right operand)
 //   pr: row-major planes,    pr[t*pstride + i*cols + j]  (X as the left operand)
 // The row-major write goes through an LDS transpose so both the FP64 read and the
 // plane writes are coalesced. dgemm_split_planes then stages the planes straight
 // into LDS with 16-byte loads: its loop is loads, LDS and MFMA only.
+//
+// Tiling of the element-wise kernels: a 64 x 64 tile per 256-lane workgroup;
+// lane t holds rows 4 (t % 16) .. +3 of the columns t / 16 + 16 q (q = 0..3), all
+// 16 values loaded before any is used (two 16-byte reads per column when the rows
+// allow), so a lane keeps its loads in flight together -- these passes are HBM
+// bound, and a load-use-load chain per element would leave them latency bound.
 // ------------------------------------------------------------------------------------
+struct Tile64 {
+  int i0, j0, r0, c0;  // the tile's origin; the lane's first row and first column
+  bool full;           // the whole tile lies inside X, 16-byte aligned rows
+  __device__ Tile64(int rows, int cols, const double *X, long ldx)
+      : i0(blockIdx.x * 64), j0(blockIdx.y * 64), r0(i0 + 4 * (threadIdx.x & 15)),
+        c0(j0 + (threadIdx.x >> 4)),
+        full(i0 + 64 <= rows && j0 + 64 <= cols && (ldx & 1) == 0 &&
+             (reinterpret_cast<uintptr_t>(X) & 15) == 0) {}
+  __device__ int col(int q) const { return c0 + 16 * q; }
+};
+
+// The lane's 16 values, straight-line (edge tiles clamp the address and zero the
+// value instead of branching, so no load waits on another).
+template <bool kFull>
+__device__ __forceinline__ void tile64_load(const Tile64 &tl, int rows, int cols,
+                                            const double *__restrict__ X, long ldx,
+                                            double (&v)[4][4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if constexpr (kFull) {
+      const double *p = X + tl.r0 + static_cast<long>(tl.col(q)) * ldx;
+      const double2 a = *reinterpret_cast<const double2 *>(p);
+      const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+      v[q][0] = a.x;
+      v[q][1] = a.y;
+      v[q][2] = b.x;
+      v[q][3] = b.y;
+    } else {
+      const int j = min(tl.col(q), cols - 1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double x = X[min(tl.r0 + r, rows - 1) + static_cast<long>(j) * ldx];
+        v[q][r] = (tl.col(q) < cols && tl.r0 + r < rows) ? x : 0.0;
+      }
+    }
+  }
+}
+
+// whether a tile's column-major plane words go out as 8-byte stores
+__device__ __forceinline__ bool col_vec_ok(int rows, long pstride, const unsigned short *pc) {
+  return (rows & 3) == 0 && (pstride & 3) == 0 && (reinterpret_cast<uintptr_t>(pc) & 7) == 0;
+}
+
+// the four 16-bit words w[0..3] (rows r0 .. r0+3 of one column) into a column-major
+// plane: one 8-byte store, or one store per row inside the matrix
+__device__ __forceinline__ void store_col4(unsigned short *__restrict__ dst, bool vec, int nrows,
+                                           const unsigned short (&w)[4]) {
+  if (vec) {
+    *reinterpret_cast<uint2 *>(dst) = make_uint2(w[0] | (static_cast<unsigned>(w[1]) << 16),
+                                                 w[2] | (static_cast<unsigned>(w[3]) << 16));
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < nrows) dst[r] = w[r];
+  }
+}
+
+// Row-major planes through LDS: every lane deposits its 16 values' plane words at
+// [row][col] of the tile (planes 0, 1 packed in one 32-bit word, plane 2 in a
+// second), then lane t writes row t / 4, columns 16 (t % 4) .. +15 of each plane:
+// two 16-byte stores per plane when the rows are 16-byte aligned.
+template <int NPL>
+struct RowStage {
+  static constexpr int W = (NPL + 1) / 2;  // 32-bit words per element
+  unsigned int w[W][64][65];
+};
+
+template <int NPL>
+__device__ __forceinline__ void row_stage_put(RowStage<NPL> &st, int rr, int cc,
+                                              const unsigned short (&p)[NPL]) {
+  unsigned int w0 = p[0];
+  if constexpr (NPL > 1) w0 |= static_cast<unsigned>(p[1]) << 16;
+  st.w[0][rr][cc] = w0;
+  if constexpr (NPL > 2) st.w[1][rr][cc] = p[2];
+}
+
+template <int NPL>
+__device__ __forceinline__ void row_stage_write(const RowStage<NPL> &st, const Tile64 &tl, int rows,
+                                                int cols, unsigned short *__restrict__ pr,
+                                                long pstride) {
+  const int t = threadIdx.x, rr = t >> 2, cc = 16 * (t & 3);
+  const int i = tl.i0 + rr, j = tl.j0 + cc;
+  if (i >= rows || j >= cols) return;
+  const bool vec = tl.full && (cols & 7) == 0 && (pstride & 7) == 0 &&
+                   (reinterpret_cast<uintptr_t>(pr) & 15) == 0;
+#pragma unroll
+  for (int t2 = 0; t2 < NPL; ++t2) {
+    unsigned short h[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const unsigned word = st.w[t2 / 2][rr][cc + c];
+      h[c] = static_cast<unsigned short>((t2 & 1) ? word >> 16 : word);
+    }
+    unsigned short *dst = pr + t2 * pstride + static_cast<long>(i) * cols + j;
+    if (vec) {
+      uint4 lo, hi;
+      lo.x = h[0] | (static_cast<unsigned>(h[1]) << 16);
+      lo.y = h[2] | (static_cast<unsigned>(h[3]) << 16);
+      lo.z = h[4] | (static_cast<unsigned>(h[5]) << 16);
+      lo.w = h[6] | (static_cast<unsigned>(h[7]) << 16);
+      hi.x = h[8] | (static_cast<unsigned>(h[9]) << 16);
+      hi.y = h[10] | (static_cast<unsigned>(h[11]) << 16);
+      hi.z = h[12] | (static_cast<unsigned>(h[13]) << 16);
+      hi.w = h[14] | (static_cast<unsigned>(h[15]) << 16);
+      reinterpret_cast<uint4 *>(dst)[0] = lo;
+      reinterpret_cast<uint4 *>(dst)[1] = hi;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (j + c < cols) dst[c] = h[c];
+    }
+  }
+}
+
+template <int NPL, bool kFull>
+__device__ __forceinline__ void split_planes_body(const Tile64 &tl, int rows, int cols,
+                                                  const double *__restrict__ X, long ldx,
+                                                  unsigned short *__restrict__ pc,
+                                                  unsigned short *__restrict__ pr, long pstride,
+                                                  RowStage<NPL> &st) {
+  double v[4][4];
+  tile64_load<kFull>(tl, rows, cols, X, ldx, v);
+  const bool cvec = kFull && col_vec_ok(rows, pstride, pc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    unsigned short p[4][NPL];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      split_bf16<NPL>(v[q][r], p[r]);
+      if (pr) row_stage_put<NPL>(st, 4 * (threadIdx.x & 15) + r, (threadIdx.x >> 4) + 16 * q, p[r]);
+    }
+    const int j = tl.col(q);
+    if (pc && j < cols) {
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        const unsigned short w[4] = {p[0][t], p[1][t], p[2][t], p[3][t]};
+        store_col4(pc + t * pstride + tl.r0 + static_cast<long>(j) * rows, cvec, rows - tl.r0, w);
+      }
+    }
+  }
+  if (!pr) return;
+  __syncthreads();
+  row_stage_write<NPL>(st, tl, rows, cols, pr, pstride);
+}
+
 template <int NPL>
 __global__ void __launch_bounds__(256)
     split_planes_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
                         unsigned short *__restrict__ pc, unsigned short *__restrict__ pr,
                         long pstride) {
-  __shared__ unsigned short tile[NPL][64][65];
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int jj = ty; jj < 64; jj += 4) {
-    const int i = i0 + tx, j = j0 + jj;
-    unsigned short p[NPL];
-#pragma unroll
-    for (int t = 0; t < NPL; ++t) p[t] = 0;
-    if (i < rows && j < cols) {
-      split_bf16<NPL>(X[i + (long)j * ldx], p);
-      if (pc) {
-#pragma unroll
-        for (int t = 0; t < NPL; ++t) pc[t * pstride + i + (long)j * rows] = p[t];
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NPL; ++t) tile[t][jj][tx] = p[t];
-  }
-  if (!pr) return;
-  __syncthreads();
-  for (int ii = ty; ii < 64; ii += 4) {
-    const int i = i0 + ii, j = j0 + tx;
-    if (i < rows && j < cols) {
-#pragma unroll
-      for (int t = 0; t < NPL; ++t) pr[t * pstride + (long)i * cols + j] = tile[t][tx][ii];
-    }
-  }
+  __shared__ RowStage<NPL> st;
+  const Tile64 tl(rows, cols, X, ldx);
+  if (tl.full) split_planes_body<NPL, true>(tl, rows, cols, X, ldx, pc, pr, pstride, st);
+  else split_planes_body<NPL, false>(tl, rows, cols, X, ldx, pc, pr, pstride, st);
 }
 
 // ---- IEEE-half planes with power-of-two scales -----------------------------------------
@@ -322,35 +450,75 @@ __global__ void __launch_bounds__(256)
 // 2^(e_row + e_col), exactly.
 constexpr int kNoExponent = -(1 << 30);  // an all-zero row / column
 
-// exponent of the largest |x| of each row (row_e) and column (col_e) of X:
-// 64 x 64 tiles, one atomicMax per row and column and tile
+// exponent of the largest |x| of each row (row_e) and column (col_e) of X, on the
+// element-wise tiling: FP64 maxima (NaN ignored), reduced across the lanes that
+// share a column (4 values halved to 1 over lane bits 0, 1, then bits 2, 3) or a
+// row (bits 4, 5, then the four waves through LDS); one ilogb and one atomicMax
+// per row and column and tile
+template <bool kFull>
+__device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, int cols,
+                                                  const double *__restrict__ X, long ldx,
+                                                  int *row_e, int *col_e, double (&red)[4][64]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[4][4];
+  tile64_load<kFull>(tl, rows, cols, X, ldx, v);
+  double cm[4], rm[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    cm[q] = fmax(fmax(fabs(v[q][0]), fabs(v[q][1])), fmax(fabs(v[q][2]), fabs(v[q][3])));
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    rm[r] = fmax(fmax(fabs(v[0][r]), fabs(v[1][r])), fmax(fabs(v[2][r]), fabs(v[3][r])));
+  // columns: after lane bit 0 a lane keeps columns q = 2 b0 + {0, 1}, after bit 1
+  // the one column q = 2 b0 + b1; bits 2, 3 complete the 16-lane maximum
+  {
+    const bool b = lane & 1;
+    const double s0 = b ? cm[0] : cm[2], s1 = b ? cm[1] : cm[3];
+    const double k0 = b ? cm[2] : cm[0], k1 = b ? cm[3] : cm[1];
+    cm[0] = fmax(k0, __shfl_xor(s0, 1, 64));
+    cm[1] = fmax(k1, __shfl_xor(s1, 1, 64));
+  }
+  {
+    const bool b = lane & 2;
+    const double s = b ? cm[0] : cm[1], k = b ? cm[1] : cm[0];
+    cm[0] = fmax(k, __shfl_xor(s, 2, 64));
+  }
+  cm[0] = fmax(cm[0], __shfl_xor(cm[0], 4, 64));
+  cm[0] = fmax(cm[0], __shfl_xor(cm[0], 8, 64));
+  if (col_e && (lane & 12) == 0) {
+    const int j = tl.col(2 * (lane & 1) + ((lane >> 1) & 1));
+    if (j < cols && cm[0] > 0.0) atomicMax(&col_e[j], ilogb(cm[0]));
+  }
+  // rows: lanes l, l ^ 16, l ^ 32, l ^ 48 share rows; after bits 4, 5 a lane keeps
+  // row 2 b4 + b5 of its four
+  {
+    const bool b = lane & 16;
+    const double s0 = b ? rm[0] : rm[2], s1 = b ? rm[1] : rm[3];
+    const double k0 = b ? rm[2] : rm[0], k1 = b ? rm[3] : rm[1];
+    rm[0] = fmax(k0, __shfl_xor(s0, 16, 64));
+    rm[1] = fmax(k1, __shfl_xor(s1, 16, 64));
+  }
+  {
+    const bool b = lane & 32;
+    const double s = b ? rm[0] : rm[1], k = b ? rm[1] : rm[0];
+    rm[0] = fmax(k, __shfl_xor(s, 32, 64));
+  }
+  red[wave][4 * (lane & 15) + 2 * ((lane >> 4) & 1) + ((lane >> 5) & 1)] = rm[0];
+  __syncthreads();
+  if (row_e && wave == 0) {
+    const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+    const int i = tl.i0 + lane;
+    if (i < rows && m > 0.0) atomicMax(&row_e[i], ilogb(m));
+  }
+}
+
 __global__ void __launch_bounds__(256)
     max_exponent_kernel(int rows, int cols, const double *__restrict__ X, long ldx, int *row_e,
                         int *col_e) {
-  __shared__ int red[4][64];
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int i = i0 + lane;
-  int rmax = kNoExponent;
-  for (int jj = wave; jj < 64; jj += 4) {
-    const int j = j0 + jj;
-    int e = kNoExponent;
-    if (i < rows && j < cols) {
-      const double v = fabs(X[i + static_cast<long>(j) * ldx]);
-      if (v > 0.0) e = ilogb(v);
-    }
-    rmax = max(rmax, e);
-    int c = e;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
-    if (col_e && lane == 0 && j < cols && c > kNoExponent) atomicMax(&col_e[j], c);
-  }
-  red[wave][lane] = rmax;
-  __syncthreads();
-  if (row_e && wave == 0 && i < rows) {
-    const int r = max(max(red[0][lane], red[1][lane]), max(red[2][lane], red[3][lane]));
-    if (r > kNoExponent) atomicMax(&row_e[i], r);
-  }
+  __shared__ double red[4][64];
+  const Tile64 tl(rows, cols, X, ldx);
+  if (tl.full) max_exponent_body<true>(tl, rows, cols, X, ldx, row_e, col_e, red);
+  else max_exponent_body<false>(tl, rows, cols, X, ldx, row_e, col_e, red);
 }
 
 // the scale exponent of a row / column whose largest |x| has exponent e
@@ -370,40 +538,69 @@ __device__ __forceinline__ void split_half(double x, int scale, unsigned short (
   p[1] = half_bits(f - half_value(p[0]));
 }
 
-// half planes of X: pc column-major scaled by col_e (right operand), pr
-// row-major scaled by row_e (left operand); the same tiling as split_planes_kernel
+// half planes of X: pc column-major scaled by col_e (right operand), pr row-major
+// scaled by row_e (left operand); the element-wise tiling and row staging above
+template <bool kFull>
+__device__ __forceinline__ void split_half_body(const Tile64 &tl, int rows, int cols,
+                                                const double *__restrict__ X, long ldx,
+                                                unsigned short *__restrict__ pc,
+                                                unsigned short *__restrict__ pr, long pstride,
+                                                const int *__restrict__ row_e,
+                                                const int *__restrict__ col_e, RowStage<2> &st) {
+  double v[4][4];
+  tile64_load<kFull>(tl, rows, cols, X, ldx, v);
+  // the exponents, straight-line like the values, before the first store
+  int rs[4] = {0, 0, 0, 0}, cs[4] = {0, 0, 0, 0};
+  if (pr) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rs[r] = half_scale(row_e[min(tl.r0 + r, rows - 1)]);
+  }
+  if (pc) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cs[q] = half_scale(col_e[min(tl.col(q), cols - 1)]);
+  }
+  const bool cvec = kFull && col_vec_ok(rows, pstride, pc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = tl.col(q);
+    if (pc && j < cols) {
+      unsigned short w0[4], w1[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        unsigned short p[2];
+        split_half(v[q][r], cs[q], p);
+        w0[r] = p[0];
+        w1[r] = p[1];
+      }
+      unsigned short *dst = pc + tl.r0 + static_cast<long>(j) * rows;
+      store_col4(dst, cvec, rows - tl.r0, w0);
+      store_col4(dst + pstride, cvec, rows - tl.r0, w1);
+    }
+    if (pr) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        unsigned short p[2];
+        split_half(v[q][r], rs[r], p);
+        row_stage_put<2>(st, 4 * (threadIdx.x & 15) + r, (threadIdx.x >> 4) + 16 * q, p);
+      }
+    }
+  }
+  if (!pr) return;
+  __syncthreads();
+  row_stage_write<2>(st, tl, rows, cols, pr, pstride);
+}
+
 __global__ void __launch_bounds__(256)
     split_planes_half_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
                              unsigned short *__restrict__ pc, unsigned short *__restrict__ pr,
                              long pstride, const int *__restrict__ row_e,
                              const int *__restrict__ col_e) {
-  __shared__ unsigned short tile[2][64][65];
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int jj = ty; jj < 64; jj += 4) {
-    const int i = i0 + tx, j = j0 + jj;
-    unsigned short p[2] = {0, 0}, q[2] = {0, 0};
-    if (i < rows && j < cols) {
-      const double x = X[i + (long)j * ldx];
-      if (pc) {
-        split_half(x, half_scale(col_e[j]), p);
-        pc[i + (long)j * rows] = p[0];
-        pc[pstride + i + (long)j * rows] = p[1];
-      }
-      if (pr) split_half(x, half_scale(row_e[i]), q);
-    }
-    tile[0][jj][tx] = q[0];
-    tile[1][jj][tx] = q[1];
-  }
-  if (!pr) return;
-  __syncthreads();
-  for (int ii = ty; ii < 64; ii += 4) {
-    const int i = i0 + ii, j = j0 + tx;
-    if (i < rows && j < cols) {
-      pr[(long)i * cols + j] = tile[0][tx][ii];
-      pr[pstride + (long)i * cols + j] = tile[1][tx][ii];
-    }
-  }
+  __shared__ RowStage<2> st;
+  const Tile64 tl(rows, cols, X, ldx);
+  if (tl.full)
+    split_half_body<true>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, st);
+  else
+    split_half_body<false>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, st);
 }
 
 __global__ void fill_exponents_kernel(int *a, int na, int *b, int nb) {

@@ -253,3 +253,17 @@ def test_fp8_buffer_sizes_and_argument_checks():
                                     ctypes.byref(o), None) == ST.INVALID_VALUE
     assert L.clm_expm_fp8tail_d(None, None, 4, pa, 4, pa, 4, pa, 512, None, None,
                                 None) == ST.ALLOC_FAILED
+
+
+@pytest.mark.parametrize("norm,want", [
+    (0.5, (0, 3, 0)), (1.0, (0, 3, 0)), (1.0 + 2.0 ** -52, (0, 3, 0)), (1.01, (0, 13, 0)),
+    (8.0, (1, 13, 1)), (100.0, (5, 13, 2)), (1000.0, (8, 6, 1)), (1e6, (18, 6, 0))])
+def test_split_schedule(norm, want):
+    """(s, polynomial format, split squarings) of the precision variant: BF16 3-way
+    while ||A / 2^s||_1 <= 1 (a matrix scaled to 1-norm 1 whose column sum rounds an
+    ulp above stays there), half planes to s = 5 with two split squarings at s = 4, 5,
+    6-way beyond, every squaring FP64 past s = 8"""
+    from calaman_g_amd.expm import expm_split_plan
+    p = expm_split_plan(norm)
+    assert (p["s"], p["ps_split"], p["split_squarings"]) == want
+    assert p["num_gemms"] == oracle.ps_plan(p["m"], 0)["num_gemms"] + p["s"]

@@ -737,6 +737,138 @@ __device__ __forceinline__ void planes2_compute(const unsigned short *S, int lan
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Epilogue of a 128 x 128 plane-product tile through LDS. The FP32 sums are laid
+// down column-major in LDS (sum[c * kSumLd + r], one 16-byte write per lane and
+// MFMA tile), then the whole workgroup walks the tile in column segments: wave w
+// takes columns w, w + NW, ..., a lane two consecutive rows. Every read of Cin,
+// X1 and the power bank and every write of C is then a 1 KiB contiguous access
+// per wave, and a lane has eight columns' reads in flight at once -- where the
+// MFMA layout gives 32-byte pieces of 16 columns per instruction, one column at
+// a time. The arithmetic is epilogue_store_s's, in its order.
+// ------------------------------------------------------------------------------------
+constexpr int kSumLd = 132;  // FP32 per LDS column of the sum tile (128 + 4 pad)
+
+template <int TM, int TN>
+__device__ __forceinline__ void sum_tile_put(float *sum, int lane, int wm0, int wn0,
+                                             const f4 (&acc)[TM][TN]) {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      *reinterpret_cast<f4 *>(&sum[(wn0 + j * 16 + (lane & 15)) * kSumLd + wm0 + i * 16 +
+                                   d_row(lane, 0)]) = acc[i][j];
+}
+
+__device__ __forceinline__ bool epilogue_vec2_ok(const GemmArgs<double> &g) {
+  auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  return (g.ldc & 1) == 0 && al(g.C) && (!g.Cin || ((g.ldcin & 1) == 0 && al(g.Cin))) &&
+         (g.nterms < 2 || ((g.ldx1 & 1) == 0 && al(g.X1))) &&
+         (g.nterms < 3 || ((g.ldbank & 1) == 0 && (g.bank_stride & 1) == 0 && al(g.bank)));
+}
+
+template <int FMT, int NW, bool kFast>
+__device__ void tile_epilogue(const GemmArgs<double> &g, const float *sum, int m0, int n0,
+                              const int *__restrict__ row_e, const int *__restrict__ col_e) {
+  constexpr int CPW = 128 / NW;  // columns per wave
+  constexpr int CH = 8;          // columns per chunk of loads in flight
+  static_assert(CPW % CH == 0, "column chunks");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rl = 2 * lane, row = m0 + rl;
+  if (kFast && epilogue_vec2_ok(g)) {
+    int rsc[2] = {0, 0};
+    if constexpr (FMT == 1) {
+      rsc[0] = half_scale(row_e[row]);
+      rsc[1] = half_scale(row_e[row + 1]);
+    }
+#pragma unroll
+    for (int c0 = 0; c0 < CPW; c0 += CH) {
+      double2 v[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int c = wave + NW * (c0 + k);
+        const float2 f = *reinterpret_cast<const float2 *>(&sum[c * kSumLd + rl]);
+        double a0 = static_cast<double>(f.x), a1 = static_cast<double>(f.y);
+        if constexpr (FMT == 1) {
+          const int cs = half_scale(col_e[n0 + c]);
+          a0 = ldexp(a0, -(rsc[0] + cs));
+          a1 = ldexp(a1, -(rsc[1] + cs));
+        }
+        v[k] = double2{g.alpha * a0, g.alpha * a1};
+      }
+      if (g.Cin) {
+        double2 cin[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+          cin[k] = *reinterpret_cast<const double2 *>(
+              &g.Cin[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldcin]);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          v[k].x = v[k].x + g.beta * cin[k].x;
+          v[k].y = v[k].y + g.beta * cin[k].y;
+        }
+      }
+      if (g.nterms > 0) {
+        double2 e[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int col = n0 + wave + NW * (c0 + k);
+          e[k] = double2{row == col ? g.coef[0] : 0.0, row + 1 == col ? g.coef[0] : 0.0};
+        }
+        if (g.nterms > 1) {
+          double2 x[CH];
+#pragma unroll
+          for (int k = 0; k < CH; ++k)
+            x[k] = *reinterpret_cast<const double2 *>(
+                &g.X1[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldx1]);
+          const double c1 = g.coef[1];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) {
+            e[k].x = c1 * x[k].x + e[k].x;
+            e[k].y = c1 * x[k].y + e[k].y;
+          }
+        }
+        for (int t = 2; t < g.nterms; ++t) {
+          const double *bk = g.bank + static_cast<long>(t - 2) * g.bank_stride;
+          double2 b[CH];
+#pragma unroll
+          for (int k = 0; k < CH; ++k)
+            b[k] = *reinterpret_cast<const double2 *>(
+                &bk[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldbank]);
+          const double ct = g.coef[t];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) {
+            e[k].x = ct * b[k].x + e[k].x;
+            e[k].y = ct * b[k].y + e[k].y;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          v[k].x = v[k].x + e[k].x;
+          v[k].y = v[k].y + e[k].y;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k)
+        *reinterpret_cast<double2 *>(&g.C[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldc]) =
+            v[k];
+    }
+  } else {
+    for (int k = 0; k < CPW; ++k) {
+      const int c = wave + NW * k, col = n0 + c;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = row + h;
+        if (kFast || (r < g.M && col < g.N)) {
+          double a = static_cast<double>(sum[c * kSumLd + rl + h]);
+          if constexpr (FMT == 1) a = ldexp(a, -(half_scale(row_e[r]) + half_scale(col_e[col])));
+          epilogue_store_s(g, r, col, a);
+        }
+      }
+    }
+  }
+}
+
 template <int SPLIT, int FMT, int NW, int KS, bool kFast>
 __global__ void __launch_bounds__(NW * 64)
     dgemm_split_planes2(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
@@ -785,20 +917,12 @@ __global__ void __launch_bounds__(NW * 64)
     planes2_compute<SPLIT, FMT, KS, TM, TN>(L + ((KT - 1) & 1) * PT::STAGE, lane, wm0, wn0, acc);
   }
 
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + d_row(lane, r);
-        const int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (kFast || (row < g.M && col < g.N)) {
-          double v = static_cast<double>(acc[i][j][r]);
-          if constexpr (FMT == 1) v = ldexp(v, -(half_scale(row_e[row]) + half_scale(col_e[col])));
-          epilogue_store_s(g, row, col, v);
-        }
-      }
+  static_assert(128 * kSumLd * 4 <= 2 * PT::STAGE * 2, "the sum tile fits in the stages");
+  float *sum = reinterpret_cast<float *>(L);
+  __syncthreads();  // every wave is done with the stages
+  sum_tile_put<TM, TN>(sum, lane, wm0, wn0, acc);
+  __syncthreads();
+  tile_epilogue<FMT, NW, kFast>(g, sum, m0, n0, row_e, col_e);
 }
 
 // ------------------------------------------------------------------------------------
@@ -888,7 +1012,11 @@ __global__ void __launch_bounds__(512)
   }
 
   __syncthreads();  // every wave is done with the stages
+  // group 1's sums, then (group 0) the whole tile's, column-major for the epilogue
   float *red = reinterpret_cast<float *>(L);
+  float *sum = red + 4 * 16 * 4 * 64;
+  static_assert((4 * 16 * 4 * 64 + 128 * kSumLd) * 4 <= 2 * PT::STAGE * 2,
+                "the reduction buffer and the sum tile fit in the stages");
   if (grp == 1) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -898,22 +1026,17 @@ __global__ void __launch_bounds__(512)
         for (int r = 0; r < 4; ++r) red[((wpos * 16 + i * 4 + j) * 4 + r) * 64 + lane] = acc[i][j][r];
   }
   __syncthreads();
-  if (grp == 1) return;
+  if (grp == 0) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float sum = acc[i][j][r] + red[((wpos * 16 + i * 4 + j) * 4 + r) * 64 + lane];
-        const int row = m0 + wm0 + i * 16 + d_row(lane, r);
-        const int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (kFast || (row < g.M && col < g.N)) {
-          double v = static_cast<double>(sum);
-          if constexpr (FMT == 1) v = ldexp(v, -(half_scale(row_e[row]) + half_scale(col_e[col])));
-          epilogue_store_s(g, row, col, v);
-        }
-      }
+        for (int r = 0; r < 4; ++r) acc[i][j][r] += red[((wpos * 16 + i * 4 + j) * 4 + r) * 64 + lane];
+    sum_tile_put<4, 4>(sum, lane, wm0, wn0, acc);
+  }
+  __syncthreads();
+  tile_epilogue<FMT, 8, kFast>(g, sum, m0, n0, row_e, col_e);
 }
 
 inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }

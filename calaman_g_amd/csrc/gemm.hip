@@ -46,8 +46,9 @@ __global__ void __launch_bounds__(NT)
   static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN sub-tile");
   constexpr int TM = WM / 16, TN = WN / 16;
 
-  __shared__ __attribute__((aligned(16))) S As[2 * TL::A_STAGE];
-  __shared__ __attribute__((aligned(16))) S Bs[2 * TL::B_STAGE];
+  __shared__ __attribute__((aligned(16))) S L[2 * (TL::A_STAGE + TL::B_STAGE)];
+  S *As = L;
+  S *Bs = L + 2 * TL::A_STAGE;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -89,16 +90,33 @@ __global__ void __launch_bounds__(NT)
                                              lane, wm0, wn0, acc);
   }
 
+  // 128 x 128 tiles: the sums go column-major into the free LDS stages and the
+  // whole workgroup writes them out in coalesced column pieces
+  constexpr int LDT = 128 + 16 / sizeof(S);
+  if constexpr (BM == 128 && BN == 128 && 128 * LDT <= 2 * (TL::A_STAGE + TL::B_STAGE)) {
+    __syncthreads();  // every wave is done with the stages
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+      for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
-        const int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (kFast || (row < g.M && col < g.N)) epilogue_store(g, row, col, S(acc[i][j][r]));
-      }
+        for (int r = 0; r < 4; ++r)
+          L[(wn0 + j * 16 + (lane & 15)) * LDT + wm0 + i * 16 + Mfma<S>::row(lane, r)] =
+              S(acc[i][j][r]);
+    __syncthreads();
+    lds_tile_epilogue<S, NT / 64, kFast>(g, m0, n0, [&](int c, int r) { return L[c * LDT + r]; });
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
+          const int col = n0 + wn0 + j * 16 + (lane & 15);
+          if (kFast || (row < g.M && col < g.N)) epilogue_store(g, row, col, S(acc[i][j][r]));
+        }
+  }
 }
 
 template <typename C, int BM, int BN, int WM, int WN, int NT, bool kFast, int BK = kBK>

@@ -31,6 +31,7 @@
 // MFMA operand layout (16x16x32 bf16 / f16): A lane l holds A[l%16][8(l/16) .. +7],
 // B lane l holds B[8(l/16) .. +7][l%16], D lane l holds rows 4 (l/16) + r.
 #include "clm_internal.hpp"
+#include "gemm_tile.hpp"
 
 namespace clm {
 namespace {
@@ -737,16 +738,9 @@ __device__ __forceinline__ void planes2_compute(const unsigned short *S, int lan
   }
 }
 
-// ------------------------------------------------------------------------------------
-// Epilogue of a 128 x 128 plane-product tile through LDS. The FP32 sums are laid
-// down column-major in LDS (sum[c * kSumLd + r], one 16-byte write per lane and
-// MFMA tile), then the whole workgroup walks the tile in column segments: wave w
-// takes columns w, w + NW, ..., a lane two consecutive rows. Every read of Cin,
-// X1 and the power bank and every write of C is then a 1 KiB contiguous access
-// per wave, and a lane has eight columns' reads in flight at once -- where the
-// MFMA layout gives 32-byte pieces of 16 columns per instruction, one column at
-// a time. The arithmetic is epilogue_store_s's, in its order.
-// ------------------------------------------------------------------------------------
+// Epilogue of a 128 x 128 plane-product tile: the FP32 sums go down column-major
+// in LDS (sum[c * kSumLd + r], one 16-byte write per lane and MFMA tile), then
+// tile::lds_tile_epilogue walks them in coalesced column pieces.
 constexpr int kSumLd = 132;  // FP32 per LDS column of the sum tile (128 + 4 pad)
 
 template <int TM, int TN>
@@ -760,113 +754,16 @@ __device__ __forceinline__ void sum_tile_put(float *sum, int lane, int wm0, int 
                                    d_row(lane, 0)]) = acc[i][j];
 }
 
-__device__ __forceinline__ bool epilogue_vec2_ok(const GemmArgs<double> &g) {
-  auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  return (g.ldc & 1) == 0 && al(g.C) && (!g.Cin || ((g.ldcin & 1) == 0 && al(g.Cin))) &&
-         (g.nterms < 2 || ((g.ldx1 & 1) == 0 && al(g.X1))) &&
-         (g.nterms < 3 || ((g.ldbank & 1) == 0 && (g.bank_stride & 1) == 0 && al(g.bank)));
-}
-
+// the tile's FP32 sums from LDS, the half format's 2^-(e_row + e_col) undone
 template <int FMT, int NW, bool kFast>
-__device__ void tile_epilogue(const GemmArgs<double> &g, const float *sum, int m0, int n0,
-                              const int *__restrict__ row_e, const int *__restrict__ col_e) {
-  constexpr int CPW = 128 / NW;  // columns per wave
-  constexpr int CH = 8;          // columns per chunk of loads in flight
-  static_assert(CPW % CH == 0, "column chunks");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rl = 2 * lane, row = m0 + rl;
-  if (kFast && epilogue_vec2_ok(g)) {
-    int rsc[2] = {0, 0};
-    if constexpr (FMT == 1) {
-      rsc[0] = half_scale(row_e[row]);
-      rsc[1] = half_scale(row_e[row + 1]);
-    }
-#pragma unroll
-    for (int c0 = 0; c0 < CPW; c0 += CH) {
-      double2 v[CH];
-#pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const int c = wave + NW * (c0 + k);
-        const float2 f = *reinterpret_cast<const float2 *>(&sum[c * kSumLd + rl]);
-        double a0 = static_cast<double>(f.x), a1 = static_cast<double>(f.y);
-        if constexpr (FMT == 1) {
-          const int cs = half_scale(col_e[n0 + c]);
-          a0 = ldexp(a0, -(rsc[0] + cs));
-          a1 = ldexp(a1, -(rsc[1] + cs));
-        }
-        v[k] = double2{g.alpha * a0, g.alpha * a1};
-      }
-      if (g.Cin) {
-        double2 cin[CH];
-#pragma unroll
-        for (int k = 0; k < CH; ++k)
-          cin[k] = *reinterpret_cast<const double2 *>(
-              &g.Cin[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldcin]);
-#pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          v[k].x = v[k].x + g.beta * cin[k].x;
-          v[k].y = v[k].y + g.beta * cin[k].y;
-        }
-      }
-      if (g.nterms > 0) {
-        double2 e[CH];
-#pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int col = n0 + wave + NW * (c0 + k);
-          e[k] = double2{row == col ? g.coef[0] : 0.0, row + 1 == col ? g.coef[0] : 0.0};
-        }
-        if (g.nterms > 1) {
-          double2 x[CH];
-#pragma unroll
-          for (int k = 0; k < CH; ++k)
-            x[k] = *reinterpret_cast<const double2 *>(
-                &g.X1[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldx1]);
-          const double c1 = g.coef[1];
-#pragma unroll
-          for (int k = 0; k < CH; ++k) {
-            e[k].x = c1 * x[k].x + e[k].x;
-            e[k].y = c1 * x[k].y + e[k].y;
-          }
-        }
-        for (int t = 2; t < g.nterms; ++t) {
-          const double *bk = g.bank + static_cast<long>(t - 2) * g.bank_stride;
-          double2 b[CH];
-#pragma unroll
-          for (int k = 0; k < CH; ++k)
-            b[k] = *reinterpret_cast<const double2 *>(
-                &bk[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldbank]);
-          const double ct = g.coef[t];
-#pragma unroll
-          for (int k = 0; k < CH; ++k) {
-            e[k].x = ct * b[k].x + e[k].x;
-            e[k].y = ct * b[k].y + e[k].y;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          v[k].x = v[k].x + e[k].x;
-          v[k].y = v[k].y + e[k].y;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < CH; ++k)
-        *reinterpret_cast<double2 *>(&g.C[row + static_cast<long>(n0 + wave + NW * (c0 + k)) * g.ldc]) =
-            v[k];
-    }
-  } else {
-    for (int k = 0; k < CPW; ++k) {
-      const int c = wave + NW * k, col = n0 + c;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = row + h;
-        if (kFast || (r < g.M && col < g.N)) {
-          double a = static_cast<double>(sum[c * kSumLd + rl + h]);
-          if constexpr (FMT == 1) a = ldexp(a, -(half_scale(row_e[r]) + half_scale(col_e[col])));
-          epilogue_store_s(g, r, col, a);
-        }
-      }
-    }
-  }
+__device__ __forceinline__ void tile_epilogue(const GemmArgs<double> &g, const float *sum, int m0,
+                                              int n0, const int *__restrict__ row_e,
+                                              const int *__restrict__ col_e) {
+  tile::lds_tile_epilogue<double, NW, kFast>(g, m0, n0, [&](int c, int r) {
+    double a = static_cast<double>(sum[c * kSumLd + r]);
+    if constexpr (FMT == 1) a = ldexp(a, -(half_scale(row_e[m0 + r]) + half_scale(col_e[n0 + c])));
+    return a;
+  });
 }
 
 template <int SPLIT, int FMT, int NW, int KS, bool kFast>

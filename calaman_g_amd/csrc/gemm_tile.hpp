@@ -284,5 +284,119 @@ __device__ __forceinline__ void cgemm_compute(
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Epilogue of a 128 x 128 real tile through LDS. The caller lays the tile's sums
+// down in LDS (after the K loop the stages are free); then the whole workgroup
+// (NW waves) walks the tile in column segments: wave w takes columns w, w + NW,
+// ..., a lane two consecutive rows. Every read of Cin, X1 and the power bank
+// and every write of C is one contiguous column piece per wave (1 KiB in FP64)
+// in 2-element vectors, with up to eight columns' reads in flight per lane --
+// where the MFMA layout gives short pieces of 16 columns per instruction, one
+// column at a time. The arithmetic is epilogue_store's, in its order.
+// fetch(c, r) returns the tile's value at row r of tile column c.
+// ------------------------------------------------------------------------------------
+template <typename S>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+template <typename S>
+__device__ __forceinline__ bool epilogue_vec2_ok(const GemmArgs<S> &g) {
+  auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & (2 * sizeof(S) - 1)) == 0; };
+  return (g.ldc & 1) == 0 && al(g.C) && (!g.Cin || ((g.ldcin & 1) == 0 && al(g.Cin))) &&
+         (g.nterms < 2 || ((g.ldx1 & 1) == 0 && al(g.X1))) &&
+         (g.nterms < 3 || ((g.ldbank & 1) == 0 && (g.bank_stride & 1) == 0 && al(g.bank)));
+}
+
+template <typename S, int NW, bool kFast, typename Fetch>
+__device__ __forceinline__ void lds_tile_epilogue(const GemmArgs<S> &g, int m0, int n0,
+                                                  Fetch fetch) {
+  using V2 = typename Vec2<S>::type;
+  using O = ops<S>;
+  constexpr int CPW = 128 / NW;            // columns per wave
+  constexpr int CH = CPW < 8 ? CPW : 8;    // columns per chunk of reads in flight
+  static_assert(128 % NW == 0 && CPW % CH == 0, "column chunks");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rl = 2 * lane, row = m0 + rl;
+  auto at = [&](const S *base, long ld, int k0, int k) {
+    return reinterpret_cast<const V2 *>(&base[row + static_cast<long>(n0 + wave + NW * (k0 + k)) * ld]);
+  };
+  if (kFast && epilogue_vec2_ok(g)) {
+#pragma unroll
+    for (int c0 = 0; c0 < CPW; c0 += CH) {
+      V2 v[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int c = wave + NW * (c0 + k);
+        v[k].x = O::mul(g.alpha, fetch(c, rl));
+        v[k].y = O::mul(g.alpha, fetch(c, rl + 1));
+      }
+      if (g.Cin) {
+        V2 cin[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) cin[k] = *at(g.Cin, g.ldcin, c0, k);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          v[k].x = O::add(v[k].x, O::mul(g.beta, cin[k].x));
+          v[k].y = O::add(v[k].y, O::mul(g.beta, cin[k].y));
+        }
+      }
+      if (g.nterms > 0) {
+        V2 e[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int col = n0 + wave + NW * (c0 + k);
+          e[k].x = row == col ? g.coef[0] : O::zero();
+          e[k].y = row + 1 == col ? g.coef[0] : O::zero();
+        }
+        if (g.nterms > 1) {
+          V2 x[CH];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) x[k] = *at(g.X1, g.ldx1, c0, k);
+          const S c1 = g.coef[1];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) {
+            e[k].x = O::mad(c1, x[k].x, e[k].x);
+            e[k].y = O::mad(c1, x[k].y, e[k].y);
+          }
+        }
+        for (int t = 2; t < g.nterms; ++t) {
+          const S *bk = g.bank + static_cast<long>(t - 2) * g.bank_stride;
+          V2 b[CH];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) b[k] = *at(bk, g.ldbank, c0, k);
+          const S ct = g.coef[t];
+#pragma unroll
+          for (int k = 0; k < CH; ++k) {
+            e[k].x = O::mad(ct, b[k].x, e[k].x);
+            e[k].y = O::mad(ct, b[k].y, e[k].y);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          v[k].x = O::add(v[k].x, e[k].x);
+          v[k].y = O::add(v[k].y, e[k].y);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k)
+        *const_cast<V2 *>(at(g.C, g.ldc, c0, k)) = v[k];
+    }
+  } else {
+    for (int k = 0; k < CPW; ++k) {
+      const int c = wave + NW * k, col = n0 + c;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (kFast || (row + h < g.M && col < g.N)) epilogue_store(g, row + h, col, fetch(c, rl + h));
+    }
+  }
+}
+
 }  // namespace tile
 }  // namespace clm

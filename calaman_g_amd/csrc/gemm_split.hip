@@ -39,6 +39,10 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+// 16 bytes of plane data in registers: a native vector, not HIP's uint4 struct,
+// whose global -> register -> LDS copy the compiler turns into one memcpy at the
+// LDS store -- issuing the next slice's loads only after this slice's MFMAs
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSplitBK = 32;
 constexpr int kSplitRow = 40;  // bf16 per LDS row (32 + 8 pad)
@@ -652,8 +656,8 @@ template <int NPL, int KS, int NT, bool kFast>
 __device__ __forceinline__ void planes2_gload(const GemmArgs<double> &g, const unsigned short *PA,
                                               const unsigned short *PB, long pstride, int m0,
                                               int n0, int k0, int tid,
-                                              uint4 (&ra)[NPL][128 * 4 * KS / NT],
-                                              uint4 (&rb)[NPL][128 * 4 * KS / NT]) {
+                                              u32x4 (&ra)[NPL][128 * 4 * KS / NT],
+                                              u32x4 (&rb)[NPL][128 * 4 * KS / NT]) {
   constexpr int CHUNKS = 4 * KS;  // 16-byte chunks per row per slice
   constexpr int IT = 128 * CHUNKS / NT;
   const long K = g.K;
@@ -665,8 +669,8 @@ __device__ __forceinline__ void planes2_gload(const GemmArgs<double> &g, const u
       const int r = c / CHUNKS, kq = (c % CHUNKS) * 8;
       const int gk = k0 + kq;
       if (kFast) {
-        ra[t][it] = *reinterpret_cast<const uint4 *>(PA + t * pstride + (long)(m0 + r) * K + gk);
-        rb[t][it] = *reinterpret_cast<const uint4 *>(PB + t * pstride + (long)(n0 + r) * K + gk);
+        ra[t][it] = *reinterpret_cast<const u32x4 *>(PA + t * pstride + (long)(m0 + r) * K + gk);
+        rb[t][it] = *reinterpret_cast<const u32x4 *>(PB + t * pstride + (long)(n0 + r) * K + gk);
       } else {
         unsigned va[8], vb[8];
 #pragma unroll
@@ -675,9 +679,9 @@ __device__ __forceinline__ void planes2_gload(const GemmArgs<double> &g, const u
           va[e] = (kin && m0 + r < g.M) ? PA[t * pstride + (long)(m0 + r) * K + gk + e] : 0u;
           vb[e] = (kin && n0 + r < g.N) ? PB[t * pstride + (long)(n0 + r) * K + gk + e] : 0u;
         }
-        ra[t][it] = uint4{va[0] | (va[1] << 16), va[2] | (va[3] << 16), va[4] | (va[5] << 16),
+        ra[t][it] = u32x4{va[0] | (va[1] << 16), va[2] | (va[3] << 16), va[4] | (va[5] << 16),
                           va[6] | (va[7] << 16)};
-        rb[t][it] = uint4{vb[0] | (vb[1] << 16), vb[2] | (vb[3] << 16), vb[4] | (vb[5] << 16),
+        rb[t][it] = u32x4{vb[0] | (vb[1] << 16), vb[2] | (vb[3] << 16), vb[4] | (vb[5] << 16),
                           vb[6] | (vb[7] << 16)};
       }
     }
@@ -685,8 +689,8 @@ __device__ __forceinline__ void planes2_gload(const GemmArgs<double> &g, const u
 
 template <int NPL, int KS, int NT>
 __device__ __forceinline__ void planes2_sstore(unsigned short *S, int tid,
-                                               const uint4 (&ra)[NPL][128 * 4 * KS / NT],
-                                               const uint4 (&rb)[NPL][128 * 4 * KS / NT]) {
+                                               const u32x4 (&ra)[NPL][128 * 4 * KS / NT],
+                                               const u32x4 (&rb)[NPL][128 * 4 * KS / NT]) {
   using PT = PlaneTile<NPL, KS>;
   constexpr int CHUNKS = 4 * KS;
   constexpr int IT = 128 * CHUNKS / NT;
@@ -696,8 +700,8 @@ __device__ __forceinline__ void planes2_sstore(unsigned short *S, int tid,
     for (int it = 0; it < IT; ++it) {
       const int c = tid + it * NT;
       const int r = c / CHUNKS, kq = (c % CHUNKS) * 8;
-      *reinterpret_cast<uint4 *>(&S[t * PT::PLANE + r * PT::ROW + kq]) = ra[t][it];
-      *reinterpret_cast<uint4 *>(&S[(NPL + t) * PT::PLANE + r * PT::ROW + kq]) = rb[t][it];
+      *reinterpret_cast<u32x4 *>(&S[t * PT::PLANE + r * PT::ROW + kq]) = ra[t][it];
+      *reinterpret_cast<u32x4 *>(&S[(NPL + t) * PT::PLANE + r * PT::ROW + kq]) = rb[t][it];
     }
 }
 
@@ -798,7 +802,7 @@ __global__ void __launch_bounds__(NW * 64)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  uint4 ra[NPL][IT], rb[NPL][IT];
+  u32x4 ra[NPL][IT], rb[NPL][IT];
   const int KT = (g.K + PT::BK - 1) / PT::BK;
   if (KT > 0) {
     planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
@@ -807,6 +811,9 @@ __global__ void __launch_bounds__(NW * 64)
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
       planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * PT::BK, tid, ra, rb);
+      // the next slice's loads go out before this slice's MFMAs, whose time covers
+      // their latency (the scheduler would otherwise sink them next to the LDS stores)
+      __builtin_amdgcn_sched_barrier(0);
       planes2_compute<SPLIT, FMT, KS, TM, TN>(L + cur * PT::STAGE, lane, wm0, wn0, acc);
       planes2_sstore<NPL, KS, NT>(L + (cur ^ 1) * PT::STAGE, tid, ra, rb);
       __syncthreads();
@@ -892,7 +899,7 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  uint4 ra[NPL][IT], rb[NPL][IT];
+  u32x4 ra[NPL][IT], rb[NPL][IT];
   const int KT = (g.K + PT::BK - 1) / PT::BK;
   if (KT > 0) {
     planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
@@ -901,6 +908,9 @@ __global__ void __launch_bounds__(512)
     for (int kt = 0; kt < KT - 1; ++kt) {
       const int cur = kt & 1;
       planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * PT::BK, tid, ra, rb);
+      // the next slice's loads go out before this slice's MFMAs, whose time covers
+      // their latency (the scheduler would otherwise sink them next to the LDS stores)
+      __builtin_amdgcn_sched_barrier(0);
       planes_kg_compute<SPLIT, FMT>(L + cur * PT::STAGE, lane, wm0, wn0, grp, acc);
       planes2_sstore<NPL, KS, NT>(L + (cur ^ 1) * PT::STAGE, tid, ra, rb);
       __syncthreads();

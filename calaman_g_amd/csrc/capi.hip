@@ -343,11 +343,18 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
 //                              of the Paterson-Stockmeyer Horner steps)
 struct PlaneSet {
   unsigned short *p = nullptr;
-  int *e = nullptr;  // half format: per-row (row-major set) or per-column exponents
+  int *e = nullptr;     // half format: per-row (row-major set) or per-column exponents,
+  mutable int gen = 0;  // tagged with the generation of the split that wrote them
 };
 struct SplitWs {
   PlaneSet PR, PCA, PCX;
   long pstride = 0;
+  mutable int generation = 0;  // splits so far in this run (exponent tags)
+  // the run's first step when it uses the half format: every exponent to generation 0
+  hipError_t start(hipStream_t st, int n) const {
+    generation = 0;
+    return launch_clear_exponents(st, n, PR.e, PCA.e, PCX.e);
+  }
   size_t carve(void *base, int n) {
     Layout l(base);
     pstride = static_cast<long>(Layout::align(static_cast<size_t>(n) * n * 2) / 2);
@@ -368,15 +375,19 @@ size_t expm_split_buffer_size(int n) {
 // planes of X as a right operand (pc, column-major) and / or a left operand (pr,
 // row-major); the half format first finds the row / column exponents
 clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
-                          const PlaneSet *pc, const PlaneSet *pr, long pstride) {
+                          const PlaneSet *pc, const PlaneSet *pr, const SplitWs &sp) {
   if (split == CLM_SPLIT_FP16X3) {
-    CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr, pc ? pc->e : nullptr));
+    const int gen = ++sp.generation;
+    if (pc) pc->gen = gen;
+    if (pr) pr->gen = gen;
+    CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr, pc ? pc->e : nullptr,
+                                      gen));
     CLM_TRY_HIP(launch_split_planes_half(st, n, n, X, ldx, pc ? pc->p : nullptr,
-                                         pr ? pr->p : nullptr, pstride, pr ? pr->e : nullptr,
-                                         pc ? pc->e : nullptr));
+                                         pr ? pr->p : nullptr, sp.pstride, pr ? pr->e : nullptr,
+                                         pc ? pc->e : nullptr, gen, gen));
   } else {
     CLM_TRY_HIP(launch_split_planes(st, split, n, n, X, ldx, pc ? pc->p : nullptr,
-                                    pr ? pr->p : nullptr, pstride));
+                                    pr ? pr->p : nullptr, sp.pstride));
   }
   return CLM_STATUS_SUCCESS;
 }
@@ -384,7 +395,7 @@ clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, lon
 // the split product C = PR-planes x PC-planes (+ epilogue in g)
 clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const PlaneSet &a,
                          const PlaneSet &b, long pstride) {
-  CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, a.p, b.p, pstride, a.e, b.e));
+  CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, a.p, b.p, pstride, a.e, b.e, a.gen, b.gen));
   return CLM_STATUS_SUCCESS;
 }
 
@@ -399,7 +410,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   const bool need_a_planes = p.num_powers > 0 || (p.r >= 1 && p.s < 2);
   if (need_a_planes)
     CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, p.num_powers > 0 ? &sp.PR : nullptr,
-                         sp.pstride));
+                         sp));
   auto plain = [&](double *C, long ldc) {
     GemmArgs<double> g;
     g.M = g.N = g.K = n;
@@ -411,14 +422,14 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   for (int k = 2; k <= p.num_powers + 1; ++k) {
     if (k > 2)
       CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(k - 3) * stride, n, nullptr,
-                           &sp.PR, sp.pstride));
+                           &sp.PR, sp));
     CLM_TRY(gemm_planes(st, plain(bank + static_cast<long>(k - 2) * stride, n), split, sp.PR,
                         sp.PCA, sp.pstride));
   }
   const PlaneSet *pcx = &sp.PCA;
   if (p.r >= 1 && p.s >= 2) {
     CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(p.s - 2) * stride, n, &sp.PCX,
-                         nullptr, sp.pstride));
+                         nullptr, sp));
     pcx = &sp.PCX;
   }
   double *buf[2] = {P, scratch};
@@ -432,7 +443,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
                                  terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
   for (int j = p.r - 1; j >= 0; --j) {
     const int next = 1 - cur;
-    CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp.pstride));
+    CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp));
     GemmArgs<double> g = plain(buf[next], ld[next]);
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
@@ -516,10 +527,10 @@ clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int 
       ++count;
     } else {
       if (k == 2) {
-        CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp.pstride));
+        CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp));
       } else {
-        if (!ca_split) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp.pstride));
-        CLM_TRY(split_planes(st, split, n, L, ldl, nullptr, &sp.PR, sp.pstride));
+        if (!ca_split) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp));
+        CLM_TRY(split_planes(st, split, n, L, ldl, nullptr, &sp.PR, sp));
       }
       ca_split = true;
       CLM_TRY(gemm_planes(st, g, split, sp.PR, sp.PCA, sp.pstride));
@@ -561,9 +572,9 @@ clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int 
     } else {
       bool &have = x_is_a ? ca_split : cx_split;
       const PlaneSet &right = x_is_a ? sp.PCA : sp.PCX;
-      if (!have) CLM_TRY(split_planes(st, split, n, X, ldx, &right, nullptr, sp.pstride));
+      if (!have) CLM_TRY(split_planes(st, split, n, X, ldx, &right, nullptr, sp));
       have = true;
-      CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp.pstride));
+      CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp));
       CLM_TRY(gemm_planes(st, g, split, sp.PR, right, sp.pstride));
     }
     cur = next;
@@ -622,6 +633,8 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   p.dtype = dtype_code<T>();
   if (variant == kFp8Tail) p.fp8_min_degree = fp8_min_degree;
   if (plan) *plan = p;
+  if (use_split && (p.ps_split == CLM_SPLIT_FP16X3 || p.split_squarings > 0))
+    CLM_TRY_HIP(sp.start(st, n));
 
   const T *src = dA;
   long ld_src = lda;
@@ -665,7 +678,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
       if (split_sq) {
         // split squarings in half planes (scaled per row / column: no FP32 overflow)
         const int sq = CLM_SPLIT_FP16X3;
-        CLM_TRY(split_planes(st, sq, n, cur, ldc, &sp.PCA, &sp.PR, sp.pstride));
+        CLM_TRY(split_planes(st, sq, n, cur, ldc, &sp.PCA, &sp.PR, sp));
         GemmArgs<double> g;
         g.M = g.N = g.K = n;
         g.C = nxt;

@@ -169,19 +169,25 @@ hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols
                                long ldx, unsigned short *pc, unsigned short *pr, long pstride);
 // the split product from pre-split planes: PA = row-major planes of A (M x K),
 // PB = column-major planes of B (K x N); g.A / g.B are not read
+// (row_e / col_e: the half format's tagged exponents of A's rows / B's columns,
+// written by generations row_gen / col_gen; unused by the bfloat16 formats)
 hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
                                     const unsigned short *PA, const unsigned short *PB,
                                     long pstride, const int *row_e = nullptr,
-                                    const int *col_e = nullptr);
-// IEEE-half planes (split = CLM_SPLIT_FP16X3): the largest-magnitude exponent of
-// every row (row_e) and column (col_e) of X, either may be null ...
+                                    const int *col_e = nullptr, int row_gen = 0, int col_gen = 0);
+// IEEE-half planes (split = CLM_SPLIT_FP16X3). Exponent arrays hold values tagged
+// with the generation (>= 1) of the split that wrote them, so they are cleared
+// (to generation 0) once per run of splits, not before each: ...
+hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c);
+// ... the largest-magnitude exponent of every row (row_e) and column (col_e) of
+// X, as generation gen, either array may be null ...
 hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const double *X, long ldx,
-                                 int *row_e, int *col_e);
+                                 int *row_e, int *col_e, int gen);
 // ... and the two half planes of X scaled by them: pc column-major by col_e
 // (right operand), pr row-major by row_e (left operand); either may be null
 hipError_t launch_split_planes_half(hipStream_t stream, int rows, int cols, const double *X,
                                     long ldx, unsigned short *pc, unsigned short *pr, long pstride,
-                                    const int *row_e, const int *col_e);
+                                    const int *row_e, const int *col_e, int row_gen, int col_gen);
 // e4m3 planes with one power-of-two exponent per 128 x 128 tile: tile (ti, tj)
 // of X is scaled by 2^e, e = the tile's entry [ti + tj * ceil(rows / 128)] of
 // ec / er, so its largest magnitude lands in (224, 448]; pc column-major

@@ -455,6 +455,20 @@ __global__ void __launch_bounds__(256)
 // 2^(e_row + e_col), exactly.
 constexpr int kNoExponent = -(1 << 30);  // an all-zero row / column
 
+// Exponent arrays hold tagged values, (gen << 12) | (e + 2048), gen = the
+// generation of the split that wrote them (gen >= 1, counted per run of splits)
+// and e clamped to [-2047, 2047] (an infinite maximum clamps to 2047, whose scale
+// 2^-2034 still sends every finite entry to 0). atomicMax over tags is the max
+// over e within a generation, and an entry an earlier generation left reads as
+// an all-zero row's or column's -- so the arrays need one clearing per run of
+// splits, not one before every split.
+__device__ __forceinline__ int exp_tag(int gen, int e) {
+  return (gen << 12) | (min(max(e, -2047), 2047) + 2048);
+}
+__device__ __forceinline__ int exp_of(int tagged, int gen) {
+  return (tagged >> 12) == gen ? (tagged & 4095) - 2048 : kNoExponent;
+}
+
 // exponent of the largest |x| of each row (row_e) and column (col_e) of X, on the
 // element-wise tiling: FP64 maxima (NaN ignored), reduced across the lanes that
 // share a column (4 values halved to 1 over lane bits 0, 1, then bits 2, 3) or a
@@ -463,7 +477,8 @@ constexpr int kNoExponent = -(1 << 30);  // an all-zero row / column
 template <bool kFull>
 __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, int cols,
                                                   const double *__restrict__ X, long ldx,
-                                                  int *row_e, int *col_e, double (&red)[4][64]) {
+                                                  int *row_e, int *col_e, int gen,
+                                                  double (&red)[4][64]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double v[4][4];
   tile64_load<kFull>(tl, rows, cols, X, ldx, v);
@@ -492,7 +507,7 @@ __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, in
   cm[0] = fmax(cm[0], __shfl_xor(cm[0], 8, 64));
   if (col_e && (lane & 12) == 0) {
     const int j = tl.col(2 * (lane & 1) + ((lane >> 1) & 1));
-    if (j < cols && cm[0] > 0.0) atomicMax(&col_e[j], ilogb(cm[0]));
+    if (j < cols && cm[0] > 0.0) atomicMax(&col_e[j], exp_tag(gen, ilogb(cm[0])));
   }
   // rows: lanes l, l ^ 16, l ^ 32, l ^ 48 share rows; after bits 4, 5 a lane keeps
   // row 2 b4 + b5 of its four
@@ -513,17 +528,17 @@ __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, in
   if (row_e && wave == 0) {
     const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
     const int i = tl.i0 + lane;
-    if (i < rows && m > 0.0) atomicMax(&row_e[i], ilogb(m));
+    if (i < rows && m > 0.0) atomicMax(&row_e[i], exp_tag(gen, ilogb(m)));
   }
 }
 
 __global__ void __launch_bounds__(256)
     max_exponent_kernel(int rows, int cols, const double *__restrict__ X, long ldx, int *row_e,
-                        int *col_e) {
+                        int *col_e, int gen) {
   __shared__ double red[4][64];
   const Tile64 tl(rows, cols, X, ldx);
-  if (tl.full) max_exponent_body<true>(tl, rows, cols, X, ldx, row_e, col_e, red);
-  else max_exponent_body<false>(tl, rows, cols, X, ldx, row_e, col_e, red);
+  if (tl.full) max_exponent_body<true>(tl, rows, cols, X, ldx, row_e, col_e, gen, red);
+  else max_exponent_body<false>(tl, rows, cols, X, ldx, row_e, col_e, gen, red);
 }
 
 // the scale exponent of a row / column whose largest |x| has exponent e
@@ -551,18 +566,19 @@ __device__ __forceinline__ void split_half_body(const Tile64 &tl, int rows, int 
                                                 unsigned short *__restrict__ pc,
                                                 unsigned short *__restrict__ pr, long pstride,
                                                 const int *__restrict__ row_e,
-                                                const int *__restrict__ col_e, RowStage<2> &st) {
+                                                const int *__restrict__ col_e, int row_gen,
+                                                int col_gen, RowStage<2> &st) {
   double v[4][4];
   tile64_load<kFull>(tl, rows, cols, X, ldx, v);
   // the exponents, straight-line like the values, before the first store
   int rs[4] = {0, 0, 0, 0}, cs[4] = {0, 0, 0, 0};
   if (pr) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) rs[r] = half_scale(row_e[min(tl.r0 + r, rows - 1)]);
+    for (int r = 0; r < 4; ++r) rs[r] = half_scale(exp_of(row_e[min(tl.r0 + r, rows - 1)], row_gen));
   }
   if (pc) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) cs[q] = half_scale(col_e[min(tl.col(q), cols - 1)]);
+    for (int q = 0; q < 4; ++q) cs[q] = half_scale(exp_of(col_e[min(tl.col(q), cols - 1)], col_gen));
   }
   const bool cvec = kFull && col_vec_ok(rows, pstride, pc);
 #pragma unroll
@@ -599,21 +615,25 @@ __global__ void __launch_bounds__(256)
     split_planes_half_kernel(int rows, int cols, const double *__restrict__ X, long ldx,
                              unsigned short *__restrict__ pc, unsigned short *__restrict__ pr,
                              long pstride, const int *__restrict__ row_e,
-                             const int *__restrict__ col_e) {
+                             const int *__restrict__ col_e, int row_gen, int col_gen) {
   __shared__ RowStage<2> st;
   const Tile64 tl(rows, cols, X, ldx);
   if (tl.full)
-    split_half_body<true>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, st);
+    split_half_body<true>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, row_gen, col_gen,
+                          st);
   else
-    split_half_body<false>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, st);
+    split_half_body<false>(tl, rows, cols, X, ldx, pc, pr, pstride, row_e, col_e, row_gen, col_gen,
+                           st);
 }
 
-__global__ void fill_exponents_kernel(int *a, int na, int *b, int nb) {
+__global__ void clear_exponents_kernel(int *a, int *b, int *c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a && i < na) a[i] = kNoExponent;
-  if (b && i < nb) b[i] = kNoExponent;
+  if (i < n) {
+    if (a) a[i] = 0;
+    if (b) b[i] = 0;
+    if (c) c[i] = 0;
+  }
 }
-
 // the 16x16x32 MFMA of a plane format: bf16 or (FMT = 1) IEEE half
 template <int FMT>
 struct PlaneMfma;
@@ -762,10 +782,13 @@ __device__ __forceinline__ void sum_tile_put(float *sum, int lane, int wm0, int 
 template <int FMT, int NW, bool kFast>
 __device__ __forceinline__ void tile_epilogue(const GemmArgs<double> &g, const float *sum, int m0,
                                               int n0, const int *__restrict__ row_e,
-                                              const int *__restrict__ col_e) {
+                                              const int *__restrict__ col_e, int row_gen,
+                                              int col_gen) {
   tile::lds_tile_epilogue<double, NW, kFast>(g, m0, n0, [&](int c, int r) {
     double a = static_cast<double>(sum[c * kSumLd + r]);
-    if constexpr (FMT == 1) a = ldexp(a, -(half_scale(row_e[m0 + r]) + half_scale(col_e[n0 + c])));
+    if constexpr (FMT == 1)
+      a = ldexp(a, -(half_scale(exp_of(row_e[m0 + r], row_gen)) +
+                     half_scale(exp_of(col_e[n0 + c], col_gen))));
     return a;
   });
 }
@@ -775,7 +798,7 @@ __global__ void __launch_bounds__(NW * 64)
     dgemm_split_planes2(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
                         const unsigned short *__restrict__ PB, const long pstride,
                         const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
-                        const int *__restrict__ col_e) {
+                        const int *__restrict__ col_e, const int row_gen, const int col_gen) {
   constexpr int NT = NW * 64;
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
   using PT = PlaneTile<NPL, KS>;
@@ -826,7 +849,7 @@ __global__ void __launch_bounds__(NW * 64)
   __syncthreads();  // every wave is done with the stages
   sum_tile_put<TM, TN>(sum, lane, wm0, wn0, acc);
   __syncthreads();
-  tile_epilogue<FMT, NW, kFast>(g, sum, m0, n0, row_e, col_e);
+  tile_epilogue<FMT, NW, kFast>(g, sum, m0, n0, row_e, col_e, row_gen, col_gen);
 }
 
 // ------------------------------------------------------------------------------------
@@ -872,7 +895,7 @@ __global__ void __launch_bounds__(512)
     dgemm_split_planes_kg(const GemmArgs<double> g, const unsigned short *__restrict__ PA,
                           const unsigned short *__restrict__ PB, const long pstride,
                           const int tiles_m, const int tiles_n, const int *__restrict__ row_e,
-                          const int *__restrict__ col_e) {
+                          const int *__restrict__ col_e, const int row_gen, const int col_gen) {
   static_assert(SPLIT == 3, "the k-grouped kernel runs the 3-product formats");
   constexpr int NT = 512, KS = 2;
   constexpr int NPL = SplitPlanes<SPLIT>::NPL;
@@ -943,7 +966,7 @@ __global__ void __launch_bounds__(512)
     sum_tile_put<4, 4>(sum, lane, wm0, wn0, acc);
   }
   __syncthreads();
-  tile_epilogue<FMT, 8, kFast>(g, sum, m0, n0, row_e, col_e);
+  tile_epilogue<FMT, 8, kFast>(g, sum, m0, n0, row_e, col_e, row_gen, col_gen);
 }
 
 inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -981,7 +1004,8 @@ hipError_t launch_split_planes(hipStream_t stream, int split, int rows, int cols
 
 hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &g, int split,
                                     const unsigned short *PA, const unsigned short *PB,
-                                    long pstride, const int *row_e, const int *col_e) {
+                                    long pstride, const int *row_e, const int *col_e, int row_gen,
+                                    int col_gen) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
   // 3-product formats: 8 waves, 64-deep slices (two LDS stages of two planes);
@@ -992,10 +1016,10 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
   const dim3 grid(tiles_m * tiles_n), block(512);
 #define CLM_SPLIT_LAUNCH(S, FMT, KS, F)                                                       \
   hipLaunchKernelGGL((dgemm_split_planes2<S, FMT, 8, KS, F>), grid, block, 0, stream, g, PA, PB, \
-                     pstride, tiles_m, tiles_n, row_e, col_e)
+                     pstride, tiles_m, tiles_n, row_e, col_e, row_gen, col_gen)
 #define CLM_SPLIT_LAUNCH_KG(FMT, F)                                                          \
   hipLaunchKernelGGL((dgemm_split_planes_kg<3, FMT, F>), grid, block, 0, stream, g, PA, PB,    \
-                     pstride, tiles_m, tiles_n, row_e, col_e)
+                     pstride, tiles_m, tiles_n, row_e, col_e, row_gen, col_gen)
   if (split == CLM_SPLIT_FP16X3) {
     if (fast) CLM_SPLIT_LAUNCH_KG(1, true);
     else CLM_SPLIT_LAUNCH_KG(1, false);
@@ -1011,24 +1035,27 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
   return hipGetLastError();
 }
 
+hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c) {
+  if (n <= 0 || (!a && !b && !c)) return hipSuccess;
+  hipLaunchKernelGGL(clear_exponents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, b, c, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const double *X, long ldx,
-                                 int *row_e, int *col_e) {
+                                 int *row_e, int *col_e, int gen) {
   if (rows <= 0 || cols <= 0 || (!row_e && !col_e)) return hipSuccess;
-  const int nmax = rows > cols ? rows : cols;
-  hipLaunchKernelGGL(fill_exponents_kernel, dim3((nmax + 255) / 256), dim3(256), 0, stream, row_e,
-                     rows, col_e, cols);
   hipLaunchKernelGGL(max_exponent_kernel, dim3((rows + 63) / 64, (cols + 63) / 64), dim3(256), 0,
-                     stream, rows, cols, X, ldx, row_e, col_e);
+                     stream, rows, cols, X, ldx, row_e, col_e, gen);
   return hipGetLastError();
 }
 
 hipError_t launch_split_planes_half(hipStream_t stream, int rows, int cols, const double *X,
                                     long ldx, unsigned short *pc, unsigned short *pr, long pstride,
-                                    const int *row_e, const int *col_e) {
+                                    const int *row_e, const int *col_e, int row_gen, int col_gen) {
   if (rows <= 0 || cols <= 0 || (!pc && !pr)) return hipSuccess;
   const dim3 grid((rows + 63) / 64, (cols + 63) / 64), block(256);
   hipLaunchKernelGGL(split_planes_half_kernel, grid, block, 0, stream, rows, cols, X, ldx, pc, pr,
-                     pstride, row_e, col_e);
+                     pstride, row_e, col_e, row_gen, col_gen);
   return hipGetLastError();
 }
 

@@ -169,7 +169,7 @@ def gemm_roofline(dev, n=N, reps=20):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    return {"kernel": "dgemm_mfma<128,128,32,32,1024,bk32> n=2048", "bound": "mfma",
+    return {"kernel": "rgemm_mfma<double,128,128,32,32,1024,bk32> n=2048", "bound": "mfma",
             "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
             "mean_launch_ms": round(t * 1e3, 4), "flop_per_launch": 2 * n ** 3}

@@ -150,3 +150,37 @@ def test_expm_split_large_result_stays_finite(dev):
     assert 1 <= plan["s"] <= 8 and plan["split_squarings"] >= 1
     assert np.all(np.isfinite(got))
     assert rel(got, ref) <= 1e-6
+
+
+def test_expm_split_reused_workspace_matches_fresh(dev):
+    """The half format's row / column exponents are tagged with the generation of the
+    split that wrote them and cleared once per call: a workspace that held a larger
+    matrix's exponents (same generations, larger exponents: without the clear they would
+    win the atomic maximum) or positive garbage (0x5A5A5A5A: a huge generation that no
+    split would ever overwrite) gives the same bits as a zeroed one, zero rows and
+    columns included"""
+    n = 96
+    rng = np.random.default_rng(31)
+    big = rng.standard_normal((n, n))
+    big *= 300.0 / np.abs(big).sum(0).max()
+    a = rng.standard_normal((n, n))
+    a[::3, :] = 0.0
+    a[:, 1::4] = 0.0
+    a *= 20.0 / np.abs(a).sum(0).max()
+    outs = []
+    for fill in ("reuse", 0, 0x5A):
+        work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
+        if fill == "reuse":
+            d_big = cm(big, dev)
+            st, plan_big = expm_split_cm(n, d_big, n, torch.empty_like(d_big), n, work)
+            assert st == 0 and plan_big["split_squarings"] > 0
+        else:
+            work.fill_(fill)
+        da = cm(a, dev)
+        de = torch.empty_like(da)
+        st, plan = expm_split_cm(n, da, n, de, n, work)
+        torch.cuda.synchronize()
+        assert st == 0 and plan["ps_split"] == 13
+        outs.append(de.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert rel(outs[0].T, oracle.expm(a)) <= 1e-6

@@ -184,3 +184,26 @@ def test_expm_split_reused_workspace_matches_fresh(dev):
         outs.append(de.cpu().numpy())
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     assert rel(outs[0].T, oracle.expm(a)) <= 1e-6
+
+
+@pytest.mark.parametrize("n,target", [(130, 8.0), (200, 100.0), (64, 0.5)])
+def test_expm_split_padded_leading_dims_same_bits(dev, n, target):
+    """odd leading dimensions (lda = n + 1, lde = n + 3) take the edge paths of the
+    element-wise split kernels (unaligned, clamped loads) and of the products' LDS
+    epilogue (element stores): the same bits as the contiguous call, padding untouched"""
+    rng = np.random.default_rng(n + 5)
+    a = rng.standard_normal((n, n))
+    a *= target / np.abs(a).sum(0).max()
+    want, plan = _run_split(a, dev)
+    lda, lde = n + 1, n + 3
+    bufa = np.full((n, lda), 3.0)
+    bufa[:, :n] = a.T
+    da = torch.from_numpy(bufa).to(dev)
+    de = torch.full((n, lde), -5.0, dtype=torch.float64, device=dev)
+    work = torch.empty(cg.expm_split_buffer_size(n), dtype=torch.uint8, device=dev)
+    st, plan2 = expm_split_cm(n, da, lda, de, lde, work)
+    torch.cuda.synchronize()
+    assert st == 0 and plan2 == plan
+    out = de.cpu().numpy()
+    assert np.all(out[:, n:] == -5.0)
+    assert np.array_equal(out[:, :n].T, want)

@@ -188,6 +188,46 @@ __device__ __forceinline__ double uniform(double v) {
 }
 
 // ---- workgroup-wide building blocks ---------------------------------------------------
+// The products' epilogue goes through LDS: after the K loop a tile's sums are laid
+// down column-major (pitch BM + 1, so the 16 columns an MFMA tile's lanes hold fall
+// in distinct banks) in the free staging LDS, PW columns at a time -- the widest of
+// BN, BN / 2, ... (a multiple of the wave tile's width WN) that fits -- and every
+// lane then stores elements in column order. Consecutive lanes hold consecutive
+// rows, so C, Cin and the power blocks are touched in whole column pieces instead
+// of the MFMA layout's 4-row pieces of 16 columns. The arithmetic is
+// epilogue_store's.
+template <typename E, int BM, int BN, int WN>
+constexpr int epilogue_pass_cols(size_t lds_bytes_) {
+  int pw = BN;
+  while (pw > WN && static_cast<size_t>(BM + 1) * pw * sizeof(E) > lds_bytes_) pw /= 2;
+  return pw;
+}
+
+// lanes write the sums of the wave tiles whose columns lie in [p0, p0 + PW); value(i,
+// j, r) is the wave's element r of MFMA tile (i, j)
+template <typename E, typename S, int BM, int TM, int TN, typename Value>
+__device__ __forceinline__ void epilogue_put(E *sumt, int p0, int lane, int wm0, int wn0,
+                                             Value value) {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sumt[(wn0 - p0 + j * 16 + (lane & 15)) * (BM + 1) + wm0 + i * 16 + Mfma<S>::row(lane, r)] =
+            value(i, j, r);
+}
+
+template <typename E, int BM, int PW, int NT>
+__device__ __forceinline__ void epilogue_drain(const GemmArgs<E> &g, const E *sumt, int m0,
+                                               int c0) {
+  for (int e = threadIdx.x; e < BM * PW; e += NT) {
+    const int r = e % BM, c = e / BM;
+    const int row = m0 + r, col = c0 + c;
+    if (row < g.M && col < g.N) epilogue_store(g, row, col, sumt[c * (BM + 1) + r]);
+  }
+}
+
 // C = A*B (+ epilogue of g) over the whole n x n output, tile by tile, by all
 // 256 lanes; ends with a barrier, so C is complete for every lane afterwards.
 template <typename S, int BM, int BN, int WM, int WN, int BK, int NT = 256>
@@ -229,16 +269,21 @@ __device__ void wg_gemm_real(const GemmArgs<S> &g, char *lds) {
       rgemm_compute<S, BM, BN, WM, WN, NT, BK>(As + last * TL::A_STAGE, Bs + last * TL::B_STAGE,
                                                 lane, wm0, wn0, acc);
     }
+    constexpr int PW = epilogue_pass_cols<S, BM, BN, WN>(2 * (TL::A_STAGE + TL::B_STAGE) * sizeof(S));
+    static_assert(BN % PW == 0 && PW % WN == 0 &&
+                      static_cast<size_t>(BM + 1) * PW * sizeof(S) <=
+                          2 * (TL::A_STAGE + TL::B_STAGE) * sizeof(S),
+                  "epilogue passes fit the staging LDS");
+    S *sumt = reinterpret_cast<S *>(lds);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
-          const int col = n0 + wn0 + j * 16 + (lane & 15);
-          if (row < g.M && col < g.N) epilogue_store(g, row, col, S(acc[i][j][r]));
-        }
+    for (int p0 = 0; p0 < BN; p0 += PW) {
+      __syncthreads();  // the stages (or the previous pass) are no longer read
+      if (wn0 >= p0 && wn0 < p0 + PW)
+        epilogue_put<S, S, BM, TM, TN>(sumt, p0, lane, wm0, wn0,
+                                       [&](int i, int j, int r) { return S(acc[i][j][r]); });
+      __syncthreads();
+      epilogue_drain<S, BM, PW, NT>(g, sumt, m0, n0 + p0);
+    }
   }
   __syncthreads();
 }
@@ -282,17 +327,21 @@ __device__ void wg_gemm_cplx(const GemmArgs<C> &g, char *lds) {
       cgemm_compute<C, BM, BN, WM, WN, NT, BK>(L + ((KT - 1) & 1) * TL::STAGE, lane, wm0, wn0,
                                                 cr, ci);
     }
+    constexpr int PW = epilogue_pass_cols<C, BM, BN, WN>(2 * TL::STAGE * sizeof(S));
+    static_assert(BN % PW == 0 && PW % WN == 0 &&
+                      static_cast<size_t>(BM + 1) * PW * sizeof(C) <= 2 * TL::STAGE * sizeof(S),
+                  "epilogue passes fit the staging LDS");
+    C *sumt = reinterpret_cast<C *>(lds);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm0 + i * 16 + Mfma<S>::row(lane, r);
-          const int col = n0 + wn0 + j * 16 + (lane & 15);
-          if (row < g.M && col < g.N)
-            epilogue_store(g, row, col, C{S(cr[i][j][r]), S(ci[i][j][r])});
-        }
+    for (int p0 = 0; p0 < BN; p0 += PW) {
+      __syncthreads();  // the stages (or the previous pass) are no longer read
+      if (wn0 >= p0 && wn0 < p0 + PW)
+        epilogue_put<C, S, BM, TM, TN>(sumt, p0, lane, wm0, wn0, [&](int i, int j, int r) {
+          return C{S(cr[i][j][r]), S(ci[i][j][r])};
+        });
+      __syncthreads();
+      epilogue_drain<C, BM, PW, NT>(g, sumt, m0, n0 + p0);
+    }
   }
   __syncthreads();
 }

@@ -398,5 +398,49 @@ __device__ __forceinline__ void lds_tile_epilogue(const GemmArgs<S> &g, int m0, 
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Epilogue of any tile through LDS in column passes (the batched kernels' products and
+// the complex products): after the K loop a tile's sums are laid
+// down column-major (pitch BM + 1, so the 16 columns an MFMA tile's lanes hold fall
+// in distinct banks) in the free staging LDS, PW columns at a time -- the widest of
+// BN, BN / 2, ... (a multiple of the wave tile's width WN) that fits -- and every
+// lane then stores elements in column order. Consecutive lanes hold consecutive
+// rows, so C, Cin and the power blocks are touched in whole column pieces instead
+// of the MFMA layout's 4-row pieces of 16 columns. The arithmetic is
+// epilogue_store's.
+template <typename E, int BM, int BN, int WN>
+constexpr int epilogue_pass_cols(size_t lds_bytes_) {
+  int pw = BN;
+  while (pw > WN && static_cast<size_t>(BM + 1) * pw * sizeof(E) > lds_bytes_) pw /= 2;
+  return pw;
+}
+
+// lanes write the sums of the wave tiles whose columns lie in [p0, p0 + PW); value(i,
+// j, r) is the wave's element r of MFMA tile (i, j)
+template <typename E, typename S, int BM, int TM, int TN, typename Value>
+__device__ __forceinline__ void epilogue_put(E *sumt, int p0, int lane, int wm0, int wn0,
+                                             Value value) {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sumt[(wn0 - p0 + j * 16 + (lane & 15)) * (BM + 1) + wm0 + i * 16 + Mfma<S>::row(lane, r)] =
+            value(i, j, r);
+}
+
+template <typename E, int BM, int PW, int NT>
+__device__ __forceinline__ void epilogue_drain(const GemmArgs<E> &g, const E *sumt, int m0,
+                                               int c0) {
+  for (int e = threadIdx.x; e < BM * PW; e += NT) {
+    const int r = e % BM, c = e / BM;
+    const int row = m0 + r, col = c0 + c;
+    if (row < g.M && col < g.N) epilogue_store(g, row, col, sumt[c * (BM + 1) + r]);
+  }
+}
+
+// C = A*B (+ epilogue of g) over the whole n x n output, tile by tile, by all
+// 256 lanes; ends with a barrier, so C is complete for every lane afterwards.
 }  // namespace tile
 }  // namespace clm

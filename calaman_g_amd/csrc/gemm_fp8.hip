@@ -249,17 +249,28 @@ __global__ void __launch_bounds__(512)
       __syncthreads();
     }
   }
+  // the epilogue through LDS, one 128-row half of the tile at a time: the four waves
+  // holding the half lay their sums down column-major, then every wave writes C (and
+  // reads Cin / the power blocks) in whole column pieces (tile::lds_tile_epilogue)
+  constexpr int kSumLd = 132;
+  static_assert(128 * kSumLd * 4 <= 2 * kStageW, "a half tile's sums fit in the stages");
+  float *sum = reinterpret_cast<float *>(L);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // the stages (or the previous half) are no longer read
+    if ((wm0 >> 7) == h) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-        const int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (kFast || (row < g.M && col < g.N))
-          tile::epilogue_store(g, row, col, static_cast<double>(acc[i][j][r]));
-      }
+        for (int j = 0; j < 4; ++j)
+          *reinterpret_cast<f4 *>(&sum[(wn0 + j * 16 + (lane & 15)) * kSumLd + (wm0 & 127) + i * 16 +
+                                       4 * (lane >> 4)]) = acc[i][j];
+    }
+    __syncthreads();
+    tile::lds_tile_epilogue<double, 8, kFast>(g, m0 + 128 * h, n0, [&](int c, int r) {
+      return static_cast<double>(sum[c * kSumLd + r]);
+    });
+  }
 }
 
 inline bool aligned16_8(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }

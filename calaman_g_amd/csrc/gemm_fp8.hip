@@ -267,7 +267,8 @@ __global__ void __launch_bounds__(512)
                                        4 * (lane >> 4)]) = acc[i][j];
     }
     __syncthreads();
-    tile::lds_tile_epilogue<double, 8, kFast>(g, m0 + 128 * h, n0, [&](int c, int r) {
+    // (four columns in flight per lane: the other half's accumulators are still live)
+    tile::lds_tile_epilogue<double, 8, kFast, 4>(g, m0 + 128 * h, n0, [&](int c, int r) {
       return static_cast<double>(sum[c * kSumLd + r]);
     });
   }

@@ -314,13 +314,13 @@ __device__ __forceinline__ bool epilogue_vec2_ok(const GemmArgs<S> &g) {
          (g.nterms < 3 || ((g.ldbank & 1) == 0 && (g.bank_stride & 1) == 0 && al(g.bank)));
 }
 
-template <typename S, int NW, bool kFast, typename Fetch>
+template <typename S, int NW, bool kFast, int MAXCH = 8, typename Fetch>
 __device__ __forceinline__ void lds_tile_epilogue(const GemmArgs<S> &g, int m0, int n0,
                                                   Fetch fetch) {
   using V2 = typename Vec2<S>::type;
   using O = ops<S>;
-  constexpr int CPW = 128 / NW;            // columns per wave
-  constexpr int CH = CPW < 8 ? CPW : 8;    // columns per chunk of reads in flight
+  constexpr int CPW = 128 / NW;                // columns per wave
+  constexpr int CH = CPW < MAXCH ? CPW : MAXCH;  // columns per chunk of reads in flight
   static_assert(128 % NW == 0 && CPW % CH == 0, "column chunks");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rl = 2 * lane, row = m0 + rl;

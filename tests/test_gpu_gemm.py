@@ -88,3 +88,22 @@ def test_gemm_rejects_bad_leading_dimension(dev):
     ctx = Context()
     x = torch.zeros(16, dtype=torch.float64, device=dev)
     assert gemm_cm(ctx, 4, 4, 4, 1.0, x, 3, x, 4, 0.0, x, 4) == cg._lib.INVALID_VALUE
+
+
+@pytest.mark.parametrize("m,n,k,ldc_pad", [(2048, 2048, 512, 2), (2048, 2048, 512, 1),
+                                           (2050, 2049, 300, 3)])
+def test_gemm_large_tiles_alpha_beta(dev, m, n, k, ldc_pad):
+    """the 128 x 128 tiles' LDS-staged epilogue: 16-byte pieces (even ldc), element
+    stores (odd ldc) and ragged edges, with alpha, beta and untouched padding"""
+    rng = np.random.default_rng(m + ldc_pad)
+    a, b, c = _rand(rng, (m, k), False), _rand(rng, (k, n), False), _rand(rng, (m, n), False)
+    lda, ldb, ldc = m + 2, k + 2, m + ldc_pad
+    ctx = Context()
+    da, db, dc = _cm_tensor(a, lda, dev), _cm_tensor(b, ldb, dev), _cm_tensor(c, ldc, dev)
+    assert gemm_cm(ctx, m, n, k, 1.5, da, lda, db, ldb, -0.5, dc, ldc) == 0
+    torch.cuda.synchronize()
+    full = dc.cpu().numpy()
+    ref = 1.5 * (a @ b) - 0.5 * c
+    np.testing.assert_allclose(full[:n, :m].T, ref, rtol=0,
+                               atol=4 * _tol(a, b) + 1e-14 * np.abs(c).max())
+    assert np.all(full[:n, m:ldc] == -777.0)

@@ -281,6 +281,18 @@ def batched_leg(dev, steps, warmup):
             "graph": True, "spot_check_bitwise_mismatches": mism}
 
 
+def make_adjoint_inputs(b=512, n=128, t=0.5, seed=5):
+    """The differentiated workload's inputs (complex128 numpy): generators
+    A = -i t H with H = (G + G^H)/2, G complex Gaussian, and unitary C (QR of a
+    complex Gaussian), b of each."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+    h = (g + np.conj(np.transpose(g, (0, 2, 1)))) / 2
+    q, _ = np.linalg.qr(rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n)))
+    return -1j * t * h, q
+
+
 def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
     """Forward + hand-written backward of L = Re tr(C^H exp(A)) over 512 complex64
     generators A = -i t H, H = (G + G^H)/2 (n = 128), C unitary (QR of a Gaussian)."""
@@ -288,11 +300,8 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
     import torch
     from calaman_g_amd.expm import expm_batched_autograd, BatchPlan
 
-    rng = np.random.default_rng(seed)
-    g = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
-    h = (g + np.conj(np.transpose(g, (0, 2, 1)))) / 2
-    a = torch.from_numpy((-1j * t * h).astype(np.complex64)).to(dev)
-    q, _ = np.linalg.qr(rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n)))
+    a_np, q = make_adjoint_inputs(b, n, t, seed)
+    a = torch.from_numpy(a_np.astype(np.complex64)).to(dev)
     c = torch.from_numpy(q.astype(np.complex64)).to(dev)
 
     def step():
@@ -318,7 +327,7 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
     from calaman_g_amd.expm import expm_plan
     from calaman_g_amd import _lib
     import calaman_g_amd as cg
-    norms = [float(np.abs(m).sum(0).max()) for m in (-1j * t * h)]
+    norms = [float(np.abs(m).sum(0).max()) for m in a_np]
     flop = 0.0
     for nm in norms:
         p = expm_plan(nm, torch.complex64)
@@ -340,6 +349,19 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
 
 FP8_N = 4096
 FP8_NORMS = (1.0, 16.0, 256.0)
+
+
+def make_fp8_inputs(n=FP8_N):
+    """The mixed-precision workload: real N(0,1)/sqrt(n) matrices from seeds 10-12,
+    rescaled to ||A||_1 in FP8_NORMS."""
+    import numpy as np
+    mats = []
+    for i, t in enumerate(FP8_NORMS):
+        rng = np.random.default_rng(SEED + 10 + i)
+        a = rng.standard_normal((n, n)) / np.sqrt(n)
+        a *= t / np.abs(a).sum(0).max()
+        mats.append(a)
+    return mats
 
 
 def fp8tail_leg(dev, steps, warmup):
@@ -375,10 +397,7 @@ def fp8tail_leg(dev, steps, warmup):
         return e0.elapsed_time(e1) / reps
 
     out = {}
-    for i, t in enumerate(FP8_NORMS):
-        rng = np.random.default_rng(SEED + 10 + i)
-        a = rng.standard_normal((n, n)) / np.sqrt(n)
-        a *= t / np.abs(a).sum(0).max()
+    for t, a in zip(FP8_NORMS, make_fp8_inputs()):
         da = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
         ref = torch.empty_like(da)
         e = torch.empty_like(da)

@@ -21,7 +21,8 @@
  *   - Horner: src/horner/interface.cppm:156-194, src/horner/horner.cu:39-91
  * The GEMM and LU the reference takes from rocBLAS/rocSOLVER are restated as a
  * deterministic blocked GEMM (every element sums k ascending, whatever the
- * thread count) and an unblocked partial-pivot LU with ?getf2's pivot rule.
+ * thread count) and LAPACK's blocked right-looking partial-pivot LU (?getrf
+ * over ?getf2 panels, with ?getf2's pivot rule) and blocked ?getrs.
  *
  * The Taylor section at the end is the build's own algorithm (truncated
  * Taylor series evaluated by Paterson-Stockmeyer, then squared), stated on
@@ -516,200 +517,372 @@ int orc_lascl_c(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda
 }
 
 /* =========================================================================
- * Deterministic GEMM. Parallel over row strips; every element of C is one
- * running sum over k = 0..n-1 in ascending order, so the result does not
- * depend on the thread count.
+ * Deterministic GEMM. Every element of C is one running sum over k in
+ * ascending order, started at zero and advanced by one fused multiply-add per
+ * k, then alpha * sum (+ beta * C): the result does not depend on the thread
+ * count or the blocking. Operands are packed once into k-contiguous strips
+ * (8 rows of A, 6 columns of B) and multiplied by an AVX2 FMA micro-kernel
+ * holding an 8 x 6 block of sums in twelve vector registers; the k range is
+ * walked in blocks of GK_KC whose partial sums are kept, in double, in a
+ * per-thread tile buffer, which leaves every element's FMA sequence unchanged.
  * ========================================================================= */
-#define GM_MB 64 /* rows per packed strip */
-#define GM_MR 8  /* micro-tile rows */
-#define GM_NR 4  /* micro-tile columns */
+#include <immintrin.h>
+
+#define GK_MR 8     /* rows per micro-tile (two 4-double vectors) */
+#define GK_NR 6     /* columns per micro-tile */
+#define GK_KC 256   /* k block */
+#define GK_MS 8     /* micro row strips per thread tile (64 rows) */
+#define GK_NS 16    /* micro column strips per thread tile (96 columns) */
+
+/* acc[c*8 + r] += sum_k a[k*8 + r] * b[k*6 + c], k ascending, one FMA per k */
+static void kernel_d_8x6(int kc, const double *a, const double *b, double *acc, int first) {
+  __m256d c0l, c0h, c1l, c1h, c2l, c2h, c3l, c3h, c4l, c4h, c5l, c5h;
+  if (first) {
+    c0l = c0h = c1l = c1h = c2l = c2h = c3l = c3h = c4l = c4h = c5l = c5h = _mm256_setzero_pd();
+  } else {
+    c0l = _mm256_loadu_pd(acc + 0); c0h = _mm256_loadu_pd(acc + 4);
+    c1l = _mm256_loadu_pd(acc + 8); c1h = _mm256_loadu_pd(acc + 12);
+    c2l = _mm256_loadu_pd(acc + 16); c2h = _mm256_loadu_pd(acc + 20);
+    c3l = _mm256_loadu_pd(acc + 24); c3h = _mm256_loadu_pd(acc + 28);
+    c4l = _mm256_loadu_pd(acc + 32); c4h = _mm256_loadu_pd(acc + 36);
+    c5l = _mm256_loadu_pd(acc + 40); c5h = _mm256_loadu_pd(acc + 44);
+  }
+  for (int k = 0; k < kc; ++k) {
+    const __m256d al = _mm256_loadu_pd(a + 8 * k), ah = _mm256_loadu_pd(a + 8 * k + 4);
+    const double *bk = b + 6 * k;
+    __m256d bb = _mm256_broadcast_sd(bk + 0);
+    c0l = _mm256_fmadd_pd(al, bb, c0l); c0h = _mm256_fmadd_pd(ah, bb, c0h);
+    bb = _mm256_broadcast_sd(bk + 1);
+    c1l = _mm256_fmadd_pd(al, bb, c1l); c1h = _mm256_fmadd_pd(ah, bb, c1h);
+    bb = _mm256_broadcast_sd(bk + 2);
+    c2l = _mm256_fmadd_pd(al, bb, c2l); c2h = _mm256_fmadd_pd(ah, bb, c2h);
+    bb = _mm256_broadcast_sd(bk + 3);
+    c3l = _mm256_fmadd_pd(al, bb, c3l); c3h = _mm256_fmadd_pd(ah, bb, c3h);
+    bb = _mm256_broadcast_sd(bk + 4);
+    c4l = _mm256_fmadd_pd(al, bb, c4l); c4h = _mm256_fmadd_pd(ah, bb, c4h);
+    bb = _mm256_broadcast_sd(bk + 5);
+    c5l = _mm256_fmadd_pd(al, bb, c5l); c5h = _mm256_fmadd_pd(ah, bb, c5h);
+  }
+  _mm256_storeu_pd(acc + 0, c0l); _mm256_storeu_pd(acc + 4, c0h);
+  _mm256_storeu_pd(acc + 8, c1l); _mm256_storeu_pd(acc + 12, c1h);
+  _mm256_storeu_pd(acc + 16, c2l); _mm256_storeu_pd(acc + 20, c2h);
+  _mm256_storeu_pd(acc + 24, c3l); _mm256_storeu_pd(acc + 28, c3h);
+  _mm256_storeu_pd(acc + 32, c4l); _mm256_storeu_pd(acc + 36, c4h);
+  _mm256_storeu_pd(acc + 40, c5l); _mm256_storeu_pd(acc + 44, c5h);
+}
+
+/* C (m x n) = alpha * A (m x k) * B (k x n) + beta * C, column-major */
+static void gemm_d(int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                   int ldb, double beta, double *C, int ldc) {
+  if (m < 1 || n < 1) return;
+  const int ms = (m + GK_MR - 1) / GK_MR, ns = (n + GK_NR - 1) / GK_NR;
+  const int kk = k < 1 ? 1 : k;
+  double *Ap = (double *)aligned_alloc(64, sizeof(double) * (size_t)ms * GK_MR * kk);
+  double *Bp = (double *)aligned_alloc(64, sizeof(double) * (size_t)ns * GK_NR * kk);
+#pragma omp parallel
+  {
+#pragma omp for schedule(static)
+    for (int si = 0; si < ms; ++si) {
+      double *dst = Ap + (size_t)si * GK_MR * kk;
+      const int i0 = si * GK_MR;
+      for (int p = 0; p < k; ++p) {
+        const double *src = A + (size_t)p * lda + i0;
+        for (int r = 0; r < GK_MR; ++r) dst[(size_t)p * GK_MR + r] = (i0 + r < m) ? src[r] : 0.0;
+      }
+    }
+#pragma omp for schedule(static)
+    for (int sj = 0; sj < ns; ++sj) {
+      double *dst = Bp + (size_t)sj * GK_NR * kk;
+      const int j0 = sj * GK_NR;
+      for (int c = 0; c < GK_NR; ++c) {
+        const double *src = B + (size_t)(j0 + c) * ldb;
+        for (int p = 0; p < k; ++p) dst[(size_t)p * GK_NR + c] = (j0 + c < n) ? src[p] : 0.0;
+      }
+    }
+    double *tile = (double *)aligned_alloc(64, sizeof(double) * GK_MS * GK_NS * GK_MR * GK_NR);
+    const int tm = (ms + GK_MS - 1) / GK_MS, tn = (ns + GK_NS - 1) / GK_NS;
+#pragma omp for collapse(2) schedule(dynamic, 1)
+    for (int ti = 0; ti < tm; ++ti) {
+      for (int tj = 0; tj < tn; ++tj) {
+        const int si0 = ti * GK_MS, sj0 = tj * GK_NS;
+        const int nsi = MIN(GK_MS, ms - si0), nsj = MIN(GK_NS, ns - sj0);
+        if (k < 1) {
+          memset(tile, 0, sizeof(double) * GK_MS * GK_NS * GK_MR * GK_NR);
+        }
+        for (int p0 = 0; p0 < k; p0 += GK_KC) {
+          const int kc = MIN(GK_KC, k - p0);
+          for (int b = 0; b < nsj; ++b) {
+            const double *bp = Bp + (size_t)(sj0 + b) * GK_NR * kk + (size_t)p0 * GK_NR;
+            for (int a = 0; a < nsi; ++a) {
+              const double *ap = Ap + (size_t)(si0 + a) * GK_MR * kk + (size_t)p0 * GK_MR;
+              kernel_d_8x6(kc, ap, bp, tile + (size_t)(b * GK_MS + a) * GK_MR * GK_NR, p0 == 0);
+            }
+          }
+        }
+        for (int b = 0; b < nsj; ++b) {
+          for (int c = 0; c < GK_NR; ++c) {
+            const int j = (sj0 + b) * GK_NR + c;
+            if (j >= n) break;
+            for (int a = 0; a < nsi; ++a) {
+              const double *acc = tile + (size_t)(b * GK_MS + a) * GK_MR * GK_NR + c * GK_MR;
+              const int i0 = (si0 + a) * GK_MR;
+              double *cc = C + (size_t)j * ldc + i0;
+              const int rr = MIN(GK_MR, m - i0);
+              for (int r = 0; r < rr; ++r) {
+                const double v = alpha * acc[r];
+                cc[r] = (beta == 0.0) ? v : fma(beta, cc[r], v);
+              }
+            }
+          }
+        }
+      }
+    }
+    free(tile);
+  }
+  free(Ap);
+  free(Bp);
+}
 
 void orc_gemm_d(int n, double alpha, const double *A, int lda, const double *B, int ldb,
                 double beta, double *C, int ldc) {
-  if (n < 1) return;
-  const int nstrips = (n + GM_MB - 1) / GM_MB;
-  double *zero_col = (double *)calloc((size_t)n, sizeof(double));
-#pragma omp parallel
-  {
-    double *Ap = (double *)aligned_alloc(64, sizeof(double) * (size_t)GM_MB * (size_t)n);
-#pragma omp for schedule(dynamic, 1)
-    for (int sb = 0; sb < nstrips; ++sb) {
-      const int i0 = sb * GM_MB;
-      const int mb = MIN(GM_MB, n - i0);
-      /* micro-panel packing: rows r0..r0+MR-1 of the strip are contiguous
-       * over k, so the micro-kernel streams one sequential buffer */
-      for (int r0 = 0; r0 < GM_MB; r0 += GM_MR) {
-        double *dst = Ap + (size_t)r0 * n;
-        for (int k = 0; k < n; ++k) {
-          const double *src = A + (size_t)k * lda + i0 + r0;
-          for (int r = 0; r < GM_MR; ++r) dst[(size_t)k * GM_MR + r] = (r0 + r < mb) ? src[r] : 0.0;
-        }
-      }
-      for (int j0 = 0; j0 < n; j0 += GM_NR) {
-        const int nr = MIN(GM_NR, n - j0);
-        const double *bc[GM_NR];
-        for (int c = 0; c < GM_NR; ++c) bc[c] = (c < nr) ? B + (size_t)(j0 + c) * ldb : zero_col;
-        for (int r0 = 0; r0 < mb; r0 += GM_MR) {
-          double acc[GM_NR][GM_MR];
-          for (int c = 0; c < GM_NR; ++c)
-            for (int r = 0; r < GM_MR; ++r) acc[c][r] = 0.0;
-          const double *ap = Ap + (size_t)r0 * n;
-          for (int k = 0; k < n; ++k) {
-            const double *a = ap + (size_t)k * GM_MR;
-#pragma GCC unroll 4
-            for (int c = 0; c < GM_NR; ++c) {
-              const double b = bc[c][k];
-#pragma GCC unroll 8
-              for (int r = 0; r < GM_MR; ++r) acc[c][r] += a[r] * b;
-            }
-          }
-          const int rr = MIN(GM_MR, mb - r0);
-          for (int c = 0; c < nr; ++c) {
-            double *cc = C + (size_t)(j0 + c) * ldc + i0 + r0;
-            for (int r = 0; r < rr; ++r) {
-              const double v = alpha * acc[c][r];
-              cc[r] = (beta == 0.0) ? v : v + beta * cc[r];
-            }
-          }
-        }
-      }
-    }
-    free(Ap);
-  }
-  free(zero_col);
+  gemm_d(n, n, n, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-/* complex: interleaved; (a+bi)(c+di) = (ac - bd) + (ad + bc)i per term */
+/* complex, interleaved: per k, re += ar*br then re -= ai*bi, im += ar*bi then
+ * im += ai*br (four fused multiply-adds), k ascending from zero; strips of 4
+ * rows (re and im planes) and 6 columns */
 #define GZ_MR 4
-#define GZ_NR 4
-void orc_gemm_z(int n, const double *alpha, const double *A, int lda, const double *B, int ldb,
-                const double *beta, double *C, int ldc) {
-  if (n < 1) return;
-  const int nstrips = (n + GM_MB - 1) / GM_MB;
-  double *zero_col = (double *)calloc(2 * (size_t)n, sizeof(double));
+#define GZ_NR 6
+#define GZ_MS 16 /* 64 rows per thread tile */
+#define GZ_NS 8  /* 48 columns per thread tile */
+
+/* a: per k {re[4], im[4]}; b: per k {re[6], im[6]}; acc: [c][re4, im4] */
+static void kernel_z_4x6(int kc, const double *a, const double *b, double *acc, int first) {
+  __m256d r[6], i[6];
+  for (int c = 0; c < 6; ++c) {
+    r[c] = first ? _mm256_setzero_pd() : _mm256_loadu_pd(acc + 8 * c);
+    i[c] = first ? _mm256_setzero_pd() : _mm256_loadu_pd(acc + 8 * c + 4);
+  }
+  for (int k = 0; k < kc; ++k) {
+    const __m256d ar = _mm256_loadu_pd(a + 8 * k), ai = _mm256_loadu_pd(a + 8 * k + 4);
+    const double *bk = b + 12 * k;
+#pragma GCC unroll 6
+    for (int c = 0; c < 6; ++c) {
+      const __m256d br = _mm256_broadcast_sd(bk + c), bi = _mm256_broadcast_sd(bk + 6 + c);
+      r[c] = _mm256_fmadd_pd(ar, br, r[c]);
+      r[c] = _mm256_fnmadd_pd(ai, bi, r[c]);
+      i[c] = _mm256_fmadd_pd(ar, bi, i[c]);
+      i[c] = _mm256_fmadd_pd(ai, br, i[c]);
+    }
+  }
+  for (int c = 0; c < 6; ++c) {
+    _mm256_storeu_pd(acc + 8 * c, r[c]);
+    _mm256_storeu_pd(acc + 8 * c + 4, i[c]);
+  }
+}
+
+static void gemm_z(int m, int n, int k, const double *alpha, const double *A, int lda,
+                   const double *B, int ldb, const double *beta, double *C, int ldc) {
+  if (m < 1 || n < 1) return;
+  const int ms = (m + GZ_MR - 1) / GZ_MR, ns = (n + GZ_NR - 1) / GZ_NR;
+  const int kk = k < 1 ? 1 : k;
   const int beta_zero = (beta[0] == 0.0 && beta[1] == 0.0);
+  double *Ap = (double *)aligned_alloc(64, sizeof(double) * (size_t)ms * 2 * GZ_MR * kk);
+  double *Bp = (double *)aligned_alloc(64, sizeof(double) * (size_t)ns * 2 * GZ_NR * kk);
 #pragma omp parallel
   {
-    /* packed strip: Apr/Api[k*MB + ii] */
-    double *Apr = (double *)aligned_alloc(64, sizeof(double) * (size_t)GM_MB * (size_t)n);
-    double *Api = (double *)aligned_alloc(64, sizeof(double) * (size_t)GM_MB * (size_t)n);
-#pragma omp for schedule(dynamic, 1)
-    for (int sb = 0; sb < nstrips; ++sb) {
-      const int i0 = sb * GM_MB;
-      const int mb = MIN(GM_MB, n - i0);
-      for (int k = 0; k < n; ++k) {
-        const double *src = A + 2 * ((size_t)k * lda + i0);
-        for (int ii = 0; ii < GM_MB; ++ii) {
-          Apr[(size_t)k * GM_MB + ii] = (ii < mb) ? src[2 * ii] : 0.0;
-          Api[(size_t)k * GM_MB + ii] = (ii < mb) ? src[2 * ii + 1] : 0.0;
+#pragma omp for schedule(static)
+    for (int si = 0; si < ms; ++si) {
+      double *dst = Ap + (size_t)si * 2 * GZ_MR * kk;
+      const int i0 = si * GZ_MR;
+      for (int p = 0; p < k; ++p) {
+        const double *src = A + 2 * ((size_t)p * lda + i0);
+        for (int q = 0; q < GZ_MR; ++q) {
+          const int ok = i0 + q < m;
+          dst[(size_t)p * 8 + q] = ok ? src[2 * q] : 0.0;
+          dst[(size_t)p * 8 + 4 + q] = ok ? src[2 * q + 1] : 0.0;
         }
       }
-      for (int j0 = 0; j0 < n; j0 += GZ_NR) {
-        const int nr = MIN(GZ_NR, n - j0);
-        const double *bc[GZ_NR];
-        for (int c = 0; c < GZ_NR; ++c)
-          bc[c] = (c < nr) ? B + 2 * (size_t)(j0 + c) * ldb : zero_col;
-        for (int r0 = 0; r0 < mb; r0 += GZ_MR) {
-          double re[GZ_NR][GZ_MR], im[GZ_NR][GZ_MR];
-          for (int c = 0; c < GZ_NR; ++c)
-            for (int r = 0; r < GZ_MR; ++r) re[c][r] = im[c][r] = 0.0;
-          for (int k = 0; k < n; ++k) {
-            const double *ar = Apr + (size_t)k * GM_MB + r0;
-            const double *ai = Api + (size_t)k * GM_MB + r0;
-#pragma GCC unroll 4
-            for (int c = 0; c < GZ_NR; ++c) {
-              const double br = bc[c][2 * k], bi = bc[c][2 * k + 1];
-#pragma GCC unroll 4
-              for (int r = 0; r < GZ_MR; ++r) {
-                re[c][r] += ar[r] * br - ai[r] * bi;
-                im[c][r] += ar[r] * bi + ai[r] * br;
-              }
+    }
+#pragma omp for schedule(static)
+    for (int sj = 0; sj < ns; ++sj) {
+      double *dst = Bp + (size_t)sj * 2 * GZ_NR * kk;
+      const int j0 = sj * GZ_NR;
+      for (int c = 0; c < GZ_NR; ++c) {
+        const double *src = B + 2 * (size_t)(j0 + c) * ldb;
+        const int ok = j0 + c < n;
+        for (int p = 0; p < k; ++p) {
+          dst[(size_t)p * 12 + c] = ok ? src[2 * p] : 0.0;
+          dst[(size_t)p * 12 + 6 + c] = ok ? src[2 * p + 1] : 0.0;
+        }
+      }
+    }
+    const size_t tile_elems = (size_t)GZ_MS * GZ_NS * 2 * GZ_MR * GZ_NR;
+    double *tile = (double *)aligned_alloc(64, sizeof(double) * tile_elems);
+    const int tm = (ms + GZ_MS - 1) / GZ_MS, tn = (ns + GZ_NS - 1) / GZ_NS;
+#pragma omp for collapse(2) schedule(dynamic, 1)
+    for (int ti = 0; ti < tm; ++ti) {
+      for (int tj = 0; tj < tn; ++tj) {
+        const int si0 = ti * GZ_MS, sj0 = tj * GZ_NS;
+        const int nsi = MIN(GZ_MS, ms - si0), nsj = MIN(GZ_NS, ns - sj0);
+        if (k < 1) memset(tile, 0, sizeof(double) * tile_elems);
+        for (int p0 = 0; p0 < k; p0 += GK_KC) {
+          const int kc = MIN(GK_KC, k - p0);
+          for (int b = 0; b < nsj; ++b) {
+            const double *bp = Bp + (size_t)(sj0 + b) * 2 * GZ_NR * kk + (size_t)p0 * 12;
+            for (int a = 0; a < nsi; ++a) {
+              const double *ap = Ap + (size_t)(si0 + a) * 2 * GZ_MR * kk + (size_t)p0 * 8;
+              kernel_z_4x6(kc, ap, bp, tile + (size_t)(b * GZ_MS + a) * 2 * GZ_MR * GZ_NR,
+                           p0 == 0);
             }
           }
-          const int rr = MIN(GZ_MR, mb - r0);
-          for (int c = 0; c < nr; ++c) {
-            double *cc = C + 2 * ((size_t)(j0 + c) * ldc + i0 + r0);
-            for (int r = 0; r < rr; ++r) {
-              double vr = alpha[0] * re[c][r] - alpha[1] * im[c][r];
-              double vi = alpha[0] * im[c][r] + alpha[1] * re[c][r];
-              if (!beta_zero) {
-                const double cr = cc[2 * r], ci = cc[2 * r + 1];
-                vr = vr + (beta[0] * cr - beta[1] * ci);
-                vi = vi + (beta[0] * ci + beta[1] * cr);
+        }
+        for (int b = 0; b < nsj; ++b) {
+          for (int c = 0; c < GZ_NR; ++c) {
+            const int j = (sj0 + b) * GZ_NR + c;
+            if (j >= n) break;
+            for (int a = 0; a < nsi; ++a) {
+              const double *acc = tile + (size_t)(b * GZ_MS + a) * 2 * GZ_MR * GZ_NR + 8 * c;
+              const int i0 = (si0 + a) * GZ_MR;
+              double *cc = C + 2 * ((size_t)j * ldc + i0);
+              const int rr = MIN(GZ_MR, m - i0);
+              for (int q = 0; q < rr; ++q) {
+                const double sr = acc[q], si = acc[4 + q];
+                double vr = alpha[0] * sr - alpha[1] * si;
+                double vi = alpha[0] * si + alpha[1] * sr;
+                if (!beta_zero) {
+                  const double cr = cc[2 * q], ci = cc[2 * q + 1];
+                  vr = vr + (beta[0] * cr - beta[1] * ci);
+                  vi = vi + (beta[0] * ci + beta[1] * cr);
+                }
+                cc[2 * q] = vr;
+                cc[2 * q + 1] = vi;
               }
-              cc[2 * r] = vr;
-              cc[2 * r + 1] = vi;
             }
           }
         }
       }
     }
-    free(Apr);
-    free(Api);
+    free(tile);
   }
-  free(zero_col);
+  free(Ap);
+  free(Bp);
+}
+
+void orc_gemm_z(int n, const double *alpha, const double *A, int lda, const double *B, int ldb,
+                const double *beta, double *C, int ldc) {
+  gemm_z(n, n, n, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 /* =========================================================================
- * LU: ?getf2 semantics (pivot = first index of the largest |.|, or
- * |re|+|im| for complex as izamax; row swaps across all columns; the
- * column below the pivot scaled by its reciprocal; rank-1 update).
+ * LU: LAPACK ?getrf's right-looking blocked algorithm (panels of LU_NB
+ * columns factored by ?getf2, row interchanges applied across the whole
+ * matrix, U12 = L11^-1 A12, A22 -= L21 U12 on the GEMM above). ?getf2's
+ * rules: pivot = first index of the largest |.| (|re|+|im| for complex, as
+ * izamax), the column below the pivot scaled by its reciprocal (divided when
+ * the pivot is below the safe minimum), rank-1 update of the panel.
  * ========================================================================= */
-int orc_getrf_d(int n, double *A, int lda, int *ipiv) {
-  int info = 0;
+#define LU_NB 64
+
+/* the j-th step of ?getf2 on rows j..m-1 of columns c0..c1-1 (all inside one
+ * panel); returns 1 for an exactly zero pivot */
+static int getf2_panel_d(int m, int c0, int c1, double *A, int lda, int *ipiv) {
+  int zero = 0;
   const double sfmin = DBL_MIN;
-  for (int j = 0; j < n; ++j) {
+  for (int j = c0; j < c1; ++j) {
     double *cj = A + (size_t)j * lda;
     int jp = j;
     double best = fabs(cj[j]);
-    for (int i = j + 1; i < n; ++i) {
+    for (int i = j + 1; i < m; ++i) {
       const double v = fabs(cj[i]);
       if (v > best) { best = v; jp = i; }
     }
     ipiv[j] = jp + 1;
     if (cj[jp] != 0.0) {
       if (jp != j) {
-        for (int c = 0; c < n; ++c) {
+        for (int c = c0; c < c1; ++c) {
           double *col = A + (size_t)c * lda;
           const double t = col[j]; col[j] = col[jp]; col[jp] = t;
         }
       }
       if (fabs(cj[j]) >= sfmin) {
         const double r = 1.0 / cj[j];
-        for (int i = j + 1; i < n; ++i) cj[i] *= r;
+        for (int i = j + 1; i < m; ++i) cj[i] *= r;
       } else {
-        for (int i = j + 1; i < n; ++i) cj[i] /= cj[j];
+        for (int i = j + 1; i < m; ++i) cj[i] /= cj[j];
       }
-    } else if (info == 0) {
-      info = j + 1;
+    } else if (!zero) {
+      zero = j + 1;
     }
-#pragma omp parallel for schedule(static) if (n - j > 64)
-    for (int c = j + 1; c < n; ++c) {
+#pragma omp parallel for schedule(static) if ((m - j) * (c1 - j) > 65536)
+    for (int c = j + 1; c < c1; ++c) {
       double *col = A + (size_t)c * lda;
       const double u = col[j];
       if (u != 0.0) {
-        for (int i = j + 1; i < n; ++i) col[i] -= cj[i] * u;
+        for (int i = j + 1; i < m; ++i) col[i] -= cj[i] * u;
       }
+    }
+  }
+  return zero;
+}
+
+/* rows j0..j1-1 of ipiv applied to columns [c0, c1) */
+static void swap_rows_d(double *A, int lda, int c0, int c1, int j0, int j1, const int *ipiv) {
+#pragma omp parallel for schedule(static) if ((c1 - c0) > 64)
+  for (int c = c0; c < c1; ++c) {
+    double *col = A + (size_t)c * lda;
+    for (int j = j0; j < j1; ++j) {
+      const int p = ipiv[j] - 1;
+      if (p != j) { const double t = col[j]; col[j] = col[p]; col[p] = t; }
+    }
+  }
+}
+
+int orc_getrf_d(int n, double *A, int lda, int *ipiv) {
+  int info = 0;
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int jb = MIN(LU_NB, n - j0);
+    const int z = getf2_panel_d(n, j0, j0 + jb, A, lda, ipiv);
+    if (z && !info) info = z;
+    swap_rows_d(A, lda, 0, j0, j0, j0 + jb, ipiv);
+    if (j0 + jb < n) {
+      swap_rows_d(A, lda, j0 + jb, n, j0, j0 + jb, ipiv);
+      /* U12 = L11^-1 A12 (unit lower), column by column */
+#pragma omp parallel for schedule(static)
+      for (int c = j0 + jb; c < n; ++c) {
+        double *col = A + (size_t)c * lda;
+        for (int p = j0; p < j0 + jb; ++p) {
+          const double u = col[p];
+          if (u != 0.0) {
+            const double *l = A + (size_t)p * lda;
+            for (int i = p + 1; i < j0 + jb; ++i) col[i] -= l[i] * u;
+          }
+        }
+      }
+      const int rest = n - j0 - jb;
+      gemm_d(rest, rest, jb, -1.0, A + (size_t)j0 * lda + j0 + jb, lda,
+             A + (size_t)(j0 + jb) * lda + j0, lda, 1.0, A + (size_t)(j0 + jb) * lda + j0 + jb,
+             lda);
     }
   }
   return info;
 }
 
-int orc_getrf_z(int n, double *A, int lda, int *ipiv) {
-  int info = 0;
+static int getf2_panel_z(int m, int c0, int c1, double *A, int lda, int *ipiv) {
+  int zero = 0;
   const double sfmin = DBL_MIN;
-  for (int j = 0; j < n; ++j) {
+  for (int j = c0; j < c1; ++j) {
     double *cj = A + 2 * (size_t)j * lda;
     int jp = j;
     double best = fabs(cj[2 * j]) + fabs(cj[2 * j + 1]);
-    for (int i = j + 1; i < n; ++i) {
+    for (int i = j + 1; i < m; ++i) {
       const double v = fabs(cj[2 * i]) + fabs(cj[2 * i + 1]);
       if (v > best) { best = v; jp = i; }
     }
     ipiv[j] = jp + 1;
     if (cj[2 * jp] != 0.0 || cj[2 * jp + 1] != 0.0) {
       if (jp != j) {
-        for (int c = 0; c < n; ++c) {
+        for (int c = c0; c < c1; ++c) {
           double *col = A + 2 * (size_t)c * lda;
           for (int q = 0; q < 2; ++q) {
             const double t = col[2 * j + q]; col[2 * j + q] = col[2 * jp + q]; col[2 * jp + q] = t;
@@ -727,28 +900,28 @@ int orc_getrf_z(int n, double *A, int lda, int *ipiv) {
           const double t = pr / pi, d = pi + pr * t;
           rr = t / d; ri = -1.0 / d;
         }
-        for (int i = j + 1; i < n; ++i) {
+        for (int i = j + 1; i < m; ++i) {
           const double xr = cj[2 * i], xi = cj[2 * i + 1];
           cj[2 * i] = xr * rr - xi * ri;
           cj[2 * i + 1] = xr * ri + xi * rr;
         }
       } else {
-        for (int i = j + 1; i < n; ++i) {
+        for (int i = j + 1; i < m; ++i) {
           const double xr = cj[2 * i], xi = cj[2 * i + 1];
           const double d = pr * pr + pi * pi;
           cj[2 * i] = (xr * pr + xi * pi) / d;
           cj[2 * i + 1] = (xi * pr - xr * pi) / d;
         }
       }
-    } else if (info == 0) {
-      info = j + 1;
+    } else if (!zero) {
+      zero = j + 1;
     }
-#pragma omp parallel for schedule(static) if (n - j > 32)
-    for (int c = j + 1; c < n; ++c) {
+#pragma omp parallel for schedule(static) if ((m - j) * (c1 - j) > 32768)
+    for (int c = j + 1; c < c1; ++c) {
       double *col = A + 2 * (size_t)c * lda;
       const double ur = col[2 * j], ui = col[2 * j + 1];
       if (ur != 0.0 || ui != 0.0) {
-        for (int i = j + 1; i < n; ++i) {
+        for (int i = j + 1; i < m; ++i) {
           const double lr = cj[2 * i], li = cj[2 * i + 1];
           col[2 * i] -= lr * ur - li * ui;
           col[2 * i + 1] -= lr * ui + li * ur;
@@ -756,78 +929,154 @@ int orc_getrf_z(int n, double *A, int lda, int *ipiv) {
       }
     }
   }
+  return zero;
+}
+
+static void swap_rows_z(double *A, int lda, int c0, int c1, int j0, int j1, const int *ipiv) {
+#pragma omp parallel for schedule(static) if ((c1 - c0) > 64)
+  for (int c = c0; c < c1; ++c) {
+    double *col = A + 2 * (size_t)c * lda;
+    for (int j = j0; j < j1; ++j) {
+      const int p = ipiv[j] - 1;
+      if (p != j) {
+        for (int q = 0; q < 2; ++q) {
+          const double t = col[2 * j + q]; col[2 * j + q] = col[2 * p + q]; col[2 * p + q] = t;
+        }
+      }
+    }
+  }
+}
+
+int orc_getrf_z(int n, double *A, int lda, int *ipiv) {
+  int info = 0;
+  const double mone[2] = {-1.0, 0.0}, one[2] = {1.0, 0.0};
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int jb = MIN(LU_NB, n - j0);
+    const int z = getf2_panel_z(n, j0, j0 + jb, A, lda, ipiv);
+    if (z && !info) info = z;
+    swap_rows_z(A, lda, 0, j0, j0, j0 + jb, ipiv);
+    if (j0 + jb < n) {
+      swap_rows_z(A, lda, j0 + jb, n, j0, j0 + jb, ipiv);
+#pragma omp parallel for schedule(static)
+      for (int c = j0 + jb; c < n; ++c) {
+        double *col = A + 2 * (size_t)c * lda;
+        for (int p = j0; p < j0 + jb; ++p) {
+          const double ur = col[2 * p], ui = col[2 * p + 1];
+          if (ur != 0.0 || ui != 0.0) {
+            const double *l = A + 2 * (size_t)p * lda;
+            for (int i = p + 1; i < j0 + jb; ++i) {
+              const double lr = l[2 * i], li = l[2 * i + 1];
+              col[2 * i] -= lr * ur - li * ui;
+              col[2 * i + 1] -= lr * ui + li * ur;
+            }
+          }
+        }
+      }
+      const int rest = n - j0 - jb;
+      gemm_z(rest, rest, jb, mone, A + 2 * ((size_t)j0 * lda + j0 + jb), lda,
+             A + 2 * ((size_t)(j0 + jb) * lda + j0), lda, one,
+             A + 2 * ((size_t)(j0 + jb) * lda + j0 + jb), lda);
+    }
+  }
   return info;
 }
 
-/* ?getrs NoTrans: row interchanges, unit-lower forward solve, upper back
- * solve, column by column as the reference ?trsm (Left, NoTrans) loops do. */
+/* ?getrs NoTrans: row interchanges, then the unit-lower and the upper solve
+ * blocked as ?trsm (Left, NoTrans) is: a diagonal block solved column by
+ * column, the rest of the right-hand sides updated by one GEMM per block. */
 int orc_getrs_d(int n, int nrhs, const double *LU, int lda, const int *ipiv, double *B, int ldb) {
-#pragma omp parallel for schedule(dynamic, 4)
-  for (int c = 0; c < nrhs; ++c) {
-    double *b = B + (size_t)c * ldb;
-    for (int i = 0; i < n; ++i) {
-      const int p = ipiv[i] - 1;
-      if (p != i) { const double t = b[i]; b[i] = b[p]; b[p] = t; }
-    }
-    for (int k = 0; k < n; ++k) {
-      const double bk = b[k];
-      if (bk != 0.0) {
-        const double *l = LU + (size_t)k * lda;
-        for (int i = k + 1; i < n; ++i) b[i] -= bk * l[i];
-      }
-    }
-    for (int k = n - 1; k >= 0; --k) {
-      if (b[k] != 0.0) {
-        const double *u = LU + (size_t)k * lda;
-        b[k] = b[k] / u[k];
+  swap_rows_d(B, ldb, 0, nrhs, 0, n, ipiv);
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int kb = MIN(LU_NB, n - k0);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nrhs; ++c) {
+      double *b = B + (size_t)c * ldb;
+      for (int k = k0; k < k0 + kb; ++k) {
         const double bk = b[k];
-        for (int i = 0; i < k; ++i) b[i] -= bk * u[i];
+        if (bk != 0.0) {
+          const double *l = LU + (size_t)k * lda;
+          for (int i = k + 1; i < k0 + kb; ++i) b[i] -= bk * l[i];
+        }
       }
     }
+    if (k0 + kb < n)
+      gemm_d(n - k0 - kb, nrhs, kb, -1.0, LU + (size_t)k0 * lda + k0 + kb, lda, B + k0, ldb, 1.0,
+             B + k0 + kb, ldb);
+  }
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int k0 = bi * LU_NB, kb = MIN(LU_NB, n - k0);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nrhs; ++c) {
+      double *b = B + (size_t)c * ldb;
+      for (int k = k0 + kb - 1; k >= k0; --k) {
+        if (b[k] != 0.0) {
+          const double *u = LU + (size_t)k * lda;
+          b[k] = b[k] / u[k];
+          const double bk = b[k];
+          for (int i = k0; i < k; ++i) b[i] -= bk * u[i];
+        }
+      }
+    }
+    if (k0 > 0)
+      gemm_d(k0, nrhs, kb, -1.0, LU + (size_t)k0 * lda, lda, B + k0, ldb, 1.0, B, ldb);
   }
   return 0;
 }
 
 int orc_getrs_z(int n, int nrhs, const double *LU, int lda, const int *ipiv, double *B, int ldb) {
-#pragma omp parallel for schedule(dynamic, 4)
-  for (int c = 0; c < nrhs; ++c) {
-    double *b = B + 2 * (size_t)c * ldb;
-    for (int i = 0; i < n; ++i) {
-      const int p = ipiv[i] - 1;
-      if (p != i) {
-        for (int q = 0; q < 2; ++q) { const double t = b[2 * i + q]; b[2 * i + q] = b[2 * p + q]; b[2 * p + q] = t; }
-      }
-    }
-    for (int k = 0; k < n; ++k) {
-      const double br = b[2 * k], bi = b[2 * k + 1];
-      if (br != 0.0 || bi != 0.0) {
-        const double *l = LU + 2 * (size_t)k * lda;
-        for (int i = k + 1; i < n; ++i) {
-          b[2 * i] -= br * l[2 * i] - bi * l[2 * i + 1];
-          b[2 * i + 1] -= br * l[2 * i + 1] + bi * l[2 * i];
+  const double mone[2] = {-1.0, 0.0}, one[2] = {1.0, 0.0};
+  swap_rows_z(B, ldb, 0, nrhs, 0, n, ipiv);
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int kb = MIN(LU_NB, n - k0);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nrhs; ++c) {
+      double *b = B + 2 * (size_t)c * ldb;
+      for (int k = k0; k < k0 + kb; ++k) {
+        const double br = b[2 * k], bi = b[2 * k + 1];
+        if (br != 0.0 || bi != 0.0) {
+          const double *l = LU + 2 * (size_t)k * lda;
+          for (int i = k + 1; i < k0 + kb; ++i) {
+            b[2 * i] -= br * l[2 * i] - bi * l[2 * i + 1];
+            b[2 * i + 1] -= br * l[2 * i + 1] + bi * l[2 * i];
+          }
         }
       }
     }
-    for (int k = n - 1; k >= 0; --k) {
-      if (b[2 * k] != 0.0 || b[2 * k + 1] != 0.0) {
-        const double *u = LU + 2 * (size_t)k * lda;
-        /* b[k] /= u[k] by Smith's method */
-        const double ur = u[2 * k], ui = u[2 * k + 1], xr = b[2 * k], xi = b[2 * k + 1];
-        double qr, qi;
-        if (fabs(ur) >= fabs(ui)) {
-          const double t = ui / ur, d = ur + ui * t;
-          qr = (xr + xi * t) / d; qi = (xi - xr * t) / d;
-        } else {
-          const double t = ur / ui, d = ui + ur * t;
-          qr = (xr * t + xi) / d; qi = (xi * t - xr) / d;
-        }
-        b[2 * k] = qr; b[2 * k + 1] = qi;
-        for (int i = 0; i < k; ++i) {
-          b[2 * i] -= qr * u[2 * i] - qi * u[2 * i + 1];
-          b[2 * i + 1] -= qr * u[2 * i + 1] + qi * u[2 * i];
+    if (k0 + kb < n)
+      gemm_z(n - k0 - kb, nrhs, kb, mone, LU + 2 * ((size_t)k0 * lda + k0 + kb), lda,
+             B + 2 * (size_t)k0, ldb, one, B + 2 * (size_t)(k0 + kb), ldb);
+  }
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int k0 = blk * LU_NB, kb = MIN(LU_NB, n - k0);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nrhs; ++c) {
+      double *b = B + 2 * (size_t)c * ldb;
+      for (int k = k0 + kb - 1; k >= k0; --k) {
+        if (b[2 * k] != 0.0 || b[2 * k + 1] != 0.0) {
+          const double *u = LU + 2 * (size_t)k * lda;
+          /* b[k] /= u[k] by Smith's method */
+          const double ur = u[2 * k], ui = u[2 * k + 1], xr = b[2 * k], xi = b[2 * k + 1];
+          double qr, qi;
+          if (fabs(ur) >= fabs(ui)) {
+            const double t = ui / ur, d = ur + ui * t;
+            qr = (xr + xi * t) / d; qi = (xi - xr * t) / d;
+          } else {
+            const double t = ur / ui, d = ui + ur * t;
+            qr = (xr * t + xi) / d; qi = (xi * t - xr) / d;
+          }
+          b[2 * k] = qr; b[2 * k + 1] = qi;
+          for (int i = k0; i < k; ++i) {
+            b[2 * i] -= qr * u[2 * i] - qi * u[2 * i + 1];
+            b[2 * i + 1] -= qr * u[2 * i + 1] + qi * u[2 * i];
+          }
         }
       }
     }
+    if (k0 > 0)
+      gemm_z(k0, nrhs, kb, mone, LU + 2 * (size_t)k0 * lda, lda, B + 2 * (size_t)k0, ldb, one, B,
+             ldb);
   }
   return 0;
 }

@@ -41,18 +41,26 @@ class ExpmPlan(ctypes.Structure):
                 ("method", ctypes.c_int), ("ps_block", ctypes.c_int), ("dtype", ctypes.c_int),
                 ("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int),
                 ("norm1", ctypes.c_double), ("fp8_gemms", ctypes.c_int),
-                ("fp8_min_degree", ctypes.c_int)]
+                ("fp8_min_degree", ctypes.c_int), ("norm_powers", ctypes.c_int),
+                ("power_norm1", ctypes.c_double * 4)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["power_norm1"] = list(self.power_norm1)[:max(self.norm_powers - 1, 0)]
+        return d
 
 
 SPLIT_BF16X3 = 3
 SPLIT_BF16X6 = 6
+SPLIT_FP16X3 = 13
 
 
 class SplitOpts(ctypes.Structure):
-    _fields_ = [("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int)]
+    _fields_ = [("ps_split", ctypes.c_int), ("split_squarings", ctypes.c_int),
+                ("plan_from", ctypes.c_int)]
+
+
+PLAN_POWERS, PLAN_NORM1 = 0, 1
 
 
 class Fp8TailOpts(ctypes.Structure):
@@ -125,6 +133,10 @@ _SIGNATURES = {
     "clm_set_stream": (_i, [_vp, _vp]),
     "clm_get_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
     "clm_expm_plan_d": (ExpmPlan, [_d]),
+    "clm_expm_plan_powers_d": (ExpmPlan, [ctypes.POINTER(_d), _i]),
+    "clm_expm_plan_powers_s": (ExpmPlan, [ctypes.POINTER(_d), _i]),
+    "clm_expm_split_plan_powers_d": (ExpmPlan, [_i, ctypes.POINTER(_d), _i,
+                                                 ctypes.POINTER(SplitOpts)]),
     "clm_taylor_theta_d": (_d, [_i]),
     "clm_pade_theta_d": (_d, [_i]),
     "clm_pade_plan_d": (ExpmPlan, [_d]),

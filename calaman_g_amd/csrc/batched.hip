@@ -419,6 +419,31 @@ __device__ void wg_scale_copy(int n, double f, const T *in, long ldi, T *out, lo
   __syncthreads();
 }
 
+// bank[k-2] = 2^(-k s) bank[k-2] in place for k = 2 .. count + 1 (ldexp: the
+// single-matrix scale_powers_kernel's arithmetic)
+template <int EW = kEw, typename T>
+__device__ void wg_scale_powers(int n, T *bank, long stride, int count, int s) {
+  const int total = n * n, nt = blockDim.x;
+  for (int c = 0; c < count; ++c) {
+    T *x = bank + static_cast<long>(c) * stride;
+    const int e2 = -(c + 2) * s;
+    for (int base = threadIdx.x; base < total; base += nt * EW) {
+      T v[EW];
+#pragma unroll
+      for (int u = 0; u < EW; ++u) {
+        const int e = base + u * nt;
+        if (e < total) v[u] = x[e];
+      }
+#pragma unroll
+      for (int u = 0; u < EW; ++u) {
+        const int e = base + u * nt;
+        if (e < total) x[e] = ops<T>::ldexp(v[u], e2);
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // B = c0 I + c1 X1 + sum_t c_t bank[t-2], the per-element order of the
 // single-matrix block build
 template <int EW = kEw, typename T>
@@ -501,6 +526,18 @@ __device__ void wg_conj_transpose(int n, const T *src, long lds_, T *dst, long l
   __syncthreads();
 }
 
+// dst = v on the n x n block
+template <typename T>
+__device__ void wg_fill(int n, T *dst, long ld, T v) {
+  const int total = n * n, nt = blockDim.x;
+  for (int e = threadIdx.x; e < total; e += nt) {
+    int i, j;
+    ew_rc(e, n, i, j);
+    dst[i + static_cast<long>(j) * ld] = v;
+  }
+  __syncthreads();
+}
+
 // the adjoint kernel's LDS: the product's stages, at least one transpose tile
 template <typename T, int CFG>
 constexpr size_t kAdjointLds() {
@@ -551,9 +588,13 @@ __device__ GemmArgs<T> square_args(int n, const T *A, long lda, const T *B, long
   return g;
 }
 
-// One matrix's chain as a single sequence of products, so the product code is
-// instantiated once per tile configuration (not once per phase):
-//   steps [0, nbank)             power bank A^2 .. A^(nbank+1) of the scaled A,
+// One matrix's chain. First the norm-only plan's power bank A^2 .. A^q of A
+// itself (q = norm_power_count, none below degree 8), whose 1-norms give the
+// executed plan; the bank and A are then scaled by 2^-(k s) (exact) to the bank
+// of A / 2^s. Then the chain as one sequence of products, so the product code
+// of the long loop is instantiated once per tile configuration:
+//   steps [0, nbank)             power bank A^2 .. A^(nbank+1) of the scaled A
+//                                (from the first power not yet formed),
 //   (block B_r built here)
 //   steps [nbank, nbank + r)     Paterson-Stockmeyer Horner steps P <- P X + B_j,
 //                                B_j formed in the product's epilogue,
@@ -564,6 +605,7 @@ template <typename T, int CFG>
 __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *lds,
                          clm_batch_result *results) {
   using R = real_t<T>;
+  constexpr bool kSingle = std::is_same_v<R, float>;
   const int n = d.n;
   const T *A = static_cast<const T *>(d.A);
   T *E = static_cast<T *>(d.E);
@@ -577,13 +619,29 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
 
   const R norm = uniform(wg_norm1(n, A, d.lda, lds));
   if (!isfinite(norm)) {
+    // no exponential: E is NaN (never stale memory) and the status says why
     if (threadIdx.x == 0 && results)
       results[d.index] = clm_batch_result{0, 0, 0, CLM_STATUS_INVALID_VALUE, double(norm)};
     if (threadIdx.x == 0 && ad) tape.head[3] = CLM_STATUS_INVALID_VALUE;
+    wg_fill(n, E, long(d.lde), ops<T>::from_real(__builtin_nan("")));
     return;  // uniform: every lane holds the same norm
   }
-  const clm_expm_plan p = taylor_plan(double(norm), std::is_same_v<R, float>);
+  const int npow =
+      uniform(norm_power_count(double(norm), taylor_plan(double(norm), kSingle).m, kSingle));
+  // the norm-only plan's bank A^2 .. A^npow of A itself, and its 1-norms
+  double pn[kMaxNormPower] = {double(norm)};
+  for (int k = 2; k <= npow; ++k) {
+    const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+    wg_gemm<T, CFG>(square_args(n, left, (k == 2) ? long(d.lda) : long(n), A, long(d.lda),
+                                bank + static_cast<long>(k - 2) * stride, long(n)),
+                    lds);
+  }
+  for (int k = 2; k <= npow; ++k)
+    pn[k - 1] =
+        double(uniform(wg_norm1(n, bank + static_cast<long>(k - 2) * stride, long(n), lds)));
+  const clm_expm_plan p = taylor_plan_powers(pn, npow, kSingle);
   const int degree = uniform(p.m), nsq = uniform(p.s);
+  const int npre = npow > 1 ? npow - 1 : 0;
   if (threadIdx.x == 0 && results)
     results[d.index] =
         clm_batch_result{degree, nsq, uniform(p.num_gemms), CLM_STATUS_SUCCESS, double(norm)};
@@ -600,6 +658,7 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   long ld_src = d.lda;
   if (nsq > 0) {
     wg_scale_copy<ew_of(CFG)>(n, ldexp(1.0, -nsq), A, d.lda, As, n);
+    if (npre > 0) wg_scale_powers<ew_of(CFG)>(n, bank, stride, npre, nsq);
     src = As;
     ld_src = n;
   }
@@ -616,7 +675,8 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
   const int total = nbank + nhorner + nsq;
   T *res = target, *spare = other;  // squaring operands once the polynomial is done
   long ld_res = ld_target, ld_spare = ld_other;
-  for (int q = 0; q <= total; ++q) {
+  // the bank powers formed before the plan are not formed again
+  for (int q = min(npre, nbank); q <= total; ++q) {
     if (q == nbank) {
       const int terms_r = min(degree - nhorner * bs + 1, bs);
       // with a tape, the Horner iterates P_r .. P_1 stay in their own blocks
@@ -713,17 +773,6 @@ __device__ void wg_block_adjoint(int n, const T *bt, const T *c, int terms, T *X
         if (e < total) y[e] = ops<T>::mad(c[t], b[u], v[u]);
       }
     }
-  }
-  __syncthreads();
-}
-
-template <typename T>
-__device__ void wg_fill(int n, T *dst, long ld, T v) {
-  const int total = n * n, nt = blockDim.x;
-  for (int e = threadIdx.x; e < total; e += nt) {
-    int i, j;
-    ew_rc(e, n, i, j);
-    dst[i + static_cast<long>(j) * ld] = v;
   }
   __syncthreads();
 }

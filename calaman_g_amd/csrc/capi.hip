@@ -100,7 +100,7 @@ size_t ps_buffer_size(int n, int degree, int s_requested) {
 // squarings after it. So:
 //   s <= 5:      polynomial products BF16 3-way (~16 bits) while ||A / 2^s||_1 <= 1,
 //                else in IEEE-half planes (FP16X3: rows / columns scaled by powers
-//                of two, ~22 bits, 3 products);
+//                of two, ~22 bits, 3 products) -- BF16 6-way for orders n <= 64;
 //   s = 6 .. 8:  polynomial BF16 6-way (~24 bits);
 //   s > 8:       polynomial BF16 6-way, every squaring FP64;
 // and the last t squarings split: t = 1 for s = 1 .. 8, 2 for s = 4, 5 with a
@@ -109,7 +109,10 @@ size_t ps_buffer_size(int n, int degree, int s_requested) {
 // overflow there, as it would in BF16 planes (FP32 first).
 // Measured against FP64 (n = 16 .. 2048, ||A||_1 from 0.1 to 3000, several
 // seeds): at most 6.9e-7 relative error; BF16 3-way polynomials at ||A / 2^s|| > 1
-// reached 1e-5 for n <= 128, half-plane polynomials at s >= 7 1.8e-6.
+// reached 1e-5 for n <= 128, half-plane polynomials at s >= 7 1.8e-6; at n <= 64
+// the 6-way polynomial errs less than the half planes (n = 16, ||A||_1 = 100:
+// 0.8e-7 against 7e-7), at n >= 256 more (n = 256, ||A||_1 = 1000: 2.7e-6 against
+// 4e-7 at s = 5).
 // The degree minimises the products weighted by their cost relative to an FP64
 // product (3-product formats 0.2, 6-way 0.35), so cheap split polynomial products
 // are traded for expensive FP64 squarings.
@@ -120,10 +123,24 @@ struct SplitChoice {
 // 1.7e-7 there, 5.5e-7 at 2); the 2^-20 slack keeps a matrix scaled to 1-norm 1,
 // whose computed column sum can land a few ulps above 1, on the measured side.
 constexpr double kBf16x3MaxScaledNorm = 1.0 + 0x1p-20;
-SplitChoice split_schedule(int s, double scaled_norm1, const clm_split_opts *opts) {
+// orders up to this one take the 6-way polynomial where larger ones take half planes
+constexpr int kSplitSmallOrder = 64;
+// The split variant plans from the norms of powers only up to this 1-norm. Fewer
+// squarings leave a larger ||A / 2^s||_1, and the polynomial's error in the
+// 16-bit formats grows with it (halving with every squaring added back): measured
+// at n = 2048, ||A||_1 = 1000, the half-plane polynomial errs 6.8e-7 at the powers'
+// s = 4 against 0.9e-7 at the norm's s = 8, and n = 1024 at 3000 reaches 1.6e-6 at
+// the powers' s = 6. Up to 256 (both benchmark norm sets) every measured case of
+// the powers' plan stays within 5.3e-7; above it the plan is the norm's. (The 2^-20
+// slack keeps a matrix scaled to 1-norm 256, whose computed column sum can land a
+// few ulps above it, on the measured side.)
+constexpr double kSplitPowerMaxNorm1 = 256.0 * (1.0 + 0x1p-20);
+SplitChoice split_schedule(int s, double scaled_norm1, int n, const clm_split_opts *opts) {
   SplitChoice c{CLM_SPLIT_BF16X6, 0};
   if (s <= 5)
-    c.ps_split = scaled_norm1 <= kBf16x3MaxScaledNorm ? CLM_SPLIT_BF16X3 : CLM_SPLIT_FP16X3;
+    c.ps_split = scaled_norm1 <= kBf16x3MaxScaledNorm ? CLM_SPLIT_BF16X3
+                 : n <= kSplitSmallOrder              ? CLM_SPLIT_BF16X6
+                                                      : CLM_SPLIT_FP16X3;
   if (opts && (opts->ps_split == CLM_SPLIT_BF16X3 || opts->ps_split == CLM_SPLIT_BF16X6 ||
                opts->ps_split == CLM_SPLIT_FP16X3))
     c.ps_split = opts->ps_split;
@@ -134,20 +151,27 @@ SplitChoice split_schedule(int s, double scaled_norm1, const clm_split_opts *opt
   return c;
 }
 
-clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
+// The split plan of an order-n matrix from norms[k-1] = ||A^k||_1, k = 1 .. q
+// (q = 1: the 1-norm alone; the norms of powers count only while ||A||_1 <=
+// kSplitPowerMaxNorm1): squarings per rung as in taylor_plan_powers at u = 2^-24,
+// the rung's cost weighted by format, the q - 1 powers already formed not counted
+// again.
+clm_expm_plan split_plan_powers(const double *pn, int q, int n, const clm_split_opts *opts) {
+  const double norm1 = pn[0];
+  if (!(norm1 <= kSplitPowerMaxNorm1)) q = 1;
+  const int have = q > 1 ? q - 1 : 0;
   int best = -1, best_s = 0;
   double best_cost = 0.0;
   for (int i = 0; i < kNumTaylor; ++i) {
-    int s = 0;
-    if (norm1 > taylor_theta_at(i, true)) {
-      s = static_cast<int>(std::ceil(std::log2(norm1 / taylor_theta_at(i, true))));
-      if (s < 0) s = 0;
-    }
-    const SplitChoice c = split_schedule(s, std::ldexp(norm1, -s), opts);
+    const clm_ps_plan pp = ps_plan(taylor_degree(i), 0);
+    const int s = power_squarings(pn, q, taylor_degree(i), taylor_theta_at(i, true));
+    if (s >= kUnusableSquarings) continue;
+    const SplitChoice c = split_schedule(s, std::ldexp(norm1, -s), n, opts);
     const double w_ps = (c.ps_split == CLM_SPLIT_BF16X6) ? 0.35 : 0.2;
     const double w_sq = 0.2;  // split squarings: half planes, 3 products
-    const double cost = ps_plan(taylor_degree(i), 0).num_gemms * w_ps +
-                        (s - c.split_squarings) * 1.0 + c.split_squarings * w_sq;
+    const int new_powers = pp.num_powers > have ? pp.num_powers - have : 0;
+    const double cost = (new_powers + pp.r) * w_ps + (s - c.split_squarings) * 1.0 +
+                        c.split_squarings * w_sq;
     if (best < 0 || cost < best_cost - 1e-12 || (std::fabs(cost - best_cost) <= 1e-12 && s < best_s)) {
       best = i;
       best_cost = cost;
@@ -155,16 +179,23 @@ clm_expm_plan split_plan(double norm1, const clm_split_opts *opts) {
     }
   }
   clm_expm_plan p{};
+  const clm_ps_plan pp = ps_plan(taylor_degree(best), 0);
   p.m = taylor_degree(best);
   p.s = best_s;
-  p.num_gemms = ps_plan(p.m, 0).num_gemms + best_s;
+  p.num_gemms = (pp.num_powers > have ? pp.num_powers : have) + pp.r + best_s;
   p.method = CLM_METHOD_TAYLOR_PS;
-  p.ps_block = ps_plan(p.m, 0).s;
+  p.ps_block = pp.s;
   p.norm1 = norm1;
-  const SplitChoice c = split_schedule(best_s, std::ldexp(norm1, -best_s), opts);
+  p.norm_powers = q < 1 ? 1 : q;
+  for (int k = 2; k <= q && k <= kMaxNormPower; ++k) p.power_norm1[k - 2] = pn[k - 1];
+  const SplitChoice c = split_schedule(best_s, std::ldexp(norm1, -best_s), n, opts);
   p.ps_split = c.ps_split;
   p.split_squarings = c.split_squarings;
   return p;
+}
+
+clm_expm_plan split_plan(double norm1, int n, const clm_split_opts *opts) {
+  return split_plan_powers(&norm1, 1, n, opts);
 }
 
 // ---- the reference's Pade ladder (src/expm/plan.cppm:51-130, expm_bridge.h:128-130) ----
@@ -285,19 +316,29 @@ clm_status_t gemm(hipStream_t st, int n, const T *A, long lda, const T *B, long 
   return CLM_STATUS_SUCCESS;
 }
 
+// The power bank A^2 .. A^k_to of A, from A^k_from on (the powers below it are
+// already in the bank): bank[k-2] = bank[k-3] A.
+template <typename T>
+clm_status_t bank_powers(hipStream_t st, int n, const T *A, long lda, T *bank, long stride,
+                         int k_from, int k_to) {
+  for (int k = k_from; k <= k_to; ++k) {
+    const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+    const long ld_left = (k == 2) ? lda : n;
+    CLM_TRY(gemm(st, n, left, ld_left, A, lda, bank + static_cast<long>(k - 2) * stride, n));
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
 // Paterson-Stockmeyer on the device, the reference's structure
 // (src/paterson_stockmeyer/interface.cppm:364-419) with each B_j built inside
-// the epilogue of the Horner product that consumes it.
+// the epilogue of the Horner product that consumes it. The first `have` powers
+// (A^2 .. A^(have+1)) are already in the bank.
 template <typename T>
 clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const T *A, long lda,
                          T *P, long ldp, T *bank, long stride, T *scratch, long ld_scratch,
-                         int s_requested, int prec = 0) {
+                         int s_requested, int have = 0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
-  for (int k = 2; k <= p.num_powers + 1; ++k) {
-    const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
-    const long ld_left = (k == 2) ? lda : n;
-    CLM_TRY(gemm(st, n, left, ld_left, A, lda, bank + static_cast<long>(k - 2) * stride, n, prec));
-  }
+  CLM_TRY(bank_powers(st, n, A, lda, bank, stride, have + 2, p.num_powers + 1));
   const T *X = (p.r < 1) ? nullptr : (p.s >= 2) ? bank + static_cast<long>(p.s - 2) * stride : A;
   const long ldx = (p.s >= 2) ? n : lda;
   T *buf[2] = {P, scratch};
@@ -327,7 +368,7 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
     g.bank = bank;
     g.bank_stride = stride;
     g.ldbank = n;
-    CLM_TRY_HIP(launch_prec(st, g, prec));
+    CLM_TRY_HIP(launch_prec(st, g, 0));
     cur = next;
   }
   return CLM_STATUS_SUCCESS;
@@ -399,38 +440,56 @@ clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const Pl
   return CLM_STATUS_SUCCESS;
 }
 
+GemmArgs<double> plain_args(int n, double *C, long ldc) {
+  GemmArgs<double> g;
+  g.M = g.N = g.K = n;
+  g.C = C;
+  g.ldc = ldc;
+  g.alpha = 1.0;
+  return g;
+}
+
+// The power bank on split planes, A^k_from .. A^k_to: A is split once (column
+// planes as the right operand, and row planes for A^2's left), each earlier
+// power once as a left operand. Returns with sp.PCA holding A's column planes
+// when any product ran (*pca_is_a).
+clm_status_t bank_powers_split(hipStream_t st, int split, int n, const double *A, long lda,
+                               double *bank, long stride, int k_from, int k_to,
+                               const SplitWs &sp, bool *pca_is_a) {
+  for (int k = k_from; k <= k_to; ++k) {
+    if (k == 2) {
+      CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp));
+    } else {
+      if (!*pca_is_a) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp));
+      CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(k - 3) * stride, n, nullptr,
+                           &sp.PR, sp));
+    }
+    *pca_is_a = true;
+    CLM_TRY(gemm_planes(st, plain_args(n, bank + static_cast<long>(k - 2) * stride, n), split,
+                        sp.PR, sp.PCA, sp.pstride));
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
 // Paterson-Stockmeyer with every product on pre-split planes: each operand is
 // split once (the power bank's left factors, A and X once each, the Horner
 // accumulator once per step), the products never re-split inside their loops.
+// The first `have` powers are already in the bank.
 clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int degree,
                                const double *A, long lda, double *P, long ldp, double *bank,
                                long stride, double *scratch, long ld_scratch, int s_requested,
-                               int split, const SplitWs &sp) {
+                               int split, const SplitWs &sp, int have = 0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
-  const bool need_a_planes = p.num_powers > 0 || (p.r >= 1 && p.s < 2);
-  if (need_a_planes)
-    CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, p.num_powers > 0 ? &sp.PR : nullptr,
-                         sp));
-  auto plain = [&](double *C, long ldc) {
-    GemmArgs<double> g;
-    g.M = g.N = g.K = n;
-    g.C = C;
-    g.ldc = ldc;
-    g.alpha = 1.0;
-    return g;
-  };
-  for (int k = 2; k <= p.num_powers + 1; ++k) {
-    if (k > 2)
-      CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(k - 3) * stride, n, nullptr,
-                           &sp.PR, sp));
-    CLM_TRY(gemm_planes(st, plain(bank + static_cast<long>(k - 2) * stride, n), split, sp.PR,
-                        sp.PCA, sp.pstride));
-  }
+  bool pca_is_a = false;
+  CLM_TRY(bank_powers_split(st, split, n, A, lda, bank, stride, have + 2, p.num_powers + 1, sp,
+                            &pca_is_a));
   const PlaneSet *pcx = &sp.PCA;
   if (p.r >= 1 && p.s >= 2) {
     CLM_TRY(split_planes(st, split, n, bank + static_cast<long>(p.s - 2) * stride, n, &sp.PCX,
                          nullptr, sp));
     pcx = &sp.PCX;
+  } else if (p.r >= 1 && !pca_is_a) {
+    CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp));
   }
   double *buf[2] = {P, scratch};
   const long ld[2] = {ldp, ld_scratch};
@@ -444,7 +503,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
   for (int j = p.r - 1; j >= 0; --j) {
     const int next = 1 - cur;
     CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp));
-    GemmArgs<double> g = plain(buf[next], ld[next]);
+    GemmArgs<double> g = plain_args(n, buf[next], ld[next]);
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
     g.X1 = A;
@@ -487,55 +546,61 @@ size_t expm_fp8tail_buffer_size(int n) {
 
 constexpr int kFp8MinDegree = 4;
 
+// The FP8-tail variant's power bank, A^k_from .. A^k_to: A^k = A^(k-1) A on FP8
+// for k >= min_degree, on the split planes below it. *ca_split / *ca_fp8 say
+// whether A's column planes / e4m3 plane exist (made once, when first needed).
+clm_status_t bank_powers_fp8(hipStream_t st, int split, int min_degree, int n, const double *A,
+                             long lda, double *bank, long stride, int k_from, int k_to,
+                             const SplitWs &sp, const Fp8Ws &f8, bool *ca_split, bool *ca_fp8,
+                             int *count) {
+  for (int k = k_from; k <= k_to; ++k) {
+    const double *L = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+    const long ldl = (k == 2) ? lda : n;
+    const GemmArgs<double> g = plain_args(n, bank + static_cast<long>(k - 2) * stride, n);
+    if (k >= min_degree) {
+      if (k == 2) {
+        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, f8.R.q, f8.CA.e, f8.R.e));
+      } else {
+        if (!*ca_fp8)
+          CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, nullptr, f8.CA.e, nullptr));
+        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, L, ldl, nullptr, f8.R.q, nullptr, f8.R.e));
+      }
+      *ca_fp8 = true;
+      CLM_TRY_HIP(launch_gemm_fp8(st, g, f8.R.q, f8.CA.q, f8.R.e, f8.CA.e));
+      ++*count;
+    } else {
+      if (k == 2) {
+        CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp));
+      } else {
+        if (!*ca_split) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp));
+        CLM_TRY(split_planes(st, split, n, L, ldl, nullptr, &sp.PR, sp));
+      }
+      *ca_split = true;
+      CLM_TRY(gemm_planes(st, g, split, sp.PR, sp.PCA, sp.pstride));
+    }
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
 // Paterson-Stockmeyer of the FP8-tail variant. A product whose result holds
 // only Taylor terms of degree >= min_degree -- the bank's A^k = A^(k-1) A for
 // k >= min_degree, and the Horner product R_(j+1) X (X = A^b), whose lowest term
 // has degree (j+1) b -- runs on FP8 MFMA from e4m3 operands with one scale per
 // 128 x 128 tile; every other product on the split planes. Operand planes are
-// made once per operand and format, when a product first needs them.
+// made once per operand and format, when a product first needs them. The first
+// `have` powers are already in the bank; *fp8_products counts this call's FP8
+// products.
 clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int degree,
                                  const double *A, long lda, double *P, long ldp, double *bank,
                                  long stride, double *scratch, long ld_scratch, int s_requested,
                                  int split, int min_degree, const SplitWs &sp, const Fp8Ws &f8,
-                                 int *fp8_products) {
+                                 int *fp8_products, int have = 0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
   const int b = p.s;
   int count = 0;
-  auto plain = [&](double *C, long ldc) {
-    GemmArgs<double> g;
-    g.M = g.N = g.K = n;
-    g.C = C;
-    g.ldc = ldc;
-    g.alpha = 1.0;
-    return g;
-  };
   bool ca_split = false, ca_fp8 = false;
-  for (int k = 2; k <= p.num_powers + 1; ++k) {
-    const double *L = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
-    const long ldl = (k == 2) ? lda : n;
-    const GemmArgs<double> g = plain(bank + static_cast<long>(k - 2) * stride, n);
-    if (k >= min_degree) {
-      if (k == 2) {
-        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, f8.R.q, f8.CA.e, f8.R.e));
-      } else {
-        if (!ca_fp8)
-          CLM_TRY_HIP(launch_quantize_fp8(st, n, n, A, lda, f8.CA.q, nullptr, f8.CA.e, nullptr));
-        CLM_TRY_HIP(launch_quantize_fp8(st, n, n, L, ldl, nullptr, f8.R.q, nullptr, f8.R.e));
-      }
-      ca_fp8 = true;
-      CLM_TRY_HIP(launch_gemm_fp8(st, g, f8.R.q, f8.CA.q, f8.R.e, f8.CA.e));
-      ++count;
-    } else {
-      if (k == 2) {
-        CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, &sp.PR, sp));
-      } else {
-        if (!ca_split) CLM_TRY(split_planes(st, split, n, A, lda, &sp.PCA, nullptr, sp));
-        CLM_TRY(split_planes(st, split, n, L, ldl, nullptr, &sp.PR, sp));
-      }
-      ca_split = true;
-      CLM_TRY(gemm_planes(st, g, split, sp.PR, sp.PCA, sp.pstride));
-    }
-  }
+  CLM_TRY(bank_powers_fp8(st, split, min_degree, n, A, lda, bank, stride, have + 2,
+                          p.num_powers + 1, sp, f8, &ca_split, &ca_fp8, &count));
   // X = A^b for the Horner steps (A itself when the block is 1 wide)
   const bool x_is_a = !(p.r >= 1 && b >= 2);
   const double *X = x_is_a ? A : bank + static_cast<long>(b - 2) * stride;
@@ -552,7 +617,7 @@ clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int 
                                  terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
   for (int j = p.r - 1; j >= 0; --j) {
     const int next = 1 - cur;
-    GemmArgs<double> g = plain(buf[next], ld[next]);
+    GemmArgs<double> g = plain_args(n, buf[next], ld[next]);
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
     g.X1 = A;
@@ -627,19 +692,63 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   real_t<T> norm = 0;
   CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
+  if (use_split) CLM_TRY_HIP(sp.start(st, n));
 
-  // the degree ladder at the element type's unit roundoff (2^-53 or 2^-24)
-  clm_expm_plan p = use_split ? split_plan(norm, split) : taylor_plan(norm, is_single<T>());
+  // The plan from the 1-norm alone (the degree ladder at the element type's unit
+  // roundoff, 2^-53 or 2^-24) names the power bank; when that bank holds A^3 the
+  // powers are formed from A itself first, in the formats that plan gives them,
+  // and their exact 1-norms -- read back together, the call's second and last
+  // synchronization -- bound the truncation error through alpha_p, which lowers s
+  // (and often the degree) for a matrix far from normal.
+  // (The FP8 tail keeps the plan from the 1-norm: its e4m3 products err relative
+  // to |X||Y|, which only a small ||A / 2^s||_1 keeps near the size of the terms.)
+  const bool single_range = is_single<T>() || use_split;
+  const clm_expm_plan p0 =
+      use_split ? split_plan(norm, n, split) : taylor_plan(norm, is_single<T>());
+  const bool norm_only = variant == kFp8Tail || (split && split->plan_from == CLM_PLAN_NORM1) ||
+                         (use_split && !(double(norm) <= kSplitPowerMaxNorm1));
+  const int q = norm_only ? 1 : norm_power_count(double(norm), p0.m, single_range);
+  double pn[kMaxNormPower] = {double(norm)};
+  int fp8_count = 0;
+  if (q > 1) {
+    const long stride = static_cast<long>(ws.stride);
+    if constexpr (std::is_same_v<T, double>) {
+      if (variant == kFp8Tail) {
+        bool ca_split = false, ca_fp8 = false;
+        CLM_TRY(bank_powers_fp8(st, p0.ps_split, fp8_min_degree, n, dA, lda, ws.bank, stride, 2,
+                                q, sp, f8, &ca_split, &ca_fp8, &fp8_count));
+      } else if (use_split) {
+        bool pca_is_a = false;
+        CLM_TRY(bank_powers_split(st, p0.ps_split, n, dA, lda, ws.bank, stride, 2, q, sp,
+                                  &pca_is_a));
+      } else {
+        CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, 2, q));
+      }
+    } else {
+      CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, 2, q));
+    }
+    // the power norms land in the first bytes of S (unused until the polynomial)
+    real_t<T> *dnorms = reinterpret_cast<real_t<T> *>(ws.S);
+    for (int k = 2; k <= q; ++k)
+      CLM_TRY_HIP(launch_norm1(st, n, ws.bank + static_cast<long>(k - 2) * stride, n, ws.colsum,
+                               dnorms + (k - 2)));
+    real_t<T> host[kMaxNormPower] = {};
+    CLM_TRY_HIP(hipMemcpyAsync(host, dnorms, sizeof(real_t<T>) * (q - 1), hipMemcpyDeviceToHost, st));
+    CLM_TRY_HIP(hipStreamSynchronize(st));
+    for (int k = 2; k <= q; ++k) pn[k - 1] = double(host[k - 2]);
+  }
+  clm_expm_plan p = use_split ? split_plan_powers(pn, q, n, split)
+                              : taylor_plan_powers(pn, q, is_single<T>());
   p.dtype = dtype_code<T>();
   if (variant == kFp8Tail) p.fp8_min_degree = fp8_min_degree;
   if (plan) *plan = p;
-  if (use_split && (p.ps_split == CLM_SPLIT_FP16X3 || p.split_squarings > 0))
-    CLM_TRY_HIP(sp.start(st, n));
 
   const T *src = dA;
   long ld_src = lda;
   if (p.s > 0) {
     CLM_TRY_HIP(launch_scale_copy(st, n, std::ldexp(1.0, -p.s), dA, lda, ws.As, n));
+    // the bank formed from A becomes the bank of A / 2^s: A^k / 2^(k s), exact
+    CLM_TRY_HIP(launch_scale_powers(st, n, ws.bank, static_cast<long>(ws.stride), q - 1, p.s));
     src = ws.As;
     ld_src = n;
   }
@@ -653,20 +762,22 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
 
   if constexpr (std::is_same_v<T, double>) {
     if (variant == kFp8Tail) {
+      int count = 0;
       CLM_TRY(ps_evaluate_fp8tail(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
                                   static_cast<long>(ws.stride), other, ld_other, p.ps_block,
-                                  p.ps_split, fp8_min_degree, sp, f8, &p.fp8_gemms));
+                                  p.ps_split, fp8_min_degree, sp, f8, &count, q - 1));
+      p.fp8_gemms = fp8_count + count;
     } else if (use_split) {
       CLM_TRY(ps_evaluate_split(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
                                 static_cast<long>(ws.stride), other, ld_other, p.ps_block,
-                                p.ps_split, sp));
+                                p.ps_split, sp, q - 1));
     } else {
       CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                             static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+                             static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1));
     }
   } else {
     CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                           static_cast<long>(ws.stride), other, ld_other, p.ps_block));
+                           static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1));
   }
   if (plan) plan->fp8_gemms = p.fp8_gemms;
 
@@ -971,8 +1082,25 @@ clm_status_t clm_get_stream(clm_ctx *ctx, clm_stream_t *stream) {
 
 clm_expm_plan clm_expm_plan_d(double norm1) { return clm::taylor_plan(norm1, false); }
 
+static int clamp_norm_count(int q) {
+  return q < 1 ? 1 : (q > clm::kMaxNormPower ? clm::kMaxNormPower : q);
+}
+clm_expm_plan clm_expm_plan_powers_d(const double *norms, int q) {
+  if (norms == nullptr) return clm::taylor_plan(0.0, false);
+  return clm::taylor_plan_powers(norms, clamp_norm_count(q), false);
+}
+clm_expm_plan clm_expm_plan_powers_s(const double *norms, int q) {
+  if (norms == nullptr) return clm::taylor_plan(0.0, true);
+  return clm::taylor_plan_powers(norms, clamp_norm_count(q), true);
+}
+clm_expm_plan clm_expm_split_plan_powers_d(int n, const double *norms, int q,
+                                           const clm_split_opts *opts) {
+  if (norms == nullptr) return clm::split_plan(0.0, n, opts);
+  return clm::split_plan_powers(norms, clamp_norm_count(q), n, opts);
+}
+
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts) {
-  return clm::split_plan(norm1, opts);
+  return clm::split_plan(norm1, clm::kSplitSmallOrder + 1, opts);
 }
 
 double clm_taylor_theta_s(int m) {
@@ -992,7 +1120,8 @@ clm_status_t clm_expm_split_d(clm_ctx *ctx, clm_stream_t stream, int n, const do
                               const clm_split_opts *opts, clm_expm_plan *plan) {
   if (opts && ((opts->ps_split != 0 && opts->ps_split != CLM_SPLIT_BF16X3 &&
                 opts->ps_split != CLM_SPLIT_BF16X6 && opts->ps_split != CLM_SPLIT_FP16X3) ||
-               opts->split_squarings < -1))
+               opts->split_squarings < -1 ||
+               (opts->plan_from != CLM_PLAN_POWERS && opts->plan_from != CLM_PLAN_NORM1)))
     return CLM_STATUS_INVALID_VALUE;
   return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, dWork,
                                 lwork_bytes, dInfo, plan, opts, clm::kSplit);
@@ -1007,7 +1136,7 @@ clm_status_t clm_expm_fp8tail_buffer_size_d(int n, size_t *bytes) {
 clm_status_t clm_expm_fp8tail_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA,
                                 int lda, double *dE, int lde, void *dWork, size_t lwork_bytes,
                                 int *dInfo, const clm_fp8tail_opts *opts, clm_expm_plan *plan) {
-  clm_split_opts split{0, -1};
+  clm_split_opts split{0, -1, CLM_PLAN_NORM1};
   int min_degree = clm::kFp8MinDegree;
   if (opts) {
     if ((opts->ps_split != 0 && opts->ps_split != CLM_SPLIT_BF16X3 &&

@@ -60,6 +60,7 @@ struct ops<double> {
   // c * x + acc, the order the reference block build uses
   __host__ __device__ static double mad(double c, double x, double acc) { return c * x + acc; }
   __host__ __device__ static double scale(double x, double r) { return x * r; }
+  __host__ __device__ static double ldexp(double x, int e) { return ::ldexp(x, e); }
   __host__ __device__ static bool is_zero(double a) { return a == 0.0; }
 };
 
@@ -72,6 +73,7 @@ struct ops<float> {
   __host__ __device__ static float mul(float a, float b) { return a * b; }
   __host__ __device__ static float mad(float c, float x, float acc) { return c * x + acc; }
   __host__ __device__ static float scale(float x, double r) { return x * static_cast<float>(r); }
+  __host__ __device__ static float ldexp(float x, int e) { return ::ldexpf(x, e); }
   __host__ __device__ static bool is_zero(float a) { return a == 0.f; }
 };
 
@@ -89,6 +91,7 @@ struct ops<zc> {
     const float f = static_cast<float>(r);
     return zc{x.re * f, x.im * f};
   }
+  __host__ __device__ static zc ldexp(zc x, int e) { return zc{::ldexpf(x.re, e), ::ldexpf(x.im, e)}; }
   __host__ __device__ static bool is_zero(zc a) { return a.re == 0.f && a.im == 0.f; }
 };
 
@@ -103,6 +106,7 @@ struct ops<zd> {
   }
   __host__ __device__ static zd mad(zd c, zd x, zd acc) { return add(mul(c, x), acc); }
   __host__ __device__ static zd scale(zd x, double r) { return zd{x.re * r, x.im * r}; }
+  __host__ __device__ static zd ldexp(zd x, int e) { return zd{::ldexp(x.re, e), ::ldexp(x.im, e)}; }
   __host__ __device__ static bool is_zero(zd a) { return a.re == 0.0 && a.im == 0.0; }
 };
 
@@ -207,7 +211,11 @@ hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);
 // ---- element-wise and reduction kernels ------------------------------------------------
 // 1-norm in two launches (colsums, NaN-propagating max into dColsum[0]).
 template <typename T>
-hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum);
+hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum,
+                        real_t<T> *out = nullptr);
+// bank[k-2] *= 2^(-k s) for k = 2 .. count + 1 (n x n blocks, leading dimension n)
+template <typename T>
+hipError_t launch_scale_powers(hipStream_t stream, int n, T *bank, long stride, int count, int s);
 // lange: which in CLM_NORM_*; result in *d_result.
 template <typename T>
 hipError_t launch_lange(hipStream_t stream, int which, size_t m, size_t n, const T *A, size_t lda,

@@ -70,9 +70,10 @@ __global__ void __launch_bounds__(kRB)
   if (tid == 0) out[col] = sdata[0];
 }
 
-// max_reduce (src/expm/expm.cu:212-245): best seeded at 0, a NaN wins and stays.
+// max_reduce (src/expm/expm.cu:212-245): best seeded at 0, a NaN wins and stays;
+// the result goes to out[0] (values[0] when out is values).
 template <typename R>
-__global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, R *values) {
+__global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, R *values, R *out) {
   __shared__ R sdata[kRB];
   const int tid = threadIdx.x;
   R best = 0;
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, R *values) {
     }
     __syncthreads();
   }
-  if (tid == 0) values[0] = sdata[0];
+  if (tid == 0) out[0] = sdata[0];
 }
 
 // single-column identity-free fold of n values into out[0] (lange stage 2)
@@ -136,6 +137,20 @@ __global__ void __launch_bounds__(kEW)
   const int i = blockIdx.x * kEW + threadIdx.x;
   const long j = blockIdx.y;
   if (i < n) out[i + j * ldo] = ops<T>::scale(in[i + j * ldi], f);
+}
+
+// bank[k-2] = 2^(-k s) bank[k-2] in place for the powers k = 2 .. count + 1 (one
+// power per grid layer): the bank formed from A becomes the bank of A / 2^s.
+// ldexp is exact where the result is normal and rounds once where it is not.
+template <typename T>
+__global__ void __launch_bounds__(kEW)
+    scale_powers_kernel(int n, T *bank, long stride, long ldb, int s) {
+  const int i = blockIdx.x * kEW + threadIdx.x;
+  const long j = blockIdx.y;
+  const int k = static_cast<int>(blockIdx.z) + 2;
+  if (i >= n) return;
+  T *x = bank + static_cast<long>(blockIdx.z) * stride + i + j * ldb;
+  *x = ops<T>::ldexp(*x, -k * s);
 }
 
 template <typename T>
@@ -189,11 +204,23 @@ inline dim3 col_grid(size_t rows, size_t cols) {
 }  // namespace
 
 template <typename T>
-hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum) {
+hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum,
+                        real_t<T> *out) {
   if (n < 1) return hipSuccess;
   hipLaunchKernelGGL((reduce_columns_kernel<T, 0, 0>), dim3(n), dim3(kRB), 0, stream, A, colsum,
                      static_cast<size_t>(n), static_cast<size_t>(lda));
-  hipLaunchKernelGGL(max_reduce_kernel<real_t<T>>, dim3(1), dim3(kRB), 0, stream, n, colsum);
+  hipLaunchKernelGGL(max_reduce_kernel<real_t<T>>, dim3(1), dim3(kRB), 0, stream, n, colsum,
+                     out ? out : colsum);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_powers(hipStream_t stream, int n, T *bank, long stride, int count, int s) {
+  if (n < 1 || count < 1 || s == 0) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((n + kEW - 1) / kEW), static_cast<unsigned>(n),
+                  static_cast<unsigned>(count));
+  hipLaunchKernelGGL(scale_powers_kernel<T>, grid, dim3(kEW), 0, stream, n, bank, stride,
+                     static_cast<long>(n), s);
   return hipGetLastError();
 }
 
@@ -284,7 +311,8 @@ hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v) {
 }
 
 #define CLM_INSTANTIATE(T)                                                                         \
-  template hipError_t launch_norm1<T>(hipStream_t, int, const T *, long, real_t<T> *);             \
+  template hipError_t launch_norm1<T>(hipStream_t, int, const T *, long, real_t<T> *, real_t<T> *); \
+  template hipError_t launch_scale_powers<T>(hipStream_t, int, T *, long, int, int);               \
   template hipError_t launch_lange<T>(hipStream_t, int, size_t, size_t, const T *, size_t,          \
                                       real_t<T> *, real_t<T> *);                                   \
   template hipError_t launch_scale_copy<T>(hipStream_t, int, double, const T *, long, T *, long);  \

@@ -114,16 +114,56 @@ __host__ __device__ constexpr int max_bank_blocks() {
   return mx;
 }
 
-__host__ __device__ inline clm_expm_plan taylor_plan(double norm1, bool single) {
+// Squarings that rung i needs from the 1-norm alone: the smallest s >= 0 with
+// norm1 / 2^s <= theta. A ratio that overflows (a rung whose theta is tiny next
+// to a huge norm) marks the rung unusable instead of casting an infinite log2.
+constexpr int kUnusableSquarings = 1 << 20;
+__host__ __device__ inline int norm_squarings(double norm1, double theta) {
+  if (!(norm1 > theta)) return 0;
+  const double ratio = norm1 / theta;
+  if (!(ratio <= 1.7976931348623157e308)) return kUnusableSquarings;
+  const int s = static_cast<int>(ceil(log2(ratio)));
+  return s < 0 ? 0 : s;
+}
+
+// ---- squarings from the norms of powers ----------------------------------------------------
+// With pn[k-1] = ||A^k||_1 known for k = 1 .. q, the backward error of T_m(A / 2^s)
+// is bounded through alpha_p = max(||A^p||^(1/p), ||A^(p+1)||^(1/(p+1))) for every p
+// with p(p-1) <= m + 1 (Al-Mohy & Higham, SIAM J. Matrix Anal. Appl. 31 (2009)
+// 970-989, Thm 4.2(a)): every k >= m + 1 is a sum of p's and (p+1)'s, so
+//   || sum_{k>m} h_k (A/2^s)^k || <= sum_{k>m} |h_k| (alpha_p / 2^s)^k,
+// and alpha_p / 2^s <= theta_m keeps the relative backward error <= u exactly as
+// ||A||_1 / 2^s <= theta_m does (alpha_p <= ||A||_1). For a matrix far from normal
+// alpha_p is far below ||A||_1 -- a dense N(0, 1) matrix of order 2048 has
+// alpha_3 ~ ||A||_1 / 11 -- so s drops by several squarings. The comparison
+// d_k <= x is made as ||A^k|| <= x^k (x^k by repeated multiplication), so the
+// host and the device agree bit for bit without k-th roots.
+constexpr int kMaxNormPower = 5;  // the largest power bank (degrees 24, 29) holds A^2 .. A^5
+__host__ __device__ inline int power_squarings(const double *pn, int q, int m, double theta) {
+  const int s_norm = norm_squarings(pn[0], theta);  // p = 1: alpha_1 = ||A||_1
+  if (q < 3 || s_norm == 0 || s_norm >= kUnusableSquarings) return s_norm;
+  for (int s = 0; s < s_norm; ++s) {
+    const double x = ldexp(theta, s);
+    double xp = x * x;
+    for (int p = 2; p + 1 <= q && p * (p - 1) <= m + 1; ++p) {
+      const double xp1 = xp * x;
+      if (pn[p - 1] <= xp && pn[p] <= xp1) return s;
+      xp = xp1;
+    }
+  }
+  return s_norm;
+}
+
+// The plan from the norms of A .. A^q: the rung minimising products + s, ties to
+// the smaller s, where the q - 1 powers already formed count whether or not the
+// rung's bank uses them. q = 1 is the plan from the 1-norm alone.
+__host__ __device__ inline clm_expm_plan taylor_plan_powers(const double *pn, int q, bool single) {
+  const int have = q > 1 ? q - 1 : 0;
   int best = -1, best_cost = 0, best_s = 0;
   for (int i = 0; i < kNumTaylor; ++i) {
-    const double theta = taylor_theta_at(i, single);
-    int s = 0;
-    if (norm1 > theta) {
-      s = static_cast<int>(ceil(log2(norm1 / theta)));
-      if (s < 0) s = 0;
-    }
-    const int cost = ps_plan(taylor_degree(i), 0).num_gemms + s;
+    const clm_ps_plan pp = ps_plan(taylor_degree(i), 0);
+    const int s = power_squarings(pn, q, taylor_degree(i), taylor_theta_at(i, single));
+    const int cost = (pp.num_powers > have ? pp.num_powers : have) + pp.r + s;
     if (best < 0 || cost < best_cost || (cost == best_cost && s < best_s)) {
       best = i;
       best_cost = cost;
@@ -137,8 +177,24 @@ __host__ __device__ inline clm_expm_plan taylor_plan(double norm1, bool single) 
   p.method = CLM_METHOD_TAYLOR_PS;
   p.ps_block = ps_plan(p.m, 0).s;
   p.dtype = 0;
-  p.norm1 = norm1;
+  p.norm1 = pn[0];
+  p.norm_powers = q < 1 ? 1 : q;
+  for (int k = 2; k <= q && k <= kMaxNormPower; ++k) p.power_norm1[k - 2] = pn[k - 1];
   return p;
+}
+
+__host__ __device__ inline clm_expm_plan taylor_plan(double norm1, bool single) {
+  return taylor_plan_powers(&norm1, 1, single);
+}
+
+// Whether the norms of powers are taken: powers of the unscaled matrix must stay
+// finite (||A^5|| <= ||A||^5; FP32 also bounds the 16-bit split planes), and at
+// least A^3 must be in the norm-only plan's bank for an alpha_p to exist.
+__host__ __device__ inline int norm_power_count(double norm1, int degree, bool single_range) {
+  if (!(norm1 <= (single_range ? 0x1p24 : 0x1p200))) return 1;
+  const int q = ps_plan(degree, 0).num_powers + 1;
+  if (q < 3) return 1;
+  return q < kMaxNormPower ? q : kMaxNormPower;
 }
 
 }  // namespace clm

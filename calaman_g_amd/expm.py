@@ -21,7 +21,10 @@ Errors are raised as ``CalamanError`` carrying the library status.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
+import threading
+import weakref
 from typing import Optional
 
 import torch
@@ -57,6 +60,23 @@ def _stream_ptr(stream: Optional[torch.cuda.Stream]) -> int:
     return s.cuda_stream
 
 
+@contextlib.contextmanager
+def _on_stream(stream: Optional[torch.cuda.Stream]):
+    """Runs a convenience wrapper's device work on `stream`: the stream first waits
+    for the work already queued on the current stream (which made the input), the
+    temporaries come from `stream`'s allocator pool, and afterwards the current
+    stream waits for `stream`, so the result is ready wherever it is used next.
+    Yields a function that marks a returned tensor as used on the current stream."""
+    cur = torch.cuda.current_stream()
+    if stream is None or stream == cur:
+        yield lambda t: t
+        return
+    stream.wait_stream(cur)
+    with torch.cuda.stream(stream):
+        yield lambda t: (t.record_stream(cur), t)[1]
+    cur.wait_stream(stream)
+
+
 def _ptr(t: Optional[torch.Tensor]):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
@@ -80,10 +100,18 @@ def from_cm(buf: torch.Tensor, n_rows: int, n_cols: int, ld: Optional[int] = Non
 
 
 # ---- plans and sizes ------------------------------------------------------------------
-def expm_plan(norm1: float, dtype: torch.dtype = _F64) -> dict:
-    """The plan expm() executes for a given 1-norm (the role of expm_plan<T>);
-    float32/complex64 use the thresholds at u = 2^-24."""
-    f = lib().clm_expm_plan_s if _is_single(dtype) else lib().clm_expm_plan_d
+def expm_plan(norm1: float, dtype: torch.dtype = _F64, power_norms=None) -> dict:
+    """The plan for a 1-norm alone (the role of expm_plan<T>); with power_norms
+    (||A^k||_1 for k = 2 .. q, as expm() reports them in plan["power_norm1"]) the
+    plan expm() executes from the norms of A .. A^q. float32/complex64 use the
+    thresholds at u = 2^-24."""
+    single = _is_single(dtype)
+    if power_norms:
+        norms = [float(norm1)] + [float(v) for v in power_norms]
+        arr = (ctypes.c_double * len(norms))(*norms)
+        f = lib().clm_expm_plan_powers_s if single else lib().clm_expm_plan_powers_d
+        return f(arr, len(norms)).as_dict()
+    f = lib().clm_expm_plan_s if single else lib().clm_expm_plan_d
     return f(float(norm1)).as_dict()
 
 
@@ -193,10 +221,11 @@ def expm_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int, dWor
 def expm_split_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
                   dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None, ps_split: int = 0,
                   split_squarings: int = -1, stream: Optional[torch.cuda.Stream] = None,
-                  ctx: Optional[Context] = None) -> tuple:
-    """clm_expm_split_d: the BF16-split precision variant; returns (status, plan dict)."""
+                  ctx: Optional[Context] = None, plan_from: int = _lib.PLAN_POWERS) -> tuple:
+    """clm_expm_split_d: the BF16-split precision variant; returns (status, plan dict).
+    plan_from: PLAN_POWERS (s from the norms of powers) or PLAN_NORM1 (||A||_1 alone)."""
     plan = ExpmPlan()
-    opts = _lib.SplitOpts(int(ps_split), int(split_squarings))
+    opts = _lib.SplitOpts(int(ps_split), int(split_squarings), int(plan_from))
     st = lib().clm_expm_split_d(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)),
                                 int(n), _ptr(dA), int(lda), _ptr(dE), int(lde), _ptr(dWork),
                                 dWork.numel() * dWork.element_size(), _ptr(dInfo),
@@ -210,9 +239,16 @@ def expm_split_buffer_size(n: int) -> int:
     return int(out.value)
 
 
-def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1) -> dict:
-    opts = _lib.SplitOpts(int(ps_split), int(split_squarings))
-    return lib().clm_expm_split_plan_d(float(norm1), ctypes.byref(opts)).as_dict()
+def expm_split_plan(norm1: float, ps_split: int = 0, split_squarings: int = -1,
+                    power_norms=None, n: int = 65) -> dict:
+    """The split plan expm_split executes for an order-n matrix from the norms of
+    A .. A^q: norm1 and power_norms (||A^k||_1 for k = 2 .. q, as the plan reports
+    them in "power_norm1"; none = the 1-norm alone)."""
+    opts = _lib.SplitOpts(int(ps_split), int(split_squarings), _lib.PLAN_POWERS)
+    norms = [float(norm1)] + [float(v) for v in (power_norms or [])]
+    arr = (ctypes.c_double * len(norms))(*norms)
+    return lib().clm_expm_split_plan_powers_d(int(n), arr, len(norms),
+                                              ctypes.byref(opts)).as_dict()
 
 
 def expm_fp8tail_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
@@ -414,32 +450,43 @@ class _BatchedAdjointState:
         self.work = self.plan.workspace()
 
 
-_ad_states: dict = {}
+# Free adjoint states per (batch, n, dtype, device, checkpoints). Every forward takes
+# a state of its own (its tape lives in the state's workspace until its backward has
+# run); the state returns to the pool when the forward's autograd node is freed --
+# after backward, or when the graph is dropped without one.
+_ad_free: dict = {}
+_ad_lock = threading.Lock()
 
 
-def _ad_state(b, n, dtype, device, checkpoints):
-    key = (b, n, dtype, str(device), checkpoints)
-    st = _ad_states.get(key)
-    if st is None:
-        st = _BatchedAdjointState(b, n, dtype, device, checkpoints)
-        _ad_states[key] = st
-    return st
+def _ad_acquire(key):
+    with _ad_lock:
+        free = _ad_free.get(key)
+        if free:
+            return free.pop()
+    return _BatchedAdjointState(*key)
+
+
+def _ad_release(key, st):
+    with _ad_lock:
+        _ad_free.setdefault(key, []).append(st)
 
 
 class ExpmBatchedFunction(torch.autograd.Function):
     """exp of a (B, n, n) batch on the batched persistent kernel, with the
-    hand-written adjoint (clm_expm_batched_backward) as its backward. The state
-    (buffers, plan, tape) is cached per shape: a second forward of the same shape
-    overwrites the tape of the first, so backward each forward before the next."""
+    hand-written adjoint (clm_expm_batched_backward) as its backward. Each forward
+    holds its own buffers, plan and tape until its autograd node is freed, so
+    several forwards of one shape may precede their backwards."""
 
     @staticmethod
     def forward(ctx, a, checkpoints=4):
         b, n, _ = a.shape
-        st = _ad_state(b, n, a.dtype, a.device, int(checkpoints))
+        key = (b, n, a.dtype, a.device, int(checkpoints))
+        st = _ad_acquire(key)
+        ctx.state = st
+        ctx.lease = weakref.finalize(ctx, _ad_release, key, st)
         # column-major storage of each A_i is A_i^T in row-major order
         st.A.copy_(a.detach().transpose(1, 2))
         check(st.plan.launch(st.work), "expm_batched forward")
-        ctx.state = st
         return st.E.transpose(1, 2).clone()
 
     @staticmethod
@@ -488,14 +535,21 @@ def batched_autotune(dtype: torch.dtype, nmax: int = 256,
 
 def expm_batched(mats, stream: Optional[torch.cuda.Stream] = None):
     """exp of every square device matrix in ``mats`` (mixed orders and element types)
-    in one batched launch; returns the list of results (row-major, like expm())."""
-    cms = [to_cm(a) for a in mats]
-    outs = [torch.empty_like(c) for c in cms]
-    plan = BatchPlan([(a.shape[0], c, a.shape[0], e, a.shape[0]) for a, c, e in zip(mats, cms, outs)])
-    work = plan.workspace()
-    check(plan.launch(work, stream=stream), "expm_batched")
-    torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
-    return [e.t() for e in outs]
+    in one batched launch; returns the list of results (row-major, like expm()).
+    Raises CalamanError naming the members whose 1-norm is not finite."""
+    with _on_stream(stream) as hand_over:
+        cms = [to_cm(a) for a in mats]
+        outs = [torch.empty_like(c) for c in cms]
+        plan = BatchPlan([(a.shape[0], c, a.shape[0], e, a.shape[0])
+                          for a, c, e in zip(mats, cms, outs)])
+        work = plan.workspace()
+        res = plan.results_buffer()
+        check(plan.launch(work, res, stream=stream), "expm_batched")
+        bad = [i for i, r in enumerate(BatchPlan.decode_results(res)) if r["status"] != 0]
+        if bad:
+            raise CalamanError(_lib.INVALID_VALUE, f"expm_batched: members {bad[:8]} "
+                               f"({len(bad)} in all) have a non-finite 1-norm")
+        return [hand_over(e).t() for e in outs]
 
 
 # ---- convenience layer (row-major torch matrices) ----------------------------------------------
@@ -506,13 +560,14 @@ def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
     n = a.shape[-1]
     if a.dim() != 2 or a.shape[0] != n:
         raise ValueError("expm expects a square matrix")
-    acm = to_cm(a)
-    ecm = torch.empty_like(acm)
-    work = torch.empty(expm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
-    info = torch.empty(2, dtype=torch.int32, device=a.device)
-    st, plan = expm_cm(n, acm, n, ecm, n, work, info, stream)
-    check(st, "expm")
-    e = ecm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+        info = torch.empty(2, dtype=torch.int32, device=a.device)
+        st, plan = expm_cm(n, acm, n, ecm, n, work, info, stream)
+        check(st, "expm")
+        e = hand_over(ecm).t()
     return (e, plan) if return_plan else e
 
 
@@ -523,12 +578,14 @@ def expm_split(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
     if a.dtype != _F64:
         raise TypeError("expm_split takes float64")
     n = a.shape[-1]
-    acm = to_cm(a)
-    ecm = torch.empty_like(acm)
-    work = torch.empty(expm_split_buffer_size(n), dtype=torch.uint8, device=a.device)
-    st, plan = expm_split_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings, stream)
-    check(st, "expm_split")
-    e = ecm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_split_buffer_size(n), dtype=torch.uint8, device=a.device)
+        st, plan = expm_split_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings,
+                                 stream)
+        check(st, "expm_split")
+        e = hand_over(ecm).t()
     return (e, plan) if return_plan else e
 
 
@@ -541,42 +598,46 @@ def expm_fp8tail(a: torch.Tensor, ps_split: int = 0, split_squarings: int = -1,
     if a.dtype != _F64:
         raise TypeError("expm_fp8tail takes float64")
     n = a.shape[-1]
-    acm = to_cm(a)
-    ecm = torch.empty_like(acm)
-    work = torch.empty(expm_fp8tail_buffer_size(n), dtype=torch.uint8, device=a.device)
-    st, plan = expm_fp8tail_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings,
-                               fp8_min_degree, stream)
-    check(st, "expm_fp8tail")
-    e = ecm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_fp8tail_buffer_size(n), dtype=torch.uint8, device=a.device)
+        st, plan = expm_fp8tail_cm(n, acm, n, ecm, n, work, None, ps_split, split_squarings,
+                                   fp8_min_degree, stream)
+        check(st, "expm_fp8tail")
+        e = hand_over(ecm).t()
     return (e, plan) if return_plan else e
 
 
 def matrix_norm1(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> float:
     _require_device(a, "a")
     n = a.shape[0]
-    acm = to_cm(a)
-    colsum = torch.empty(n, dtype=_real_dtype(a.dtype), device=a.device)
-    return matrix_norm1_cm(n, acm, n, colsum, stream)
+    with _on_stream(stream):
+        acm = to_cm(a)
+        colsum = torch.empty(n, dtype=_real_dtype(a.dtype), device=a.device)
+        return matrix_norm1_cm(n, acm, n, colsum, stream)
 
 
 def lange(which: str, a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
     """The 'max_abs' | 'one' | 'inf' | 'frobenius' norm of a, as a device scalar."""
     code = {"max_abs": 0, "one": 1, "inf": 2, "frobenius": 3}[which]
     m, n = a.shape
-    acm = to_cm(a)
-    out = torch.empty(1, dtype=_real_dtype(a.dtype), device=a.device)
-    scratch = torch.empty(max(m, n, 1), dtype=_real_dtype(a.dtype), device=a.device)
-    check(lange_cm(code, m, n, acm, max(m, 1), out, scratch, stream), "lange")
-    return out[0]
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        out = torch.empty(1, dtype=_real_dtype(a.dtype), device=a.device)
+        scratch = torch.empty(max(m, n, 1), dtype=_real_dtype(a.dtype), device=a.device)
+        check(lange_cm(code, m, n, acm, max(m, 1), out, scratch, stream), "lange")
+        return hand_over(out)[0]
 
 
 def lascl(cfrom: float, cto: float, a: torch.Tensor,
           stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
     """a * (cto / cfrom) through ?lascl's guarded multipliers (returns a new matrix)."""
     m, n = a.shape
-    acm = to_cm(a)
-    check(lascl_cm(cfrom, cto, m, n, acm, max(m, 1), stream), "lascl")
-    return acm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        check(lascl_cm(cfrom, cto, m, n, acm, max(m, 1), stream), "lascl")
+        return hand_over(acm).t()
 
 
 def paterson_stockmeyer(coeffs: torch.Tensor, a: torch.Tensor, s_requested: int = 0,
@@ -584,13 +645,15 @@ def paterson_stockmeyer(coeffs: torch.Tensor, a: torch.Tensor, s_requested: int 
     n = a.shape[0]
     degree = coeffs.numel() - 1
     ctx = _ctx_for(stream)
-    acm = to_cm(a)
-    pcm = torch.empty_like(acm)
-    nbytes = ps_buffer_size(n, degree, s_requested, a.dtype)
-    work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=a.device)
-    check(paterson_stockmeyer_cm(ctx, n, coeffs.contiguous(), degree, acm, n, pcm, n, work,
-                                 s_requested), "paterson_stockmeyer")
-    return pcm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        cf = coeffs.contiguous()
+        pcm = torch.empty_like(acm)
+        nbytes = ps_buffer_size(n, degree, s_requested, a.dtype)
+        work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=a.device)
+        check(paterson_stockmeyer_cm(ctx, n, cf, degree, acm, n, pcm, n, work, s_requested),
+              "paterson_stockmeyer")
+        return hand_over(pcm).t()
 
 
 def horner(coeffs: torch.Tensor, a: torch.Tensor,
@@ -598,11 +661,13 @@ def horner(coeffs: torch.Tensor, a: torch.Tensor,
     n = a.shape[0]
     degree = coeffs.numel() - 1
     ctx = _ctx_for(stream)
-    acm = to_cm(a)
-    pcm = torch.empty_like(acm)
-    work = torch.empty(horner_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
-    check(horner_cm(ctx, n, coeffs.contiguous(), degree, acm, n, pcm, n, work), "horner")
-    return pcm.t()
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        cf = coeffs.contiguous()
+        pcm = torch.empty_like(acm)
+        work = torch.empty(horner_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+        check(horner_cm(ctx, n, cf, degree, acm, n, pcm, n, work), "horner")
+        return hand_over(pcm).t()
 
 
 def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torch.Tensor] = None,
@@ -612,11 +677,12 @@ def gemm(a: torch.Tensor, b: torch.Tensor, alpha=1.0, beta=0.0, c: Optional[torc
     if k != k2:
         raise ValueError("inner dimensions differ")
     ctx = _ctx_for(stream)
-    acm, bcm = to_cm(a), to_cm(b)
-    ccm = to_cm(c) if c is not None else torch.zeros((n, m), dtype=a.dtype, device=a.device)
-    check(gemm_cm(ctx, m, n, k, alpha, acm, max(m, 1), bcm, max(k, 1), beta, ccm, max(m, 1)),
-          "gemm")
-    return ccm.t()
+    with _on_stream(stream) as hand_over:
+        acm, bcm = to_cm(a), to_cm(b)
+        ccm = to_cm(c) if c is not None else torch.zeros((n, m), dtype=a.dtype, device=a.device)
+        check(gemm_cm(ctx, m, n, k, alpha, acm, max(m, 1), bcm, max(k, 1), beta, ccm,
+                      max(m, 1)), "gemm")
+        return hand_over(ccm).t()
 
 
 __all__ = [

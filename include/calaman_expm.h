@@ -105,13 +105,27 @@ typedef struct clm_expm_plan {
   double norm1;  /* the 1-norm the plan was chosen from */
   int fp8_gemms;      /* polynomial products that ran on FP8 (clm_expm_fp8tail_d) */
   int fp8_min_degree; /* the lowest Taylor degree an FP8 product may carry (0 otherwise) */
+  int norm_powers;    /* q: the plan used ||A^k||_1 for k = 1 .. q (1: the 1-norm alone) */
+  double power_norm1[4]; /* ||A^k||_1 of the unscaled powers k = 2 .. q (0 beyond q) */
 } clm_expm_plan;
 
-/* The plan clm_expm_d/z execute for a given 1-norm (the role of expm_plan<T>,
+/* The plan for a given 1-norm alone (the role of expm_plan<T>,
  * src/expm/plan.cppm:112-130): over the Taylor degrees
  * {1,2,3,5,8,11,15,19,24,29}, minimise Paterson-Stockmeyer products + s with
- * s = max(0, ceil(log2(norm1 / theta_m))), ties to the smaller s. */
+ * s = max(0, ceil(log2(norm1 / theta_m))), ties to the smaller s.
+ * clm_expm_* start from it; when its power bank holds A^3 (degree >= 8) and
+ * ||A||_1 <= 2^200 (2^24 for the single-precision types and the split / FP8
+ * variants) they form that bank from the unscaled matrix first, take the exact
+ * 1-norms of the powers and execute clm_expm_plan_powers_* of them. */
 clm_expm_plan clm_expm_plan_d(double norm1);
+/* The plan from norms[k-1] = ||A^k||_1, k = 1 .. q (1 <= q <= 5): s is the smallest
+ * with alpha_p / 2^s <= theta_m for some p with p(p-1) <= m+1, where
+ * alpha_p = max(||A^p||^(1/p), ||A^(p+1)||^(1/(p+1))) (Al-Mohy & Higham 2009,
+ * Thm 4.2; alpha_1 = ||A||_1), and the q - 1 powers already formed count as
+ * products whether or not the chosen degree's bank uses them. No reference
+ * counterpart: the reference plans from ||A||_1 alone. */
+clm_expm_plan clm_expm_plan_powers_d(const double *norms, int q);
+clm_expm_plan clm_expm_plan_powers_s(const double *norms, int q);
 /* Backward-error threshold theta_m of the degree-m Taylor approximant at
  * u = 2^-53 (0 for a degree off the ladder); the counterpart of pade_theta<T>
  * (src/expm/plan.cppm:51-85). */
@@ -177,14 +191,28 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
  * bits per product, between X3 and X6 at the cost of X3) */
 #define CLM_SPLIT_FP16X3 13
 typedef struct clm_split_opts {
-  int ps_split;        /* 0 = auto (for s <= 5: BF16X3 while ||A/2^s||_1 <= 1, else FP16X3;
-                          BF16X6 above), or CLM_SPLIT_BF16X3 / BF16X6 / FP16X3 */
+  int ps_split;        /* 0 = auto (for s <= 5: BF16X3 while ||A/2^s||_1 <= 1, else FP16X3,
+                          BF16X6 for n <= 64; BF16X6 above), or CLM_SPLIT_BF16X3 / BF16X6 /
+                          FP16X3 */
   int split_squarings; /* -1 = auto: 0 when s = 0 or s > 8; 2 with an FP16X3 polynomial
                           and s = 4, 5; otherwise 1. Else 0..s. Split squarings use
                           FP16X3 planes whatever the polynomial's format */
+  int plan_from;       /* CLM_PLAN_POWERS (0): s from the norms of powers when the bank
+                          holds A^3 and ||A||_1 <= 256; CLM_PLAN_NORM1 (1): from ||A||_1
+                          alone */
 } clm_split_opts;
-/* the plan clm_expm_split_d executes for a 1-norm under opts (NULL = auto) */
+#define CLM_PLAN_POWERS 0
+#define CLM_PLAN_NORM1 1
+/* the split plan for a 1-norm alone under opts (NULL = auto) for an order above
+ * 64, and the plan clm_expm_split_d executes for order n from norms[k-1] =
+ * ||A^k||_1, k = 1 .. q (see clm_expm_plan_powers_d: the powers are formed in the
+ * norm-only plan's formats when its bank holds A^3; the split variant uses their
+ * norms only while ||A||_1 <= 256, above which its polynomial's 16-bit error at
+ * the larger ||A / 2^s||_1 approaches 1e-6). Orders n <= 64 take the BF16 6-way
+ * polynomial where larger ones take IEEE-half planes. */
 clm_expm_plan clm_expm_split_plan_d(double norm1, const clm_split_opts *opts);
+clm_expm_plan clm_expm_split_plan_powers_d(int n, const double *norms, int q,
+                                           const clm_split_opts *opts);
 double clm_taylor_theta_s(int m); /* theta_m at u = 2^-24 */
 /* workspace: the FP64 layout of clm_expm_buffer_size_d plus nine n x n bfloat16
  * planes (each operand is split into planes once, then multiplied) */
@@ -263,7 +291,12 @@ clm_status_t clm_expm_batched_plan_destroy(clm_batch_plan *plan);
 /* device workspace bytes for one launch of the plan */
 clm_status_t clm_expm_batched_buffer_size(const clm_batch_plan *plan, size_t *bytes);
 /* One launch per precision present (FP32 pair, FP64 pair) on `stream`, plus a
- * one-lane queue reset; asynchronous; dResults optional (device array of count). */
+ * one-lane queue reset; asynchronous; dResults optional (device array of count).
+ * A member whose 1-norm is not finite gets CLM_STATUS_INVALID_VALUE in its result
+ * and NaN in its n x n output. A plan may have ONE launch (or graph replay, or
+ * backward) in flight: its side streams and fork/join events are the plan's own,
+ * so two launches of one plan on different streams would wait on each other's
+ * events; give concurrent launches plans of their own. */
 clm_status_t clm_expm_batched(clm_ctx *ctx, clm_stream_t stream, const clm_batch_plan *plan,
                               void *dWork, size_t lwork_bytes, clm_batch_result *dResults);
 /* The same launch captured once into a hipGraph (per stream, workspace and

@@ -79,8 +79,11 @@ def lib():
         for f in ("orc_pade_d", "orc_pade_z"):
             getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                       ctypes.c_int, _ip]
-        for f in ("orc_expm_d", "orc_expm_z", "orc_expm_taylor_d", "orc_expm_taylor_z"):
+        for f in ("orc_expm_d", "orc_expm_z"):
             getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip]
+        for f in ("orc_expm_taylor_d", "orc_expm_taylor_z"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                      _dp]
         L.orc_ps_plan.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
         for f in ("orc_ps_d", "orc_ps_z"):
             getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
@@ -93,6 +96,8 @@ def lib():
         L.orc_inv_factorials.restype = _dp
         L.orc_taylor_plan_d.argtypes = [ctypes.c_double, _ip]
         L.orc_taylor_plan_s.argtypes = [ctypes.c_double, _ip]
+        L.orc_taylor_plan_powers_d.argtypes = [_dp, ctypes.c_int, _ip]
+        L.orc_taylor_plan_powers_s.argtypes = [_dp, ctypes.c_int, _ip]
         _lib = L
     return _lib
 
@@ -273,18 +278,28 @@ def expm_status(A) -> int:
 
 
 def expm_taylor(A, return_plan: bool = False):
-    """The build's algorithm on the CPU: Taylor / Paterson-Stockmeyer + squaring."""
+    """The build's algorithm on the CPU: the norm-only plan's power bank formed
+    from A, the plan from the exact 1-norms of A .. A^q (taylor_plan_powers),
+    Taylor / Paterson-Stockmeyer at A / 2^s, s squarings. The plan, when
+    returned, is a dict: degree, s, num_gemms, ps_block, norm_powers (q) and
+    power_norm1 (||A^k||_1 for k = 2 .. q)."""
     c = _is_complex(A)
     dt = np.complex128 if c else np.float64
     A = _fortran(A, dt)
     n = A.shape[0]
     E = np.zeros((n, n), dtype=dt, order="F")
-    plan = (ctypes.c_int * 4)()
+    plan = (ctypes.c_int * 5)()
+    pnorm = np.zeros(4)
     f = lib().orc_expm_taylor_z if c else lib().orc_expm_taylor_d
-    st = f(n, _ptr(A), n, _ptr(E), n, plan)
+    st = f(n, _ptr(A), n, _ptr(E), n, plan, _ptr(pnorm))
     if st != 0:
         raise ValueError(f"orc_expm_taylor status {st}")
-    return (E, tuple(plan)) if return_plan else E
+    if not return_plan:
+        return E
+    q = int(plan[4])
+    info = dict(zip(("degree", "s", "num_gemms", "ps_block", "norm_powers"), tuple(plan)))
+    info["power_norm1"] = [float(v) for v in pnorm[:max(q - 1, 0)]]
+    return E, info
 
 
 # ---- Frechet derivative -----------------------------------------------------------
@@ -361,8 +376,20 @@ def taylor_theta(degree: int, single: bool = False) -> float:
 
 
 def taylor_plan(norm1: float, single: bool = False):
+    """The plan from the 1-norm alone."""
     out = (ctypes.c_int * 4)()
     (lib().orc_taylor_plan_s if single else lib().orc_taylor_plan_d)(norm1, out)
+    return dict(zip(("degree", "s", "num_gemms", "ps_block"), tuple(out)))
+
+
+def taylor_plan_powers(norms, single: bool = False):
+    """The plan from norms[k-1] = ||A^k||_1, k = 1 .. q: s from
+    alpha_p = max(||A^p||^(1/p), ||A^(p+1)||^(1/(p+1))) (Al-Mohy & Higham 2009,
+    Thm 4.2), the q - 1 powers already formed counted once."""
+    pn = np.ascontiguousarray(norms, dtype=np.float64)
+    out = (ctypes.c_int * 4)()
+    f = lib().orc_taylor_plan_powers_s if single else lib().orc_taylor_plan_powers_d
+    f(_ptr(pn), len(pn), out)
     return dict(zip(("degree", "s", "num_gemms", "ps_block"), tuple(out)))
 
 

@@ -1308,8 +1308,11 @@ static void build_block(int n, double *Bk, int ldb, const double *A, int lda,
   }
 }
 
-static int ps_impl(int n, const double *coeffs, int degree, const double *A, int lda, double *P,
-                   int ldp, int s_requested, int cplx) {
+/* Paterson-Stockmeyer; the first `have` powers (A^2 .. A^(have+1)) may be
+ * given in pre (n x n, leading dimension n), the rest are formed here. */
+static int ps_impl_have(int n, const double *coeffs, int degree, const double *A, int lda,
+                        double *P, int ldp, int s_requested, int cplx, double *const *pre,
+                        int have) {
   if (n < 1 || degree < 0 || lda < n || ldp < n || s_requested < 0) return -1;
   if (coeffs == NULL || A == NULL || P == NULL) return -1;
   int pl[5];
@@ -1317,12 +1320,21 @@ static int ps_impl(int n, const double *coeffs, int degree, const double *A, int
   const int s = pl[0], r = pl[1], num_powers = pl[2];
   const size_t es = cplx ? 2 : 1;
   const size_t nn = (size_t)n * n;
-  double **pw = (double **)calloc((size_t)(num_powers > 0 ? num_powers : 1), sizeof(double *));
-  for (int k = 0; k < num_powers; ++k) pw[k] = (double *)malloc(sizeof(double) * es * nn);
+  const int npw = num_powers > have ? num_powers : have;
+  double **pw = (double **)calloc((size_t)(npw > 0 ? npw : 1), sizeof(double *));
+  int *owned = (int *)calloc((size_t)(npw > 0 ? npw : 1), sizeof(int));
+  for (int k = 0; k < npw; ++k) {
+    if (k < have && pre) {
+      pw[k] = pre[k];
+    } else {
+      pw[k] = (double *)malloc(sizeof(double) * es * nn);
+      owned[k] = 1;
+    }
+  }
   double *scratch = (r >= 1) ? (double *)malloc(sizeof(double) * es * nn) : NULL;
   const double one_z[2] = {1.0, 0.0}, zero_z[2] = {0.0, 0.0};
 
-  for (int k = 2; k <= num_powers + 1; ++k) {
+  for (int k = 2 + (pre ? have : 0); k <= num_powers + 1; ++k) {
     const double *left = (k == 2) ? A : pw[k - 3];
     const int ldl = (k == 2) ? lda : n;
     if (cplx) orc_gemm_z(n, one_z, left, ldl, A, lda, zero_z, pw[k - 2], n);
@@ -1343,10 +1355,17 @@ static int ps_impl(int n, const double *coeffs, int degree, const double *A, int
     cur = next;
   }
 #undef TERMS
-  for (int k = 0; k < num_powers; ++k) free(pw[k]);
+  for (int k = 0; k < npw; ++k)
+    if (owned[k]) free(pw[k]);
   free(pw);
+  free(owned);
   free(scratch);
   return 0;
+}
+
+static int ps_impl(int n, const double *coeffs, int degree, const double *A, int lda, double *P,
+                   int ldp, int s_requested, int cplx) {
+  return ps_impl_have(n, coeffs, degree, A, lda, P, ldp, s_requested, cplx, NULL, 0);
 }
 
 int orc_ps_d(int n, const double *coeffs, int degree, const double *A, int lda, double *P, int ldp,
@@ -1432,18 +1451,47 @@ static double taylor_theta(int degree, const double *tab) {
 double orc_taylor_theta_d(int degree) { return taylor_theta(degree, kTaylorThetaD); }
 double orc_taylor_theta_s(int degree) { return taylor_theta(degree, kTaylorThetaS); }
 
-/* Minimize ps_gemms(d) + s over the ladder; ties go to the smaller s. */
-static void taylor_plan(double norm1, const double *tab, int *out) {
+/* Squarings rung i needs from the 1-norm alone; a ratio that overflows marks
+ * the rung unusable (a huge norm against a tiny theta). */
+#define ORC_UNUSABLE_SQUARINGS (1 << 20)
+static int norm_squarings(double norm1, double theta) {
+  if (!(norm1 > theta)) return 0;
+  const double ratio = norm1 / theta;
+  if (!(ratio <= DBL_MAX)) return ORC_UNUSABLE_SQUARINGS;
+  const int s = (int)ceil(log2(ratio));
+  return s < 0 ? 0 : s;
+}
+
+/* With pn[k-1] = ||A^k||_1 for k = 1..q: the smallest s with
+ * alpha_p / 2^s <= theta_m for some p with p(p-1) <= m+1, where
+ * alpha_p = max(||A^p||^(1/p), ||A^(p+1)||^(1/(p+1))) (Al-Mohy & Higham 2009,
+ * Thm 4.2(a)); alpha_1 = ||A||_1. d_k <= x is tested as ||A^k|| <= x^k. */
+static int power_squarings(const double *pn, int q, int m, double theta) {
+  const int s_norm = norm_squarings(pn[0], theta);
+  if (q < 3 || s_norm == 0 || s_norm >= ORC_UNUSABLE_SQUARINGS) return s_norm;
+  for (int s = 0; s < s_norm; ++s) {
+    const double x = ldexp(theta, s);
+    double xp = x * x;
+    for (int p = 2; p + 1 <= q && p * (p - 1) <= m + 1; ++p) {
+      const double xp1 = xp * x;
+      if (pn[p - 1] <= xp && pn[p] <= xp1) return s;
+      xp = xp1;
+    }
+  }
+  return s_norm;
+}
+
+/* Minimize max(ps_powers(d), q-1) + ps_horner(d) + s over the ladder (the q-1
+ * powers formed to take their norms count once); ties go to the smaller s.
+ * q = 1: the plan from the 1-norm alone. */
+static void taylor_plan_powers(const double *pn, int q, const double *tab, int *out) {
+  const int have = q > 1 ? q - 1 : 0;
   int best = -1, best_cost = 0, best_s = 0;
   for (int i = 0; i < 10; ++i) {
     int pl[5];
     orc_ps_plan(kTaylorDegrees[i], 0, pl);
-    int s = 0;
-    if (norm1 > tab[i]) {
-      s = (int)ceil(log2(norm1 / tab[i]));
-      if (s < 0) s = 0;
-    }
-    const int cost = pl[4] + s;
+    const int s = power_squarings(pn, q, kTaylorDegrees[i], tab[i]);
+    const int cost = (pl[2] > have ? pl[2] : have) + pl[1] + s;
     if (best < 0 || cost < best_cost || (cost == best_cost && s < best_s)) {
       best = i; best_cost = cost; best_s = s;
     }
@@ -1452,17 +1500,57 @@ static void taylor_plan(double norm1, const double *tab, int *out) {
   orc_ps_plan(kTaylorDegrees[best], 0, pl);
   out[0] = kTaylorDegrees[best]; out[1] = best_s; out[2] = best_cost; out[3] = pl[0];
 }
-void orc_taylor_plan_d(double norm1, int *out) { taylor_plan(norm1, kTaylorThetaD, out); }
-void orc_taylor_plan_s(double norm1, int *out) { taylor_plan(norm1, kTaylorThetaS, out); }
+void orc_taylor_plan_d(double norm1, int *out) { taylor_plan_powers(&norm1, 1, kTaylorThetaD, out); }
+void orc_taylor_plan_s(double norm1, int *out) { taylor_plan_powers(&norm1, 1, kTaylorThetaS, out); }
+void orc_taylor_plan_powers_d(const double *pn, int q, int *out) {
+  taylor_plan_powers(pn, q, kTaylorThetaD, out);
+}
+void orc_taylor_plan_powers_s(const double *pn, int q, int *out) {
+  taylor_plan_powers(pn, q, kTaylorThetaS, out);
+}
 
+/* The powers whose norms are taken: the norm-only plan's bank A^2..A^q when it
+ * holds A^3 and ||A||_1 <= 2^200 (1 = none). */
+static int norm_power_count(double norm1, int degree) {
+  if (!(norm1 <= 0x1p200)) return 1;
+  int pl[5];
+  orc_ps_plan(degree, 0, pl);
+  const int q = pl[2] + 1;
+  if (q < 3) return 1;
+  return q < 5 ? q : 5;
+}
+
+/* The build's Taylor expm: the norm-only plan's power bank formed from A itself,
+ * its exact 1-norms, the plan of taylor_plan_powers, then A / 2^s and the bank
+ * scaled in place by 2^(-k s) (ldexp, exact for normal results), the polynomial
+ * continued from the powers already formed, s squarings. plan[4] = q and
+ * pnorm[k-2] = ||A^k||_1 (k = 2..q) when given. */
 static int expm_taylor_impl(int n, const double *A, int lda, double *E, int lde, int *plan,
-                            int cplx) {
+                            double *pnorm, int cplx) {
   if (n < 1 || n > ORC_MAX_DIM || lda < n || lde < n || A == NULL || E == NULL) return -1;
   const double norm = cplx ? orc_norm1_z(n, A, lda) : orc_norm1_d(n, A, lda);
   if (!isfinite(norm)) return -2;
+  const size_t es = cplx ? 2 : 1;
+  const size_t nn = (size_t)n * n;
+  int p0[4];
+  orc_taylor_plan_d(norm, p0);
+  const int q = norm_power_count(norm, p0[0]);
+  double pn[5] = {norm, 0, 0, 0, 0};
+  double *pw[4] = {NULL, NULL, NULL, NULL};
+  const double one_z[2] = {1.0, 0.0}, zero_z[2] = {0.0, 0.0};
+  for (int k = 2; k <= q; ++k) {
+    pw[k - 2] = (double *)malloc(sizeof(double) * es * nn);
+    const double *left = (k == 2) ? A : pw[k - 3];
+    const int ldl = (k == 2) ? lda : n;
+    if (cplx) orc_gemm_z(n, one_z, left, ldl, A, lda, zero_z, pw[k - 2], n);
+    else orc_gemm_d(n, 1.0, left, ldl, A, lda, 0.0, pw[k - 2], n);
+    pn[k - 1] = cplx ? orc_norm1_z(n, pw[k - 2], n) : orc_norm1_d(n, pw[k - 2], n);
+  }
   int p[4];
-  orc_taylor_plan_d(norm, p);
-  if (plan) { plan[0] = p[0]; plan[1] = p[1]; plan[2] = p[2]; plan[3] = p[3]; }
+  taylor_plan_powers(pn, q, kTaylorThetaD, p);
+  if (plan) { plan[0] = p[0]; plan[1] = p[1]; plan[2] = p[2]; plan[3] = p[3]; plan[4] = q; }
+  if (pnorm)
+    for (int k = 2; k <= 5; ++k) pnorm[k - 2] = (k <= q) ? pn[k - 1] : 0.0;
   const double *src = A;
   int ld_src = lda;
   double *As = NULL;
@@ -1470,21 +1558,26 @@ static int expm_taylor_impl(int n, const double *A, int lda, double *E, int lde,
     As = scaled_copy(n, A, lda, p[1], cplx);
     src = As;
     ld_src = n;
+    for (int k = 2; k <= q; ++k)
+      for (size_t e = 0; e < es * nn; ++e) pw[k - 2][e] = ldexp(pw[k - 2][e], -k * p[1]);
   }
   double coef[66];
   for (int k = 0; k <= p[0]; ++k) {
     if (cplx) { coef[2 * k] = kInvFact[k]; coef[2 * k + 1] = 0.0; }
     else coef[k] = kInvFact[k];
   }
-  ps_impl(n, coef, p[0], src, ld_src, E, lde, p[3], cplx);
+  ps_impl_have(n, coef, p[0], src, ld_src, E, lde, p[3], cplx, pw, q - 1);
   free(As);
+  for (int k = 0; k < 4; ++k) free(pw[k]);
   square_s_times(n, p[1], E, lde, cplx);
   return 0;
 }
 
-int orc_expm_taylor_d(int n, const double *A, int lda, double *E, int lde, int *plan) {
-  return expm_taylor_impl(n, A, lda, E, lde, plan, 0);
+int orc_expm_taylor_d(int n, const double *A, int lda, double *E, int lde, int *plan,
+                      double *pnorm) {
+  return expm_taylor_impl(n, A, lda, E, lde, plan, pnorm, 0);
 }
-int orc_expm_taylor_z(int n, const double *A, int lda, double *E, int lde, int *plan) {
-  return expm_taylor_impl(n, A, lda, E, lde, plan, 1);
+int orc_expm_taylor_z(int n, const double *A, int lda, double *E, int lde, int *plan,
+                      double *pnorm) {
+  return expm_taylor_impl(n, A, lda, E, lde, plan, pnorm, 1);
 }

@@ -105,8 +105,15 @@ const double *orc_inv_factorials(void); /* 1/k!, k = 0..32, correctly rounded */
 /* out[4] = {degree, s, num_gemms, ps_block} */
 void orc_taylor_plan_d(double norm1, int *out);
 void orc_taylor_plan_s(double norm1, int *out);
-int orc_expm_taylor_d(int n, const double *A, int lda, double *E, int lde, int *plan);
-int orc_expm_taylor_z(int n, const double *A, int lda, double *E, int lde, int *plan);
+/* the plan from pn[k-1] = ||A^k||_1, k = 1..q (out: degree, s, products, block) */
+void orc_taylor_plan_powers_d(const double *pn, int q, int *out);
+void orc_taylor_plan_powers_s(const double *pn, int q, int *out);
+/* plan: {degree, s, products, block, q}; pnorm (4 doubles, may be NULL):
+ * ||A^k||_1 for k = 2..q of the unscaled powers */
+int orc_expm_taylor_d(int n, const double *A, int lda, double *E, int lde, int *plan,
+                      double *pnorm);
+int orc_expm_taylor_z(int n, const double *A, int lda, double *E, int lde, int *plan,
+                      double *pnorm);
 
 /* threads the oracle uses (OpenMP) */
 int orc_num_threads(void);

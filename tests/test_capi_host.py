@@ -58,6 +58,73 @@ def test_single_plans_match_oracle(norm):
     assert (q.m, q.s, q.num_gemms) == oracle.expm_plan(norm, single=True)
 
 
+def _plan_powers(norms, single=False):
+    arr = (ctypes.c_double * len(norms))(*norms)
+    f = L.clm_expm_plan_powers_s if single else L.clm_expm_plan_powers_d
+    return f(arr, len(norms))
+
+
+# norms of A .. A^5 of the n = 2048 N(0,1) matrices at ||A||_1 = 0.5 / 8 / 100 (their
+# k-th roots: 0.5 .082 .045 .033; 8 1.32 .71 .52 .44; 100 16.4 8.98 6.60 5.50)
+_CFG2_NORMS = [
+    [0.5, 0.082 ** 2, 0.045 ** 3, 0.033 ** 4],
+    [8.0, 1.319 ** 2, 0.710 ** 3, 0.524 ** 4, 0.436 ** 5],
+    [100.0, 16.389 ** 2, 8.978 ** 3, 6.604 ** 4, 5.504 ** 5],
+]
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_power_plan_matches_oracle(single):
+    rng = np.random.default_rng(5)
+    cases = list(_CFG2_NORMS)
+    for _ in range(300):
+        q = int(rng.integers(1, 6))
+        nrm = float(10.0 ** rng.uniform(-6, 3.5))
+        # ||A^k|| <= ||A||^k; the k-th roots anywhere from tiny to ||A||
+        roots = nrm * 10.0 ** rng.uniform(-3, 0, size=q - 1)
+        cases.append([nrm] + [float(r ** k) for k, r in zip(range(2, q + 1), roots)])
+    for norms in cases:
+        p = _plan_powers(norms, single)
+        t = oracle.taylor_plan_powers(norms, single)
+        assert (p.m, p.s, p.num_gemms, p.ps_block) == (t["degree"], t["s"], t["num_gemms"],
+                                                       t["ps_block"]), norms
+        assert p.norm_powers == len(norms) and p.norm1 == norms[0]
+        assert list(p.power_norm1)[:len(norms) - 1] == norms[1:]
+
+
+def test_power_plan_known_answers():
+    # the n = 2048 benchmark matrices: (m, s, products) drop from (15,0,6) / (24,2,10) /
+    # (29,5,14) with the 1-norm alone
+    got = [(p.m, p.s, p.num_gemms) for p in map(_plan_powers, _CFG2_NORMS)]
+    assert got == [(8, 0, 5), (15, 0, 7), (29, 1, 10)]
+    # norms of a normal matrix (||A^k|| = ||A||^k): the plan from the 1-norm alone
+    for nrm in (0.3, 4.0, 100.0, 1e4):
+        q0 = min(oracle.ps_plan(oracle.taylor_plan(nrm)["degree"])["num_powers"] + 1, 5)
+        p = _plan_powers([nrm ** k for k in range(1, q0 + 1)])
+        t = oracle.taylor_plan(nrm)
+        assert (p.m, p.s, p.num_gemms) == (t["degree"], t["s"], t["num_gemms"])
+    # A^2 = 0 (nilpotent of index 2): exp(A) = I + A, degree 1, no squaring, the
+    # formed powers still counted
+    p = _plan_powers([50.0, 0.0, 0.0, 0.0, 0.0])
+    assert (p.m, p.s, p.num_gemms) == (1, 0, 4)
+    # q = 1 or 2 cannot refine (alpha_2 needs ||A^3||)
+    for norms in ([100.0], [100.0, 1e-6]):
+        p = _plan_powers(norms)
+        t = oracle.taylor_plan(100.0)
+        assert (p.m, p.s) == (t["degree"], t["s"])
+
+
+def test_plan_huge_norms_defined():
+    """A norm whose ratio to theta_1 overflows: no undefined cast, the rung is skipped
+    and the plan squares (the reference's ladder squares too)."""
+    for nrm in (1e293, 1e300, 1.7e308):
+        p = L.clm_expm_plan_d(nrm)
+        t = oracle.taylor_plan(nrm)
+        assert p.m > 1 and p.s > 900 and (p.m, p.s) == (t["degree"], t["s"])
+        q = L.clm_expm_plan_s(nrm)
+        assert q.m > 1 and q.s > 900
+
+
 def test_thresholds_match_oracle():
     for m in list(range(0, 35)):
         assert L.clm_taylor_theta_d(m) == oracle.taylor_theta(m)
@@ -267,3 +334,28 @@ def test_split_schedule(norm, want):
     p = expm_split_plan(norm)
     assert (p["s"], p["ps_split"], p["split_squarings"]) == want
     assert p["num_gemms"] == oracle.ps_plan(p["m"], 0)["num_gemms"] + p["s"]
+
+
+def _split_plan_powers(n, norms):
+    arr = (ctypes.c_double * len(norms))(*norms)
+    opts = _lib.SplitOpts(0, -1, _lib.PLAN_POWERS)
+    return L.clm_expm_split_plan_powers_d(n, arr, len(norms), ctypes.byref(opts))
+
+
+def test_split_power_plan_rules():
+    # the n = 2048 benchmark matrices: no squaring at 0.5 and 8, one at 100, all split
+    got = [(p.m, p.s, p.ps_split, p.split_squarings)
+           for p in (_split_plan_powers(2048, nm) for nm in _CFG2_NORMS)]
+    assert got == [(5, 0, 3, 0), (11, 0, 13, 0), (24, 1, 13, 1)]
+    # above ||A||_1 = 256 the norms of powers are not used: the 1-norm's plan
+    for nrm in (300.0, 1000.0, 3000.0):
+        norms = [nrm, (0.1 * nrm) ** 2, (0.05 * nrm) ** 3, (0.04 * nrm) ** 4, (0.03 * nrm) ** 5]
+        p = _split_plan_powers(1024, norms)
+        q = L.clm_expm_split_plan_d(nrm, None)
+        assert (p.m, p.s, p.ps_split, p.split_squarings, p.norm_powers) == (
+            q.m, q.s, q.ps_split, q.split_squarings, 1)
+    # orders up to 64 take the 6-way polynomial where larger ones take half planes
+    norms = [100.0, 26.7 ** 2, 17.3 ** 3, 13.8 ** 4, 12.1 ** 5]
+    small, large = _split_plan_powers(64, norms), _split_plan_powers(65, norms)
+    assert small.ps_split == _lib.SPLIT_BF16X6 and large.ps_split == _lib.SPLIT_FP16X3
+    assert small.s <= 5 and large.s <= 5

@@ -150,7 +150,9 @@ def test_batched_non_finite_member(dev):
     torch.cuda.synchronize()
     r = BatchPlan.decode_results(res)
     assert r[1]["status"] == _lib.INVALID_VALUE
-    assert np.all(entries[1][3].cpu().numpy() == -5.0)  # untouched
+    out = entries[1][3].cpu().numpy()
+    assert np.all(np.isnan(out[:, :30]))  # no exponential: NaN, never stale memory
+    assert np.all(out[:, 30:] == -5.0)  # the padding rows untouched
     for i in (0, 2):
         want, _ = _single(inputs[i], dev)
         assert np.array_equal(entries[i][3].cpu().numpy()[:, :entries[i][0]].T, want)
@@ -220,3 +222,25 @@ def test_expm_batched_convenience(dev):
     for a, e in zip(mats, outs):
         want = cg.expm(a)
         assert torch.equal(e, want)
+
+
+def test_expm_batched_convenience_raises_on_non_finite_member(dev):
+    rng = np.random.default_rng(8)
+    mats = [torch.from_numpy(_matrix(rng, n, torch.float32, 2.0)).to(dev) for n in (9, 17, 5)]
+    mats[1][2, 3] = float("inf")
+    with pytest.raises(cg.CalamanError, match=r"members \[1\]"):
+        cg.expm_batched(mats)
+
+
+def test_expm_batched_convenience_on_a_side_stream(dev):
+    """inputs made on the current stream, the launch on another: the wrapper orders
+    the two and hands the results back to the current stream"""
+    rng = np.random.default_rng(9)
+    host = [_matrix(rng, n, torch.complex64, 4.0) for n in (64, 130)]
+    want = [cg.expm(torch.from_numpy(h).to(dev)) for h in host]
+    side = torch.cuda.Stream()
+    for _ in range(3):
+        mats = [torch.from_numpy(h).to(dev) * 1.0 for h in host]  # made on the current stream
+        outs = cg.expm_batched(mats, stream=side)
+        for o, w in zip(outs, want):
+            assert torch.equal(o, w)

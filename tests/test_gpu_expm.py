@@ -72,13 +72,22 @@ def test_expm_matches_oracle(dev, cplx, n, target):
     ref, pade_plan = oracle.expm(a, return_plan=True)
     err = rel(got, ref)
     assert err <= PARITY, f"n={n} |A|={target} plan={plan} err={err:.3e}"
-    # the executed plan is the one predicted from the norm
-    tp = oracle.taylor_plan(plan["norm1"])
+    # the executed plan is the rule applied to the norms of A .. A^q the device took
+    norms = [plan["norm1"]] + plan["power_norm1"]
+    tp = oracle.taylor_plan_powers(norms)
     assert (plan["m"], plan["s"], plan["num_gemms"], plan["ps_block"]) == (
         tp["degree"], tp["s"], tp["num_gemms"], tp["ps_block"])
+    # the powers whose norms are taken: the norm-only plan's bank, from A^3 on
+    p0 = oracle.taylor_plan(plan["norm1"])
+    q0 = oracle.ps_plan(p0["degree"])["num_powers"] + 1
+    assert plan["norm_powers"] == (min(q0, 5) if q0 >= 3 else 1)
+    assert plan["s"] <= p0["s"]
     # and agrees with the CPU statement of the same algorithm more tightly
-    tay = oracle.expm_taylor(a)
+    tay, tplan = oracle.expm_taylor(a, return_plan=True)
     assert rel(got, tay) <= 1e-13
+    if not cplx:  # the FP64 products are the oracle's bit for bit: so are the norms
+        assert norms == [plan["norm1"]] + tplan["power_norm1"]
+        assert (plan["m"], plan["s"]) == (tplan["degree"], tplan["s"])
 
 
 @pytest.mark.parametrize("cplx", [False, True])
@@ -229,3 +238,30 @@ def test_undersized_workspace(dev):
     work = torch.empty(need - 1, dtype=torch.uint8, device=dev)
     st, _ = expm_cm(n, da, n, de, n, work)
     assert st == _lib.ALLOC_FAILED
+
+
+def test_convenience_wrappers_on_a_side_stream(dev):
+    """expm / expm_split / paterson_stockmeyer / horner / gemm / lange / lascl with a
+    stream other than the current one: the input is made on the current stream,
+    the wrapper orders its work after it and hands the result back; every result
+    equals the same call on the current stream, over repeated rounds"""
+    rng = np.random.default_rng(21)
+    n = 300
+    host = scaled_gaussian(n, 6.0, seed=21)
+    coef = torch.from_numpy(rng.standard_normal(9)).to(dev)
+    side = torch.cuda.Stream()
+    base = torch.from_numpy(host).to(dev)
+    want = {
+        "expm": cg.expm(base), "split": cg.expm_split(base),
+        "ps": cg.paterson_stockmeyer(coef, base), "horner": cg.horner(coef, base),
+        "gemm": cg.gemm(base, base), "lange": cg.lange("one", base),
+        "lascl": cg.lascl(4.0, 1.0, base)}
+    for _ in range(3):
+        a = base * 1.0  # made on the current stream just before the call
+        got = {
+            "expm": cg.expm(a, stream=side), "split": cg.expm_split(a, stream=side),
+            "ps": cg.paterson_stockmeyer(coef, a, stream=side),
+            "horner": cg.horner(coef, a, stream=side), "gemm": cg.gemm(a, a, stream=side),
+            "lange": cg.lange("one", a, stream=side), "lascl": cg.lascl(4.0, 1.0, a, stream=side)}
+        for k in want:
+            assert torch.equal(got[k], want[k]), k

@@ -144,8 +144,8 @@ def test_expm_single_accuracy(dev, cplx, n, target):
         torch_err = _rel(torch.linalg.matrix_exp(t).cpu().numpy(), ref)
         bound = max(4 * torch_err, 32 * EPS32)
     assert err <= bound, f"n={n} |A|={target} plan={plan} err={err:.3e} torch32={torch_err:.3e}"
-    # the plan is the single-precision ladder's for this norm
-    tp = cg.expm_plan(plan["norm1"], _tdt(cplx))
+    # the plan is the single-precision ladder's for the norms of A .. A^q
+    tp = cg.expm_plan(plan["norm1"], _tdt(cplx), plan["power_norm1"])
     assert (plan["m"], plan["s"], plan["num_gemms"]) == (tp["m"], tp["s"], tp["num_gemms"])
 
 

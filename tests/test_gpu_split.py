@@ -96,13 +96,19 @@ def test_expm_split_meets_1e6(dev, n, target):
     assert err <= 1e-6, (plan, err)
     s = plan["s"]
     small = plan["norm1"] / 2.0 ** s <= 1.0 + 2.0 ** -20
-    assert plan["ps_split"] == ((3 if small else 13) if s <= 5 else 6)
-    # the last squarings split (2 for s = 4, 5, else 1); with s > 8 every squaring
-    # stays FP64
-    want_t = 0 if (s == 0 or s > 8) else (2 if (s in (4, 5) and not small) else 1)
+    half = 6 if n <= 64 else 13  # orders up to 64 take the 6-way polynomial
+    assert plan["ps_split"] == ((3 if small else half) if s <= 5 else 6)
+    # the last squarings split (2 for s = 4, 5 with half planes, else 1); with s > 8
+    # every squaring stays FP64
+    want_t = 0 if (s == 0 or s > 8) else (2 if (s in (4, 5) and plan["ps_split"] == 13) else 1)
     assert plan["split_squarings"] == want_t
-    # the plan is the one the host predicts
-    assert plan == {**cg.expm_split_plan(plan["norm1"]), "dtype": 1}
+    # the plan is the one the host predicts from the norms of A .. A^q (taken only
+    # up to ||A||_1 = 256)
+    assert plan == {**cg.expm_split_plan(plan["norm1"], power_norms=plan["power_norm1"], n=n),
+                    "dtype": 1}
+    assert plan["s"] <= cg.expm_split_plan(plan["norm1"], n=n)["s"]
+    if plan["norm1"] > 256:
+        assert plan["norm_powers"] == 1
 
 
 def test_expm_split_2048(dev):

@@ -164,12 +164,16 @@ def _theta_bisect(m, u, K=140):
     return float(lo)
 
 
-def test_theta_rederived():
-    u = 2.0 ** -53
-    assert _theta_bisect(18, u) == pytest.approx(1.090863719290036, rel=1e-12)
-    assert _theta_bisect(8, u) == pytest.approx(oracle.taylor_theta(8), rel=1e-12)
-    assert _theta_bisect(11, 2.0 ** -24) == pytest.approx(oracle.taylor_theta(11, single=True),
-                                                          rel=1e-12)
+def test_theta18_rederived():
+    assert _theta_bisect(18, 2.0 ** -53) == pytest.approx(1.090863719290036, rel=1e-12)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("m", oracle.TAYLOR_DEGREES)
+def test_theta_rederived(m, single):
+    """every one of the 20 ladder thresholds, from the exact series"""
+    u = 2.0 ** -24 if single else 2.0 ** -53
+    assert _theta_bisect(m, u) == pytest.approx(oracle.taylor_theta(m, single), rel=1e-12)
 
 
 def test_taylor_ladder_costs():
@@ -192,3 +196,71 @@ def test_taylor_plan_rule():
         assert (p["degree"], p["s"], p["num_gemms"]) == (d, s, g)
         assert norm / 2 ** s <= oracle.taylor_theta(d)
     assert oracle.taylor_plan(0.0)["num_gemms"] == 0
+
+
+# ---- squarings from the norms of powers ---------------------------------------------------------
+def _alpha_ok(norms, m, x):
+    """alpha_p <= x for some p with p(p-1) <= m+1 (p = 1: ||A||_1), k-th roots in
+    float arithmetic -- an independent statement of the power rule"""
+    if norms[0] <= x:
+        return True
+    q = len(norms)
+    for p in range(2, q):
+        if p * (p - 1) <= m + 1:
+            if max(norms[p - 1] ** (1.0 / p), norms[p] ** (1.0 / (p + 1))) <= x * (1 + 1e-12):
+                return True
+    return False
+
+
+def test_power_plan_rule():
+    """The plan from norms of powers: s is the smallest with alpha_p / 2^s <= theta_m
+    (Al-Mohy & Higham 2009, Thm 4.2) and the rung minimises products + s, with the
+    formed powers counted once -- checked by brute force over rungs and s."""
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        q = int(rng.integers(3, 6))
+        nrm = float(10.0 ** rng.uniform(-3, 3))
+        roots = nrm * 10.0 ** rng.uniform(-2.5, 0, size=q - 1)
+        norms = [nrm] + [float(r ** k) for k, r in zip(range(2, q + 1), roots)]
+        p = oracle.taylor_plan_powers(norms)
+        best = None
+        for m in oracle.TAYLOR_DEGREES:
+            th = oracle.taylor_theta(m)
+            s = 0
+            while not _alpha_ok(norms, m, th * 2.0 ** s):
+                s += 1
+            pp = oracle.ps_plan(m)
+            cost = max(pp["num_powers"], q - 1) + pp["r"] + s
+            if best is None or cost < best[2] or (cost == best[2] and s < best[1]):
+                best = (m, s, cost)
+        # the float k-th roots may sit an ulp either side of a threshold: allow one
+        # squaring of slack there, never a cheaper plan than the brute force finds
+        assert p["num_gemms"] <= best[2] + 1 and p["num_gemms"] >= best[2] - 1, (norms, p, best)
+        assert _alpha_ok(norms, p["degree"], oracle.taylor_theta(p["degree"]) * 2.0 ** p["s"] *
+                         (1 + 1e-9))
+        if p["s"] > 0:  # minimal: one squaring fewer violates the bound for this degree
+            assert not _alpha_ok(norms, p["degree"],
+                                 oracle.taylor_theta(p["degree"]) * 2.0 ** (p["s"] - 1) * (1 - 1e-9))
+
+
+def test_power_plan_expm_accuracy_non_normal():
+    """Matrices far from normal (||A||_1 >> spectral radius): the plan from power norms
+    squares far less than the norm-only plan and the Taylor expm stays within 1e-13
+    of the Pade oracle; an upper-triangular Jordan-like block and a dense Gaussian."""
+    rng = np.random.default_rng(3)
+    n = 48
+    cases = []
+    h = n // 2  # [[D1, B], [0, D2]]: ||B||_1 = 60 against a spectrum in [-0.3, 0.3]
+    blk = np.zeros((n, n))
+    blk[:h, h:] = rng.uniform(0, 1, (h, h))
+    blk *= 60.0 / np.abs(blk).sum(0).max()
+    blk += np.diag(rng.uniform(-0.3, 0.3, n))
+    cases.append(blk)
+    g = rng.standard_normal((n, n))
+    cases.append(g * (100.0 / np.abs(g).sum(0).max()))
+    for a in cases:
+        e, plan = oracle.expm_taylor(a, return_plan=True)
+        p0 = oracle.taylor_plan(np.abs(a).sum(0).max())
+        assert plan["s"] < p0["s"]
+        ref = oracle.expm(a)
+        assert np.linalg.norm(e - ref) / np.linalg.norm(ref) <= 1e-13

@@ -154,6 +154,7 @@ _SIGNATURES = {
     "clm_expm_buffer_size_d": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_buffer_size_z": (_i, [_i, ctypes.POINTER(_sz)]),
     "clm_expm_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
+    "clm_expm_noalloc_d": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, ctypes.POINTER(ExpmPlan)]),
     "clm_expm_z": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp, ctypes.POINTER(ExpmPlan)]),
     "clm_matrix_norm1_d": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_d)]),
     "clm_matrix_norm1_z": (_i, [_vp, _i, _vp, _i, _vp, ctypes.POINTER(_d)]),

@@ -1025,6 +1025,34 @@ clm_status_t lascl_impl(clm_stream_t stream, real_t<T> cfrom, real_t<T> cto, siz
   return CLM_STATUS_SUCCESS;
 }
 
+// ---- the FP64 expm entry point that allocates nothing -------------------------------------
+// Its workspace is the file-scope device array `scratch`: one copy per GPU, held
+// until the process ends, sized for the expm workspace at n = CLM_EXPM_NOALLOC_MAX_N.
+// In host code `scratch` names a host-side stand-in, not device memory; the
+// device address of each GPU's copy comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address is ever used. Nothing
+// in the workspace must start at zero: every region is written before it is read.
+constexpr size_t kNoAllocBytes = 4096 + 6 * (size_t(CLM_EXPM_NOALLOC_MAX_N) *
+                                              CLM_EXPM_NOALLOC_MAX_N * sizeof(double));
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mutex;
+void *g_noalloc_ptr[64] = {};
+
+// the device address of this GPU's `scratch` (looked up on the first call per device)
+clm_status_t noalloc_workspace(int device, void **out) {
+  if (device < 0 || device >= 64) return CLM_STATUS_NOT_INITIALIZED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr)
+      return CLM_STATUS_EXECUTION_FAILED;
+    g_noalloc_ptr[device] = ptr;
+  }
+  *out = g_noalloc_ptr[device];
+  return CLM_STATUS_SUCCESS;
+}
+
 }  // namespace
 }  // namespace clm
 
@@ -1170,6 +1198,25 @@ clm_status_t clm_expm_buffer_size_z(int n, size_t *bytes) {
   if (n < 1 || bytes == nullptr) return CLM_STATUS_INVALID_VALUE;
   *bytes = clm::expm_buffer_size<zd>(n);
   return CLM_STATUS_SUCCESS;
+}
+
+clm_status_t clm_expm_noalloc_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA,
+                                int lda, double *dE, int lde, int *dInfo, clm_expm_plan *plan) {
+  if (n < 1 || n > CLM_EXPM_NOALLOC_MAX_N) return CLM_STATUS_INVALID_VALUE;
+  if (clm::expm_buffer_size<double>(n) > clm::kNoAllocBytes) return CLM_STATUS_INVALID_VALUE;
+  if (lda < n || lde < n || dA == nullptr || dE == nullptr) return CLM_STATUS_INVALID_VALUE;
+  int dev = 0;
+  if (ctx) {
+    dev = ctx->device;
+  } else if (hipGetDevice(&dev) != hipSuccess) {
+    return CLM_STATUS_NOT_INITIALIZED;
+  }
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != dev) return CLM_STATUS_INVALID_VALUE;
+  void *work = nullptr;
+  CLM_TRY(clm::noalloc_workspace(dev, &work));
+  return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, work,
+                                clm::kNoAllocBytes, dInfo, plan);
 }
 
 clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,

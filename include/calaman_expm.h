@@ -174,6 +174,21 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
                         int lda, clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes,
                         int *dInfo, clm_expm_plan *plan);
 
+/* clm_expm_d without a workspace argument, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. The workspace is a file-scope
+ * device array inside the library (about 12 MiB per GPU, held until the process
+ * ends), so n is at most CLM_EXPM_NOALLOC_MAX_N (larger n: CLM_STATUS_INVALID_VALUE)
+ * and the call allocates nothing -- no hipMalloc, hipFree or allocating library
+ * routine. Every call on one GPU shares that GPU's array: calls that overlap on a
+ * GPU (two host threads, or two streams without ordering) race and must be
+ * serialized by the caller. ctx's device (or the current one for NULL) must be
+ * the current device. Otherwise clm_expm_d's contract: same kernels, same bits,
+ * one synchronization for the 1-norm and one for the norms of powers. */
+#define CLM_EXPM_NOALLOC_MAX_N 512
+clm_status_t clm_expm_noalloc_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA,
+                                int lda, double *dE, int lde, int *dInfo, clm_expm_plan *plan);
+
 /* ---- precision variant ---------------------------------------------------------------
  * No reference counterpart (the build's precision addition). The same algorithm
  * as clm_expm_d with its matrix products moved onto BF16 MFMA: each FP64 operand is

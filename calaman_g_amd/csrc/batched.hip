@@ -588,9 +588,9 @@ __device__ GemmArgs<T> square_args(int n, const T *A, long lda, const T *B, long
   return g;
 }
 
-// One matrix's chain. First the norm-only plan's power bank A^2 .. A^q of A
-// itself (q = norm_power_count, none below degree 8), whose 1-norms give the
-// executed plan; the bank and A are then scaled by 2^-(k s) (exact) to the bank
+// One matrix's chain. First A^2, A^3 of A itself (none when the norm-only plan's
+// degree is below 8), then A^4, A^5 while worth_next_power, whose 1-norms give
+// the executed plan; the bank and A are then scaled by 2^-(k s) (exact) to the bank
 // of A / 2^s. Then the chain as one sequence of products, so the product code
 // of the long loop is instantiated once per tile configuration:
 //   steps [0, nbank)             power bank A^2 .. A^(nbank+1) of the scaled A
@@ -626,19 +626,27 @@ __device__ void expm_one(const BatchDesc &d, char *work, const T *coef, char *ld
     wg_fill(n, E, long(d.lde), ops<T>::from_real(__builtin_nan("")));
     return;  // uniform: every lane holds the same norm
   }
-  const int npow =
+  // A^2, A^3 of A itself and their 1-norms, then A^4, A^5 while worth_next_power
+  int npow =
       uniform(norm_power_count(double(norm), taylor_plan(double(norm), kSingle).m, kSingle));
-  // the norm-only plan's bank A^2 .. A^npow of A itself, and its 1-norms
   double pn[kMaxNormPower] = {double(norm)};
-  for (int k = 2; k <= npow; ++k) {
-    const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
-    wg_gemm<T, CFG>(square_args(n, left, (k == 2) ? long(d.lda) : long(n), A, long(d.lda),
-                                bank + static_cast<long>(k - 2) * stride, long(n)),
-                    lds);
+  if (npow > 1) {
+    int formed = 1;
+    for (;;) {
+      for (int k = formed + 1; k <= npow; ++k) {
+        const T *left = (k == 2) ? A : bank + static_cast<long>(k - 3) * stride;
+        wg_gemm<T, CFG>(square_args(n, left, (k == 2) ? long(d.lda) : long(n), A, long(d.lda),
+                                    bank + static_cast<long>(k - 2) * stride, long(n)),
+                        lds);
+      }
+      for (int k = formed + 1; k <= npow; ++k)
+        pn[k - 1] =
+            double(uniform(wg_norm1(n, bank + static_cast<long>(k - 2) * stride, long(n), lds)));
+      formed = npow;
+      if (!worth_next_power(pn, npow, kSingle)) break;
+      npow = uniform(npow + 1);
+    }
   }
-  for (int k = 2; k <= npow; ++k)
-    pn[k - 1] =
-        double(uniform(wg_norm1(n, bank + static_cast<long>(k - 2) * stride, long(n), lds)));
   const clm_expm_plan p = taylor_plan_powers(pn, npow, kSingle);
   const int degree = uniform(p.m), nsq = uniform(p.s);
   const int npre = npow > 1 ? npow - 1 : 0;

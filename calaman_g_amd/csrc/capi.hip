@@ -13,6 +13,8 @@
 // Paterson-Stockmeyer instead of the reference's Pade approximant + LU: same
 // accuracy target (backward error <= u), no LU, and every product is an MFMA GEMM.
 // Argument checks come before any HIP call, so rejected calls never touch a device.
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -24,8 +26,14 @@
 
 namespace clm {
 namespace {
-std::mutex g_ctx_mutex;
-clm_ctx *g_default_ctx[64] = {};
+// the library's default context of each device (static storage, set up before
+// main: resolving one allocates and writes nothing)
+struct DefaultContexts {
+  clm_ctx c[64];
+  DefaultContexts() {
+    for (int i = 0; i < 64; ++i) c[i].device = i;
+  }
+} g_default_ctx;
 }  // namespace
 
 clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
@@ -35,14 +43,7 @@ clm_status_t resolve_ctx(clm_ctx *ctx, clm_ctx **out) {
   }
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return CLM_STATUS_NOT_INITIALIZED;
-  std::lock_guard<std::mutex> lock(g_ctx_mutex);
-  if (!g_default_ctx[dev]) {
-    clm_ctx *c = new clm_ctx;
-    c->device = dev;
-    c->stream = nullptr;
-    g_default_ctx[dev] = c;
-  }
-  *out = g_default_ctx[dev];
+  *out = &g_default_ctx.c[dev];
   return CLM_STATUS_SUCCESS;
 }
 
@@ -156,7 +157,8 @@ SplitChoice split_schedule(int s, double scaled_norm1, int n, const clm_split_op
 // kSplitPowerMaxNorm1): squarings per rung as in taylor_plan_powers at u = 2^-24,
 // the rung's cost weighted by format, the q - 1 powers already formed not counted
 // again.
-clm_expm_plan split_plan_powers(const double *pn, int q, int n, const clm_split_opts *opts) {
+clm_expm_plan split_plan_powers(const double *pn, int q, int n, const clm_split_opts *opts,
+                                double *cost_out = nullptr) {
   const double norm1 = pn[0];
   if (!(norm1 <= kSplitPowerMaxNorm1)) q = 1;
   const int have = q > 1 ? q - 1 : 0;
@@ -178,6 +180,7 @@ clm_expm_plan split_plan_powers(const double *pn, int q, int n, const clm_split_
       best_s = s;
     }
   }
+  if (cost_out) *cost_out = best_cost;
   clm_expm_plan p{};
   const clm_ps_plan pp = ps_plan(taylor_degree(best), 0);
   p.m = taylor_degree(best);
@@ -196,6 +199,19 @@ clm_expm_plan split_plan_powers(const double *pn, int q, int n, const clm_split_
 
 clm_expm_plan split_plan(double norm1, int n, const clm_split_opts *opts) {
   return split_plan_powers(&norm1, 1, n, opts);
+}
+
+// worth_next_power for the split variant's weighted cost: forming A^(q+1) (at the
+// weight w_new of the format it is formed in) must cost less than it saves.
+bool split_worth_next_power(const double *pn, int q, int n, const clm_split_opts *opts,
+                            double w_new) {
+  if (q < 3 || q >= kMaxNormPower) return false;
+  double trial[kMaxNormPower] = {};
+  for (int k = 0; k < q; ++k) trial[k] = pn[k];
+  double now = 0.0, with_next = 0.0;
+  (void)split_plan_powers(pn, q, n, opts, &now);
+  (void)split_plan_powers(trial, q + 1, n, opts, &with_next);
+  return with_next + w_new < now - 1e-12;
 }
 
 // ---- the reference's Pade ladder (src/expm/plan.cppm:51-130, expm_bridge.h:128-130) ----
@@ -235,8 +251,9 @@ clm_expm_plan pade_plan(double norm1) {
       return p;
     }
   }
-  int s = static_cast<int>(std::ceil(std::log2(norm1 / pade_theta(13))));
-  if (s < 0) s = 0;
+  // a ratio that overflows (or a NaN) takes the same bounded answer as the Taylor
+  // ladder's, not a cast of an infinite log2
+  const int s = norm_squarings(norm1, pade_theta(13));
   p.m = 13;
   p.s = s;
   p.num_gemms = pade_num_gemms(13) + s;
@@ -259,8 +276,12 @@ clm_expm_plan pade_plan_single(float norm1) {
     }
   }
   const float theta = static_cast<float>(pade_theta_single(13));
-  int s = static_cast<int>(std::ceil(std::log2(norm1 / theta)));
-  if (s < 0) s = 0;
+  const float ratio = norm1 / theta;
+  int s = 0;
+  if (!(ratio <= std::numeric_limits<float>::max()))
+    s = std::isnan(ratio) ? 0 : kUnusableSquarings;
+  else
+    s = std::max(0, static_cast<int>(std::ceil(std::log2(ratio))));
   p.m = 13;
   p.s = s;
   p.num_gemms = pade_num_gemms(13) + s;
@@ -269,14 +290,17 @@ clm_expm_plan pade_plan_single(float norm1) {
 
 // ---- expm workspace --------------------------------------------------------------------------
 //   [colsum | n doubles ]  per-column |.| sums; colsum[0] ends as the 1-norm
+//   [coef   | 32 T      ]  Taylor coefficients with 2^-s folded in (see expm_impl)
 //   [As     | n*n T     ]  2^-s A
 //   [bank   | B x stride]  A^2 .. A^(B+1) of the scaled matrix, B = max_bank_blocks()
 //   [S      | n*n T     ]  Paterson-Stockmeyer scratch, then the squaring ping-pong
 // Six n*n blocks at the top of the ladder, against the reference's seven
 // (src/expm/README.md:206-211); nothing for an LU.
+constexpr int kCoefSlots = 32;  // Taylor coefficients 0 .. 29 (the ladder's top degree)
 template <typename T>
 struct ExpmWs {
   real_t<T> *colsum = nullptr;
+  T *coef = nullptr;  // the coefficients with 2^-s folded in
   T *As = nullptr;
   T *bank = nullptr;
   T *S = nullptr;
@@ -286,6 +310,7 @@ struct ExpmWs {
     const size_t nn = static_cast<size_t>(n) * n;
     stride = power_stride<T>(n);
     colsum = l.take<real_t<T>>(static_cast<size_t>(n));
+    coef = l.take<T>(kCoefSlots);
     As = l.take<T>(nn);
     bank = l.take<T>(static_cast<size_t>(max_bank_blocks()) * stride);
     S = l.take<T>(nn);
@@ -332,11 +357,13 @@ clm_status_t bank_powers(hipStream_t st, int n, const T *A, long lda, T *bank, l
 // Paterson-Stockmeyer on the device, the reference's structure
 // (src/paterson_stockmeyer/interface.cppm:364-419) with each B_j built inside
 // the epilogue of the Horner product that consumes it. The first `have` powers
-// (A^2 .. A^(have+1)) are already in the bank.
+// (A^2 .. A^(have+1)) are already in the bank. x_scale multiplies every Horner
+// product P X (2^-(b s) when A and its bank are unscaled and the 2^-s is folded
+// into the coefficients; exact, as every operand scaling here is a power of two).
 template <typename T>
 clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const T *A, long lda,
                          T *P, long ldp, T *bank, long stride, T *scratch, long ld_scratch,
-                         int s_requested, int have = 0) {
+                         int s_requested, int have = 0, double x_scale = 1.0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
   CLM_TRY(bank_powers(st, n, A, lda, bank, stride, have + 2, p.num_powers + 1));
   const T *X = (p.r < 1) ? nullptr : (p.s >= 2) ? bank + static_cast<long>(p.s - 2) * stride : A;
@@ -360,7 +387,7 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
     g.ldb = ldx;
     g.C = buf[next];
     g.ldc = ld[next];
-    g.alpha = one<T>();
+    g.alpha = ops<T>::from_real(x_scale);
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
     g.X1 = A;
@@ -478,7 +505,7 @@ clm_status_t bank_powers_split(hipStream_t st, int split, int n, const double *A
 clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int degree,
                                const double *A, long lda, double *P, long ldp, double *bank,
                                long stride, double *scratch, long ld_scratch, int s_requested,
-                               int split, const SplitWs &sp, int have = 0) {
+                               int split, const SplitWs &sp, int have = 0, double x_scale = 1.0) {
   const clm_ps_plan p = ps_plan(degree, s_requested);
   bool pca_is_a = false;
   CLM_TRY(bank_powers_split(st, split, n, A, lda, bank, stride, have + 2, p.num_powers + 1, sp,
@@ -504,6 +531,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
     const int next = 1 - cur;
     CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp));
     GemmArgs<double> g = plain_args(n, buf[next], ld[next]);
+    g.alpha = x_scale;
     g.coef = coef + static_cast<long>(j) * p.s;
     g.nterms = terms(j);
     g.X1 = A;
@@ -648,14 +676,88 @@ clm_status_t ps_evaluate_fp8tail(hipStream_t st, int n, const double *coef, int 
   return CLM_STATUS_SUCCESS;
 }
 
+// ---- reading norms back ----------------------------------------------------------------
+// The plan waits on the device twice (the 1-norm, then the norms of the powers).
+// A host-mapped, coherent slot that a one-lane kernel fills and then flags with a
+// sequence number lets the host see the values ~6 us after they exist, against
+// ~23 us for a pageable copy and a stream synchronization (measured on MI355X).
+// Each device has a ring of slots taken in turn; the host spins on the flag for up
+// to kSpinLimit, then falls back to a blocking synchronization, so long waits
+// (the powers of a large matrix) do not hold a core.
+struct ReadbackSlot {
+  double vals[kMaxNormPower];
+  unsigned seq;
+  unsigned pad[21];  // 128 bytes a slot
+};
+constexpr int kRingSlots = 64;
+struct ReadbackRing {
+  ReadbackSlot *host = nullptr;
+  ReadbackSlot *dev = nullptr;
+  std::atomic<unsigned> next{0};
+};
+ReadbackRing g_ring[64];
+std::mutex g_ring_mutex;
+constexpr auto kSpinLimit = std::chrono::microseconds(2000);
+
+ReadbackRing *ring_of(int device) {
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(g_ring_mutex);
+  ReadbackRing &r = g_ring[device];
+  if (!r.host) {
+    void *h = nullptr, *d = nullptr;
+    if (hipHostMalloc(&h, sizeof(ReadbackSlot) * kRingSlots,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+      return nullptr;
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+      (void)hipHostFree(h);
+      return nullptr;
+    }
+    std::memset(h, 0, sizeof(ReadbackSlot) * kRingSlots);
+    r.dev = static_cast<ReadbackSlot *>(d);
+    r.host = static_cast<ReadbackSlot *>(h);
+  }
+  return &r;
+}
+
+// out[0 .. count-1] = the device values src[0 .. count-1] once the stream reaches
+// them. pinned = false takes a pageable copy and a stream synchronization and
+// allocates nothing.
+template <typename R>
+clm_status_t read_back(hipStream_t st, int device, const R *src, int count, double *out,
+                       bool pinned) {
+  ReadbackRing *ring = pinned ? ring_of(device) : nullptr;
+  if (!ring) {
+    R host[kMaxNormPower] = {};
+    CLM_TRY_HIP(hipMemcpyAsync(host, src, sizeof(R) * count, hipMemcpyDeviceToHost, st));
+    CLM_TRY_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < count; ++i) out[i] = static_cast<double>(host[i]);
+    return CLM_STATUS_SUCCESS;
+  }
+  const unsigned ticket = ring->next.fetch_add(1, std::memory_order_relaxed);
+  const unsigned seq = ticket + 1 == 0 ? 1 : ticket + 1;
+  const int slot = static_cast<int>(ticket % kRingSlots);
+  ReadbackSlot *h = ring->host + slot;
+  CLM_TRY_HIP(launch_publish(st, src, count, ring->dev[slot].vals, &ring->dev[slot].seq, seq));
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(&h->seq, __ATOMIC_ACQUIRE) != seq) {
+    if (std::chrono::steady_clock::now() - t0 > kSpinLimit) {
+      CLM_TRY_HIP(hipStreamSynchronize(st));
+      if (__atomic_load_n(&h->seq, __ATOMIC_ACQUIRE) != seq) return CLM_STATUS_EXECUTION_FAILED;
+      break;
+    }
+    __builtin_ia32_pause();
+  }
+  for (int i = 0; i < count; ++i) out[i] = h->vals[i];
+  return CLM_STATUS_SUCCESS;
+}
+
 template <typename T>
 clm_status_t read_norm1(hipStream_t st, int n, const T *A, long lda, real_t<T> *colsum,
-                        real_t<T> *norm) {
+                        real_t<T> *norm, int device = -1, bool pinned = false) {
   CLM_TRY_HIP(launch_norm1(st, n, A, lda, colsum));
-  real_t<T> host = 0;
-  CLM_TRY_HIP(hipMemcpyAsync(&host, colsum, sizeof(host), hipMemcpyDeviceToHost, st));
-  CLM_TRY_HIP(hipStreamSynchronize(st));
-  *norm = host;
+  double v = 0;
+  CLM_TRY(read_back(st, device, colsum, 1, &v, pinned));
+  *norm = static_cast<real_t<T>>(v);
   return CLM_STATUS_SUCCESS;
 }
 
@@ -666,7 +768,7 @@ template <typename T>
 clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda, T *dE, int lde,
                        void *work, size_t bytes, int *dInfo, clm_expm_plan *plan,
                        const clm_split_opts *split = nullptr, int variant = kPlain,
-                       int fp8_min_degree = kFp8MinDegree) {
+                       int fp8_min_degree = kFp8MinDegree, bool may_allocate = true) {
   if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
       dE == nullptr)
     return CLM_STATUS_INVALID_VALUE;
@@ -690,7 +792,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   }
 
   real_t<T> norm = 0;
-  CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm));
+  CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm, c->device, may_allocate));
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
   if (use_split) CLM_TRY_HIP(sp.start(st, n));
 
@@ -707,35 +809,51 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
       use_split ? split_plan(norm, n, split) : taylor_plan(norm, is_single<T>());
   const bool norm_only = variant == kFp8Tail || (split && split->plan_from == CLM_PLAN_NORM1) ||
                          (use_split && !(double(norm) <= kSplitPowerMaxNorm1));
-  const int q = norm_only ? 1 : norm_power_count(double(norm), p0.m, single_range);
+  int q = norm_only ? 1 : norm_power_count(double(norm), p0.m, single_range);
   double pn[kMaxNormPower] = {double(norm)};
   int fp8_count = 0;
   if (q > 1) {
+    // A^2 and A^3, their norms; then A^4, A^5 one at a time while worth_next_power
+    // says the power can lower the plan's cost (one readback each, no product spent
+    // on a power the plan would not use)
     const long stride = static_cast<long>(ws.stride);
-    if constexpr (std::is_same_v<T, double>) {
-      if (variant == kFp8Tail) {
-        bool ca_split = false, ca_fp8 = false;
-        CLM_TRY(bank_powers_fp8(st, p0.ps_split, fp8_min_degree, n, dA, lda, ws.bank, stride, 2,
-                                q, sp, f8, &ca_split, &ca_fp8, &fp8_count));
-      } else if (use_split) {
-        bool pca_is_a = false;
-        CLM_TRY(bank_powers_split(st, p0.ps_split, n, dA, lda, ws.bank, stride, 2, q, sp,
-                                  &pca_is_a));
-      } else {
-        CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, 2, q));
-      }
-    } else {
-      CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, 2, q));
-    }
-    // the power norms land in the first bytes of S (unused until the polynomial)
+    bool pca_is_a = false, ca_split = false, ca_fp8 = false;
+    const double w_new = p0.ps_split == CLM_SPLIT_BF16X6 ? 0.35 : 0.2;
+    // the power norms land in the first bytes of S and their column sums in As
+    // (both unused until the polynomial; As holds q - 1 <= 4 columns of sums for n >= 4)
     real_t<T> *dnorms = reinterpret_cast<real_t<T> *>(ws.S);
-    for (int k = 2; k <= q; ++k)
-      CLM_TRY_HIP(launch_norm1(st, n, ws.bank + static_cast<long>(k - 2) * stride, n, ws.colsum,
-                               dnorms + (k - 2)));
-    real_t<T> host[kMaxNormPower] = {};
-    CLM_TRY_HIP(hipMemcpyAsync(host, dnorms, sizeof(real_t<T>) * (q - 1), hipMemcpyDeviceToHost, st));
-    CLM_TRY_HIP(hipStreamSynchronize(st));
-    for (int k = 2; k <= q; ++k) pn[k - 1] = double(host[k - 2]);
+    int formed = 1;  // A^1 .. A^formed exist
+    for (;;) {
+      if constexpr (std::is_same_v<T, double>) {
+        if (variant == kFp8Tail) {
+          CLM_TRY(bank_powers_fp8(st, p0.ps_split, fp8_min_degree, n, dA, lda, ws.bank, stride,
+                                  formed + 1, q, sp, f8, &ca_split, &ca_fp8, &fp8_count));
+        } else if (use_split) {
+          CLM_TRY(bank_powers_split(st, p0.ps_split, n, dA, lda, ws.bank, stride, formed + 1, q,
+                                    sp, &pca_is_a));
+        } else {
+          CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, formed + 1, q));
+        }
+      } else {
+        CLM_TRY(bank_powers<T>(st, n, dA, lda, ws.bank, stride, formed + 1, q));
+      }
+      const int fresh = q - formed;  // powers A^(formed+1) .. A^q
+      if (n >= 4) {
+        CLM_TRY_HIP(launch_norm1_powers(st, n, ws.bank + static_cast<long>(formed - 1) * stride,
+                                        stride, fresh, reinterpret_cast<real_t<T> *>(ws.As),
+                                        dnorms + (formed - 1)));
+      } else {
+        for (int k = formed + 1; k <= q; ++k)
+          CLM_TRY_HIP(launch_norm1(st, n, ws.bank + static_cast<long>(k - 2) * stride, n,
+                                   ws.colsum, dnorms + (k - 2)));
+      }
+      CLM_TRY(read_back(st, c->device, dnorms + (formed - 1), fresh, pn + formed, may_allocate));
+      formed = q;
+      const bool more = use_split ? split_worth_next_power(pn, q, n, split, w_new)
+                                  : worth_next_power(pn, q, is_single<T>());
+      if (!more) break;
+      ++q;
+    }
   }
   clm_expm_plan p = use_split ? split_plan_powers(pn, q, n, split)
                               : taylor_plan_powers(pn, q, is_single<T>());
@@ -743,11 +861,27 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   if (variant == kFp8Tail) p.fp8_min_degree = fp8_min_degree;
   if (plan) *plan = p;
 
+  // T_m(A / 2^s). With the bank formed from A itself, A and the bank stay unscaled
+  // and the 2^-s goes into the arithmetic instead: block j's coefficient of A^t is
+  // c_(jb+t) 2^(-ts) and every Horner product P X (X = A^b) is scaled by 2^(-bs).
+  // Every scaling is by a power of two, so each product and epilogue sum is bit for
+  // bit the one formed from A / 2^s while no intermediate leaves the normal range:
+  // the unscaled P A^b is 2^(bs) times the scaled one, so the fold is taken for
+  // b s <= 128 in FP64 and <= 32 in the split variant (its planes' FP32 range);
+  // not for FP32 / complex64, where a product of the scaled operands can reach the
+  // denormal range that the unscaled one avoids (measured: one complex64 case of
+  // 120 differed in its last bits). Otherwise the bank is scaled in place.
   const T *src = dA;
   long ld_src = lda;
-  if (p.s > 0) {
+  const T *pcoef = coef;
+  double x_scale = 1.0;
+  const int fold_max = is_single<T>() ? 0 : use_split ? 32 : 128;
+  if (p.s > 0 && q > 1 && p.ps_block * p.s <= fold_max) {
+    CLM_TRY_HIP(launch_scale_coef(st, coef, ws.coef, p.m + 1, p.ps_block, p.s));
+    pcoef = ws.coef;
+    x_scale = std::ldexp(1.0, -p.ps_block * p.s);
+  } else if (p.s > 0) {
     CLM_TRY_HIP(launch_scale_copy(st, n, std::ldexp(1.0, -p.s), dA, lda, ws.As, n));
-    // the bank formed from A becomes the bank of A / 2^s: A^k / 2^(k s), exact
     CLM_TRY_HIP(launch_scale_powers(st, n, ws.bank, static_cast<long>(ws.stride), q - 1, p.s));
     src = ws.As;
     ld_src = n;
@@ -768,16 +902,18 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
                                   p.ps_split, fp8_min_degree, sp, f8, &count, q - 1));
       p.fp8_gemms = fp8_count + count;
     } else if (use_split) {
-      CLM_TRY(ps_evaluate_split(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
+      CLM_TRY(ps_evaluate_split(st, n, pcoef, p.m, src, ld_src, target, ld_target, ws.bank,
                                 static_cast<long>(ws.stride), other, ld_other, p.ps_block,
-                                p.ps_split, sp, q - 1));
+                                p.ps_split, sp, q - 1, x_scale));
     } else {
-      CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                             static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1));
+      CLM_TRY(ps_evaluate<T>(st, n, pcoef, p.m, src, ld_src, target, ld_target, ws.bank,
+                             static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1,
+                             x_scale));
     }
   } else {
-    CLM_TRY(ps_evaluate<T>(st, n, coef, p.m, src, ld_src, target, ld_target, ws.bank,
-                           static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1));
+    CLM_TRY(ps_evaluate<T>(st, n, pcoef, p.m, src, ld_src, target, ld_target, ws.bank,
+                           static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1,
+                           x_scale));
   }
   if (plan) plan->fp8_gemms = p.fp8_gemms;
 
@@ -1032,8 +1168,8 @@ clm_status_t lascl_impl(clm_stream_t stream, real_t<T> cfrom, real_t<T> cto, siz
 // device address of each GPU's copy comes from hipGetSymbolAddress, once per
 // device with that device current, and only that address is ever used. Nothing
 // in the workspace must start at zero: every region is written before it is read.
-constexpr size_t kNoAllocBytes = 4096 + 6 * (size_t(CLM_EXPM_NOALLOC_MAX_N) *
-                                              CLM_EXPM_NOALLOC_MAX_N * sizeof(double));
+constexpr size_t kNoAllocBytes = 4096 + 256 + 6 * (size_t(CLM_EXPM_NOALLOC_MAX_N) *
+                                                    CLM_EXPM_NOALLOC_MAX_N * sizeof(double));
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 std::mutex g_noalloc_mutex;
@@ -1054,6 +1190,16 @@ clm_status_t noalloc_workspace(int device, void **out) {
 }
 
 }  // namespace
+
+template <typename R>
+clm_status_t read_device_values(hipStream_t st, int device, const R *src, int count, double *out) {
+  if (src == nullptr || out == nullptr || count < 1 || count > kMaxNormPower)
+    return CLM_STATUS_INVALID_VALUE;
+  return read_back(st, device, src, count, out, true);
+}
+template clm_status_t read_device_values<double>(hipStream_t, int, const double *, int, double *);
+template clm_status_t read_device_values<float>(hipStream_t, int, const float *, int, double *);
+
 }  // namespace clm
 
 using clm::zc;
@@ -1216,7 +1362,8 @@ clm_status_t clm_expm_noalloc_d(clm_ctx *ctx, clm_stream_t stream, int n, const 
   void *work = nullptr;
   CLM_TRY(clm::noalloc_workspace(dev, &work));
   return clm::expm_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dE, lde, work,
-                                clm::kNoAllocBytes, dInfo, plan);
+                                clm::kNoAllocBytes, dInfo, plan, nullptr, clm::kPlain,
+                                clm::kFp8MinDegree, /*may_allocate=*/false);
 }
 
 clm_status_t clm_expm_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,

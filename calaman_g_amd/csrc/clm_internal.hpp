@@ -213,9 +213,23 @@ hipError_t launch_gemm(hipStream_t stream, const GemmArgs<T> &g);
 template <typename T>
 hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<T> *colsum,
                         real_t<T> *out = nullptr);
+// the 1-norms of count n x n blocks (leading dimension n) `stride` elements apart:
+// column sums into colsums (count * n values), maxima into out[0 .. count-1];
+// each bit for bit launch_norm1's
+template <typename T>
+hipError_t launch_norm1_powers(hipStream_t stream, int n, const T *bank, long stride, int count,
+                               real_t<T> *colsums, real_t<T> *out);
 // bank[k-2] *= 2^(-k s) for k = 2 .. count + 1 (n x n blocks, leading dimension n)
 template <typename T>
 hipError_t launch_scale_powers(hipStream_t stream, int n, T *bank, long stride, int count, int s);
+// vals[0 .. count-1] = src (device -> host-mapped memory), then *flag = seq (system
+// scope, after the values)
+template <typename R>
+hipError_t launch_publish(hipStream_t stream, const R *src, int count, double *vals,
+                          unsigned *flag, unsigned seq);
+// out[k] = 2^(-(k mod b) s) c[k] for k < count (device arrays)
+template <typename T>
+hipError_t launch_scale_coef(hipStream_t stream, const T *c, T *out, int count, int b, int s);
 // lange: which in CLM_NORM_*; result in *d_result.
 template <typename T>
 hipError_t launch_lange(hipStream_t stream, int which, size_t m, size_t n, const T *A, size_t lda,
@@ -240,6 +254,11 @@ template <typename T>
 hipError_t launch_copy(hipStream_t stream, int n, const T *src, long lds, T *dst, long ldd);
 // d[0..count) = v
 hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v);
+
+// src[0 .. count-1] (count <= 5) once the stream reaches them: a flag polled in
+// host-mapped memory, then a blocking synchronization after 2 ms
+template <typename R>
+clm_status_t read_device_values(hipStream_t st, int device, const R *src, int count, double *out);
 
 // Device tables of 1/k!, k = 0..32 (correctly rounded), for the Taylor coefficients.
 const double *inv_factorials_d(int device);

@@ -43,14 +43,17 @@ __device__ __forceinline__ R nan_max(R a, R b) {
 }
 
 // pre: 0 |x|, 1 |x|^2; op: 0 plus, 1 nan_max. One block per column.
+// A grid layer y > 0 reduces the matrix at A + y * mstride into out + y * gridDim.x.
 template <typename T, int PRE, int OP>
 __global__ void __launch_bounds__(kRB)
     reduce_columns_kernel(const T *__restrict__ A, real_t<T> *__restrict__ out, size_t m,
-                          size_t lda) {
+                          size_t lda, long mstride = 0) {
   using R = real_t<T>;
   __shared__ R sdata[kRB];
   const size_t col = blockIdx.x;
   const unsigned tid = threadIdx.x;
+  A += static_cast<long>(blockIdx.y) * mstride;
+  out += static_cast<size_t>(blockIdx.y) * gridDim.x;
   const T *column = A + col * lda;
   const unsigned nactive = (m < kRB) ? static_cast<unsigned>(m) : kRB;
   if (tid < nactive) {
@@ -71,11 +74,14 @@ __global__ void __launch_bounds__(kRB)
 }
 
 // max_reduce (src/expm/expm.cu:212-245): best seeded at 0, a NaN wins and stays;
-// the result goes to out[0] (values[0] when out is values).
+// the result goes to out[0] (values[0] when out is values). Block b reduces
+// values + b * n into out[b].
 template <typename R>
 __global__ void __launch_bounds__(kRB) max_reduce_kernel(int n, R *values, R *out) {
   __shared__ R sdata[kRB];
   const int tid = threadIdx.x;
+  values += static_cast<long>(blockIdx.x) * n;
+  out += blockIdx.x;
   R best = 0;
   for (int i = tid; i < n; i += kRB) {
     const R v = values[i];
@@ -153,6 +159,15 @@ __global__ void __launch_bounds__(kEW)
   *x = ops<T>::ldexp(*x, -k * s);
 }
 
+// out[k] = 2^(-(k mod b) s) c[k], k < count: the Paterson-Stockmeyer coefficients
+// with the 2^-s scaling of A folded in (block j multiplies c[jb + t] by A^t).
+template <typename T>
+__global__ void scale_coef_kernel(const T *__restrict__ c, T *__restrict__ out, int count, int b,
+                                  int s) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) out[k] = ops<T>::ldexp(c[k], -(k % b) * s);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kEW) lascl_kernel(size_t m, double mul, T *A, size_t lda) {
   const size_t i = static_cast<size_t>(blockIdx.x) * kEW + threadIdx.x;
@@ -192,6 +207,17 @@ __global__ void __launch_bounds__(kEW)
   if (i < n) dst[i + j * ldd] = src[i + j * lds];
 }
 
+// Copies count values into a host-mapped readback slot and then publishes seq in
+// the slot's flag, after a system-scope fence: a host that sees seq sees the values.
+template <typename R>
+__global__ void publish_kernel(const R *__restrict__ src, int count, double *vals, unsigned *flag,
+                               unsigned seq) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < count; ++i) vals[i] = static_cast<double>(src[i]);
+  __threadfence_system();
+  __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void fill_int_kernel(int *d, int count, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) d[i] = v;
@@ -215,12 +241,31 @@ hipError_t launch_norm1(hipStream_t stream, int n, const T *A, long lda, real_t<
 }
 
 template <typename T>
+hipError_t launch_norm1_powers(hipStream_t stream, int n, const T *bank, long stride, int count,
+                               real_t<T> *colsums, real_t<T> *out) {
+  if (n < 1 || count < 1) return hipSuccess;
+  hipLaunchKernelGGL((reduce_columns_kernel<T, 0, 0>), dim3(n, count), dim3(kRB), 0, stream, bank,
+                     colsums, static_cast<size_t>(n), static_cast<size_t>(n), stride);
+  hipLaunchKernelGGL(max_reduce_kernel<real_t<T>>, dim3(count), dim3(kRB), 0, stream, n, colsums,
+                     out);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_scale_powers(hipStream_t stream, int n, T *bank, long stride, int count, int s) {
   if (n < 1 || count < 1 || s == 0) return hipSuccess;
   const dim3 grid(static_cast<unsigned>((n + kEW - 1) / kEW), static_cast<unsigned>(n),
                   static_cast<unsigned>(count));
   hipLaunchKernelGGL(scale_powers_kernel<T>, grid, dim3(kEW), 0, stream, n, bank, stride,
                      static_cast<long>(n), s);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_coef(hipStream_t stream, const T *c, T *out, int count, int b, int s) {
+  if (count < 1 || b < 1) return hipSuccess;
+  hipLaunchKernelGGL(scale_coef_kernel<T>, dim3((count + 63) / 64), dim3(64), 0, stream, c, out,
+                     count, b, s);
   return hipGetLastError();
 }
 
@@ -304,6 +349,17 @@ hipError_t launch_copy(hipStream_t stream, int n, const T *src, long lds, T *dst
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_publish(hipStream_t stream, const R *src, int count, double *vals,
+                          unsigned *flag, unsigned seq) {
+  hipLaunchKernelGGL(publish_kernel<R>, dim3(1), dim3(64), 0, stream, src, count, vals, flag, seq);
+  return hipGetLastError();
+}
+template hipError_t launch_publish<double>(hipStream_t, const double *, int, double *, unsigned *,
+                                           unsigned);
+template hipError_t launch_publish<float>(hipStream_t, const float *, int, double *, unsigned *,
+                                          unsigned);
+
 hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v) {
   if (count < 1) return hipSuccess;
   hipLaunchKernelGGL(fill_int_kernel, dim3((count + 63) / 64), dim3(64), 0, stream, d, count, v);
@@ -312,7 +368,10 @@ hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v) {
 
 #define CLM_INSTANTIATE(T)                                                                         \
   template hipError_t launch_norm1<T>(hipStream_t, int, const T *, long, real_t<T> *, real_t<T> *); \
+  template hipError_t launch_scale_coef<T>(hipStream_t, const T *, T *, int, int, int);            \
   template hipError_t launch_scale_powers<T>(hipStream_t, int, T *, long, int, int);               \
+  template hipError_t launch_norm1_powers<T>(hipStream_t, int, const T *, long, int, real_t<T> *,  \
+                                             real_t<T> *);                                         \
   template hipError_t launch_lange<T>(hipStream_t, int, size_t, size_t, const T *, size_t,          \
                                       real_t<T> *, real_t<T> *);                                   \
   template hipError_t launch_scale_copy<T>(hipStream_t, int, double, const T *, long, T *, long);  \

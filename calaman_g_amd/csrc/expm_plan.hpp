@@ -189,12 +189,24 @@ __host__ __device__ inline clm_expm_plan taylor_plan(double norm1, bool single) 
 
 // Whether the norms of powers are taken: powers of the unscaled matrix must stay
 // finite (||A^5|| <= ||A||^5; FP32 also bounds the 16-bit split planes), and at
-// least A^3 must be in the norm-only plan's bank for an alpha_p to exist.
+// least A^3 must be in the norm-only plan's bank for an alpha_p to exist. Returns
+// the powers formed first: A^2 and A^3 (3), or none (1).
 __host__ __device__ inline int norm_power_count(double norm1, int degree, bool single_range) {
   if (!(norm1 <= (single_range ? 0x1p24 : 0x1p200))) return 1;
-  const int q = ps_plan(degree, 0).num_powers + 1;
-  if (q < 3) return 1;
-  return q < kMaxNormPower ? q : kMaxNormPower;
+  return ps_plan(degree, 0).num_powers + 1 >= 3 ? 3 : 1;
+}
+
+// Whether forming A^(q+1) can lower the cost of the plan from the norms of
+// A .. A^q: the best plan with ||A^(q+1)|| taken as 0 (the most that power's norm
+// can help) against the best plan without it, the new power counted as a product.
+// A power the final plan's bank would need anyway costs nothing extra, so a bank
+// that keeps growing keeps being formed; one that would be wasted is not.
+__host__ __device__ inline bool worth_next_power(const double *pn, int q, bool single) {
+  if (q < 3 || q >= kMaxNormPower) return false;
+  double trial[kMaxNormPower] = {};
+  for (int k = 0; k < q; ++k) trial[k] = pn[k];
+  return taylor_plan_powers(trial, q + 1, single).num_gemms <
+         taylor_plan_powers(pn, q, single).num_gemms;
 }
 
 }  // namespace clm

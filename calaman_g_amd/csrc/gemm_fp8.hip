@@ -19,8 +19,16 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 128;        // quantization tile edge = k-tile depth
-constexpr int kRowB = kT + 16;  // LDS row (bytes): 16-byte pad
+constexpr int kT = 128;   // quantization tile edge = k-tile depth
+constexpr int kRowB = kT;  // LDS row (bytes): one k-tile of a row, unpadded
+// byte offset of (row, k) in an LDS plane, k a multiple of 16: the row's eight
+// 16-byte chunks permuted by an XOR with the row index mod 8, which makes the
+// fragment reads (16 lanes on 16 rows, one chunk each) and the staging writes
+// (8 lanes on one row's 8 chunks) free of bank conflicts (measured with
+// SQ_LDS_BANK_CONFLICT on MI355X; 144-byte padded rows conflicted)
+__device__ __forceinline__ int fp8_off(int row, int k) {
+  return row * kRowB + (((k >> 4) ^ (row & 7)) << 4);
+}
 
 __device__ __forceinline__ unsigned fp8_pair(float a, float b) {
   return static_cast<unsigned>(__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false)) & 0xffffu;
@@ -172,18 +180,19 @@ __device__ __forceinline__ void fp8w_sstore(unsigned char *S, int tid, const u4v
 #pragma unroll
   for (int it = 0; it < 4; ++it) {
     const int c = tid + it * 512;
-    *reinterpret_cast<u4v *>(&S[(c >> 3) * kRowB + (c & 7) * 16]) = ra[it];
+    *reinterpret_cast<u4v *>(&S[fp8_off(c >> 3, (c & 7) * 16)]) = ra[it];
   }
 #pragma unroll
   for (int it = 0; it < 2; ++it) {
     const int c = tid + it * 512;
-    *reinterpret_cast<u4v *>(&S[kStageA + (c >> 3) * kRowB + (c & 7) * 16]) = rb[it];
+    *reinterpret_cast<u4v *>(&S[kStageA + fp8_off(c >> 3, (c & 7) * 16)]) = rb[it];
   }
 }
 
-__device__ __forceinline__ i8v lds_i8v(const unsigned char *p) {
-  const u4v lo = *reinterpret_cast<const u4v *>(p);
-  const u4v hi = *reinterpret_cast<const u4v *>(p + 16);
+// a lane's 32 fragment bytes of one row: k .. k + 31 (two chunks, swizzled apart)
+__device__ __forceinline__ i8v lds_i8v(const unsigned char *S, int row, int k) {
+  const u4v lo = *reinterpret_cast<const u4v *>(S + fp8_off(row, k));
+  const u4v hi = *reinterpret_cast<const u4v *>(S + fp8_off(row, k + 16));
   i8v v;
   v[0] = lo[0]; v[1] = lo[1]; v[2] = lo[2]; v[3] = lo[3];
   v[4] = hi[0]; v[5] = hi[1]; v[6] = hi[2]; v[7] = hi[3];
@@ -196,10 +205,9 @@ __device__ __forceinline__ void fp8w_compute(const unsigned char *S, int lane, i
   const int koff = 32 * (lane >> 4);
   i8v a[4], b[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) a[i] = lds_i8v(&S[(wm0 + i * 16 + (lane & 15)) * kRowB + koff]);
+  for (int i = 0; i < 4; ++i) a[i] = lds_i8v(S, wm0 + i * 16 + (lane & 15), koff);
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    b[j] = lds_i8v(&S[kStageA + (wn0 + j * 16 + (lane & 15)) * kRowB + koff]);
+  for (int j = 0; j < 4; ++j) b[j] = lds_i8v(S + kStageA, wn0 + j * 16 + (lane & 15), koff);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll

@@ -664,12 +664,25 @@ struct PlaneMfma<1> {
 // spilled to scratch and ran 2.7x slower). The k-steps run in ascending order and the products of a
 // k-step in a fixed order (smallest terms first).
 // ------------------------------------------------------------------------------------
+// LDS layout of a plane tile (128 rows of BK k-values, 16-bit). 64-deep slices
+// (KS = 2) use unpadded 128-byte rows with the row's eight 16-byte chunks permuted
+// by an XOR with the row index mod 8: the b128 fragment reads (16 lanes on 16
+// rows, one chunk each) and the staging writes (8 lanes on one row's 8 chunks) are
+// both free of bank conflicts, measured with SQ_LDS_BANK_CONFLICT on MI355X -- the
+// 144-byte padded rows took 1.1 extra cycles per fragment read (5.1 LDS cycles per
+// b128 read against 4.0, the 256 B/clk rate). 32-deep slices keep a 16-byte pad.
 template <int NPL, int KS>
 struct PlaneTile {
   static constexpr int BK = 32 * KS;
-  static constexpr int ROW = BK + 8;  // 16-byte pad per row: conflict-free b128 reads
+  static constexpr bool kSwizzle = (KS == 2);
+  static constexpr int ROW = kSwizzle ? BK : BK + 8;
   static constexpr int PLANE = 128 * ROW;
   static constexpr int STAGE = 2 * NPL * PLANE;  // A and B planes
+  // element offset of (row, k) in a plane, k a multiple of 8 (a 16-byte chunk)
+  __device__ static constexpr int off(int row, int k) {
+    if constexpr (kSwizzle) return row * ROW + (((k >> 3) ^ (row & 7)) << 3);
+    else return row * ROW + k;
+  }
 };
 
 template <int NPL, int KS, int NT, bool kFast>
@@ -720,8 +733,8 @@ __device__ __forceinline__ void planes2_sstore(unsigned short *S, int tid,
     for (int it = 0; it < IT; ++it) {
       const int c = tid + it * NT;
       const int r = c / CHUNKS, kq = (c % CHUNKS) * 8;
-      *reinterpret_cast<u32x4 *>(&S[t * PT::PLANE + r * PT::ROW + kq]) = ra[t][it];
-      *reinterpret_cast<u32x4 *>(&S[(NPL + t) * PT::PLANE + r * PT::ROW + kq]) = rb[t][it];
+      *reinterpret_cast<u32x4 *>(&S[t * PT::PLANE + PT::off(r, kq)]) = ra[t][it];
+      *reinterpret_cast<u32x4 *>(&S[(NPL + t) * PT::PLANE + PT::off(r, kq)]) = rb[t][it];
     }
 }
 
@@ -740,11 +753,11 @@ __device__ __forceinline__ void planes2_compute(const unsigned short *S, int lan
 #pragma unroll
       for (int i = 0; i < TM; ++i)
         a[t][i] = *reinterpret_cast<const V *>(
-            &S[t * PT::PLANE + (wm0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
+            &S[t * PT::PLANE + PT::off(wm0 + i * 16 + (lane & 15), koff)]);
 #pragma unroll
       for (int j = 0; j < TN; ++j)
         b[t][j] = *reinterpret_cast<const V *>(
-            &S[(NPL + t) * PT::PLANE + (wn0 + j * 16 + (lane & 15)) * PT::ROW + koff]);
+            &S[(NPL + t) * PT::PLANE + PT::off(wn0 + j * 16 + (lane & 15), koff)]);
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -876,9 +889,9 @@ __device__ __forceinline__ void planes_kg_compute(const unsigned short *S, int l
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       a[t][i] = *reinterpret_cast<const V *>(
-          &S[t * PT::PLANE + (wm0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
+          &S[t * PT::PLANE + PT::off(wm0 + i * 16 + (lane & 15), koff)]);
       b[t][i] = *reinterpret_cast<const V *>(
-          &S[(NPL + t) * PT::PLANE + (wn0 + i * 16 + (lane & 15)) * PT::ROW + koff]);
+          &S[(NPL + t) * PT::PLANE + PT::off(wn0 + i * 16 + (lane & 15), koff)]);
     }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -903,9 +916,11 @@ __global__ void __launch_bounds__(512)
   constexpr int IT = 128 * 4 * KS / NT;
   static_assert(IT >= 1 && (128 * 4 * KS) % NT == 0, "staging shape");
   // the reduction buffer: 4 waves x 16 tiles x 4 rows x 64 lanes of FP32
-  static_assert(4 * 16 * 4 * 64 * 4 <= 2 * PT::STAGE * 2, "reduction fits in the stages");
+  // the stages, or the reduction buffer and the column-major sum tile if larger
+  constexpr int kRedFloats = 4 * 16 * 4 * 64 + 128 * kSumLd;
+  constexpr int kLdsShorts = (2 * PT::STAGE > 2 * kRedFloats) ? 2 * PT::STAGE : 2 * kRedFloats;
 
-  __shared__ __attribute__((aligned(16))) unsigned short L[2 * PT::STAGE];
+  __shared__ __attribute__((aligned(16))) unsigned short L[kLdsShorts];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -945,8 +960,7 @@ __global__ void __launch_bounds__(512)
   // group 1's sums, then (group 0) the whole tile's, column-major for the epilogue
   float *red = reinterpret_cast<float *>(L);
   float *sum = red + 4 * 16 * 4 * 64;
-  static_assert((4 * 16 * 4 * 64 + 128 * kSumLd) * 4 <= 2 * PT::STAGE * 2,
-                "the reduction buffer and the sum tile fit in the stages");
+  static_assert(kRedFloats * 4 <= kLdsShorts * 2, "the reduction buffer and the sum tile fit");
   if (grp == 1) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)

@@ -136,7 +136,13 @@ void orc_expm_plan_d(double norm1, int *out) {
       return;
     }
   }
-  int s = (int)ceil(log2(norm1 / orc_pade_theta_d(13)));
+  /* a ratio that overflows (or a NaN norm) gets a bounded s, not a cast of an
+   * infinite log2: 2^20 marks it unusable, NaN plans s = 0 (the reference rejects
+   * both before it plans) */
+  const double ratio = norm1 / orc_pade_theta_d(13);
+  int s;
+  if (!(ratio <= DBL_MAX)) s = isnan(ratio) ? 0 : (1 << 20);
+  else s = (int)ceil(log2(ratio));
   if (s < 0) s = 0;
   out[0] = 13; out[1] = s; out[2] = orc_pade_num_gemms(13) + s;
 }
@@ -149,7 +155,10 @@ void orc_expm_plan_s(float norm1, int *out) {
       return;
     }
   }
-  int s = (int)ceilf(log2f(norm1 / orc_pade_theta_s(13)));
+  const float ratio = norm1 / orc_pade_theta_s(13);
+  int s;
+  if (!(ratio <= FLT_MAX)) s = isnan(ratio) ? 0 : (1 << 20);
+  else s = (int)ceilf(log2f(ratio));
   if (s < 0) s = 0;
   out[0] = 13; out[1] = s; out[2] = orc_pade_num_gemms(13) + s;
 }
@@ -528,6 +537,12 @@ int orc_lascl_c(float cfrom, float cto, size_t m, size_t n, float *A, size_t lda
  * ========================================================================= */
 #include <immintrin.h>
 
+/* aligned_alloc wants a size that is a multiple of the alignment (C11 7.22.3.1) */
+static double *alloc64(size_t count) {
+  const size_t bytes = (sizeof(double) * count + 63) & ~(size_t)63;
+  return (double *)aligned_alloc(64, bytes ? bytes : 64);
+}
+
 #define GK_MR 8     /* rows per micro-tile (two 4-double vectors) */
 #define GK_NR 6     /* columns per micro-tile */
 #define GK_KC 256   /* k block */
@@ -577,8 +592,8 @@ static void gemm_d(int m, int n, int k, double alpha, const double *A, int lda, 
   if (m < 1 || n < 1) return;
   const int ms = (m + GK_MR - 1) / GK_MR, ns = (n + GK_NR - 1) / GK_NR;
   const int kk = k < 1 ? 1 : k;
-  double *Ap = (double *)aligned_alloc(64, sizeof(double) * (size_t)ms * GK_MR * kk);
-  double *Bp = (double *)aligned_alloc(64, sizeof(double) * (size_t)ns * GK_NR * kk);
+  double *Ap = alloc64((size_t)ms * GK_MR * kk);
+  double *Bp = alloc64((size_t)ns * GK_NR * kk);
 #pragma omp parallel
   {
 #pragma omp for schedule(static)
@@ -599,7 +614,7 @@ static void gemm_d(int m, int n, int k, double alpha, const double *A, int lda, 
         for (int p = 0; p < k; ++p) dst[(size_t)p * GK_NR + c] = (j0 + c < n) ? src[p] : 0.0;
       }
     }
-    double *tile = (double *)aligned_alloc(64, sizeof(double) * GK_MS * GK_NS * GK_MR * GK_NR);
+    double *tile = alloc64(GK_MS * GK_NS * GK_MR * GK_NR);
     const int tm = (ms + GK_MS - 1) / GK_MS, tn = (ns + GK_NS - 1) / GK_NS;
 #pragma omp for collapse(2) schedule(dynamic, 1)
     for (int ti = 0; ti < tm; ++ti) {
@@ -687,8 +702,8 @@ static void gemm_z(int m, int n, int k, const double *alpha, const double *A, in
   const int ms = (m + GZ_MR - 1) / GZ_MR, ns = (n + GZ_NR - 1) / GZ_NR;
   const int kk = k < 1 ? 1 : k;
   const int beta_zero = (beta[0] == 0.0 && beta[1] == 0.0);
-  double *Ap = (double *)aligned_alloc(64, sizeof(double) * (size_t)ms * 2 * GZ_MR * kk);
-  double *Bp = (double *)aligned_alloc(64, sizeof(double) * (size_t)ns * 2 * GZ_NR * kk);
+  double *Ap = alloc64((size_t)ms * 2 * GZ_MR * kk);
+  double *Bp = alloc64((size_t)ns * 2 * GZ_NR * kk);
 #pragma omp parallel
   {
 #pragma omp for schedule(static)
@@ -718,7 +733,7 @@ static void gemm_z(int m, int n, int k, const double *alpha, const double *A, in
       }
     }
     const size_t tile_elems = (size_t)GZ_MS * GZ_NS * 2 * GZ_MR * GZ_NR;
-    double *tile = (double *)aligned_alloc(64, sizeof(double) * tile_elems);
+    double *tile = alloc64(tile_elems);
     const int tm = (ms + GZ_MS - 1) / GZ_MS, tn = (ns + GZ_NS - 1) / GZ_NS;
 #pragma omp for collapse(2) schedule(dynamic, 1)
     for (int ti = 0; ti < tm; ++ti) {
@@ -1509,19 +1524,29 @@ void orc_taylor_plan_powers_s(const double *pn, int q, int *out) {
   taylor_plan_powers(pn, q, kTaylorThetaS, out);
 }
 
-/* The powers whose norms are taken: the norm-only plan's bank A^2..A^q when it
- * holds A^3 and ||A||_1 <= 2^200 (1 = none). */
+/* The powers formed first: A^2 and A^3 when the norm-only plan's bank holds A^3
+ * and ||A||_1 <= 2^200 (3), else none (1). */
 static int norm_power_count(double norm1, int degree) {
   if (!(norm1 <= 0x1p200)) return 1;
   int pl[5];
   orc_ps_plan(degree, 0, pl);
-  const int q = pl[2] + 1;
-  if (q < 3) return 1;
-  return q < 5 ? q : 5;
+  return pl[2] + 1 >= 3 ? 3 : 1;
 }
 
-/* The build's Taylor expm: the norm-only plan's power bank formed from A itself,
- * its exact 1-norms, the plan of taylor_plan_powers, then A / 2^s and the bank
+/* Whether forming A^(q+1) can lower the plan's cost: the best plan with
+ * ||A^(q+1)|| taken as 0 against the best plan without it (q < 5). */
+static int worth_next_power(const double *pn, int q, const double *tab) {
+  if (q < 3 || q >= 5) return 0;
+  double trial[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < q; ++k) trial[k] = pn[k];
+  int with_next[4], now[4];
+  taylor_plan_powers(trial, q + 1, tab, with_next);
+  taylor_plan_powers(pn, q, tab, now);
+  return with_next[2] < now[2];
+}
+
+/* The build's Taylor expm: A^2, A^3 (then A^4, A^5 while worth_next_power) formed
+ * from A itself, their exact 1-norms, the plan of taylor_plan_powers, then A / 2^s and the bank
  * scaled in place by 2^(-k s) (ldexp, exact for normal results), the polynomial
  * continued from the powers already formed, s squarings. plan[4] = q and
  * pnorm[k-2] = ||A^k||_1 (k = 2..q) when given. */
@@ -1534,10 +1559,11 @@ static int expm_taylor_impl(int n, const double *A, int lda, double *E, int lde,
   const size_t nn = (size_t)n * n;
   int p0[4];
   orc_taylor_plan_d(norm, p0);
-  const int q = norm_power_count(norm, p0[0]);
+  int q = norm_power_count(norm, p0[0]);
   double pn[5] = {norm, 0, 0, 0, 0};
   double *pw[4] = {NULL, NULL, NULL, NULL};
   const double one_z[2] = {1.0, 0.0}, zero_z[2] = {0.0, 0.0};
+  /* A^2, A^3, then A^4, A^5 one at a time while worth_next_power */
   for (int k = 2; k <= q; ++k) {
     pw[k - 2] = (double *)malloc(sizeof(double) * es * nn);
     const double *left = (k == 2) ? A : pw[k - 3];
@@ -1545,6 +1571,7 @@ static int expm_taylor_impl(int n, const double *A, int lda, double *E, int lde,
     if (cplx) orc_gemm_z(n, one_z, left, ldl, A, lda, zero_z, pw[k - 2], n);
     else orc_gemm_d(n, 1.0, left, ldl, A, lda, 0.0, pw[k - 2], n);
     pn[k - 1] = cplx ? orc_norm1_z(n, pw[k - 2], n) : orc_norm1_d(n, pw[k - 2], n);
+    if (k == q && worth_next_power(pn, q, kTaylorThetaD)) ++q;
   }
   int p[4];
   taylor_plan_powers(pn, q, kTaylorThetaD, p);

@@ -164,17 +164,17 @@ def test_expm_and_horner_buffer_sizes():
         out = ctypes.c_size_t()
         assert L.clm_expm_buffer_size_d(n, ctypes.byref(out)) == 0
         colsum = ((n * 8 + 255) // 256) * 256
-        # colsum + As + a 4-power bank + one scratch block: six n x n blocks
-        assert out.value == colsum + 6 * blk
+        # colsum + 32 coefficients + As + a 4-power bank + one scratch block: six n x n blocks
+        assert out.value == colsum + 256 + 6 * blk
         assert L.clm_expm_buffer_size_z(n, ctypes.byref(out)) == 0
         zblk = ((n * n * 16 + 255) // 256) * 256
-        assert out.value == colsum + 6 * zblk
+        assert out.value == colsum + 512 + 6 * zblk
     assert L.clm_expm_buffer_size_d(0, ctypes.byref(out)) == ST.INVALID_VALUE
     for n in (1, 7, 64, 2048):
         colsum = ((n * 4 + 255) // 256) * 256
         for f, es in ((L.clm_expm_buffer_size_s, 4), (L.clm_expm_buffer_size_c, 8)):
             assert f(n, ctypes.byref(out)) == 0
-            assert out.value == colsum + 6 * (((n * n * es + 255) // 256) * 256)
+            assert out.value == colsum + 256 + 6 * (((n * n * es + 255) // 256) * 256)
         assert L.clm_horner_buffer_size_s(n) == ((n * n * 4 + 255) // 256) * 256
         assert L.clm_horner_buffer_size_c(n) == ((n * n * 8 + 255) // 256) * 256
     assert L.clm_paterson_stockmeyer_power_stride_s(5) == 64  # 100 B -> 256 B

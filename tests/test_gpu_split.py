@@ -106,9 +106,7 @@ def test_expm_split_meets_1e6(dev, n, target):
     # up to ||A||_1 = 256)
     assert plan == {**cg.expm_split_plan(plan["norm1"], power_norms=plan["power_norm1"], n=n),
                     "dtype": 1}
-    assert plan["s"] <= cg.expm_split_plan(plan["norm1"], n=n)["s"]
-    if plan["norm1"] > 256:
-        assert plan["norm_powers"] == 1
+    assert plan["norm_powers"] in ((1,) if plan["norm1"] > 256 else (1, 3, 4, 5))
 
 
 def test_expm_split_2048(dev):

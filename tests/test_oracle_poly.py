@@ -245,7 +245,8 @@ def test_power_plan_rule():
 
 def test_power_plan_expm_accuracy_non_normal():
     """Matrices far from normal (||A||_1 >> spectral radius): the plan from power norms
-    squares far less than the norm-only plan and the Taylor expm stays within 1e-13
+    takes fewer products than the norm-only plan (it may trade a squaring for a lower
+    degree, the powers formed for the norms counted) and the Taylor expm stays within 1e-13
     of the Pade oracle; an upper-triangular Jordan-like block and a dense Gaussian."""
     rng = np.random.default_rng(3)
     n = 48
@@ -261,6 +262,6 @@ def test_power_plan_expm_accuracy_non_normal():
     for a in cases:
         e, plan = oracle.expm_taylor(a, return_plan=True)
         p0 = oracle.taylor_plan(np.abs(a).sum(0).max())
-        assert plan["s"] < p0["s"]
+        assert plan["num_gemms"] < p0["num_gemms"]
         ref = oracle.expm(a)
         assert np.linalg.norm(e - ref) / np.linalg.norm(ref) <= 1e-13

@@ -5,5 +5,6 @@ C ABI of include/calaman_expm.h); this package is its Python host mirror.
 """
 from ._lib import CalamanError, LIB_PATH, lib  # noqa: F401
 from .expm import *  # noqa: F401,F403
+from . import cg  # noqa: F401
 
 __version__ = "0.1.0"

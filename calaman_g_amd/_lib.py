@@ -12,6 +12,7 @@ import re
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcalaman_expm.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "calaman_expm.h")
+CG_HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "calaman_cg.h")
 
 # status codes (calaman_expm.h)
 CLM_DOMAIN_BLAS = 0x00010000
@@ -107,6 +108,63 @@ class BatchAdEntry(ctypes.Structure):
 class BatchResult(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int), ("s", ctypes.c_int), ("num_gemms", ctypes.c_int),
                 ("status", ctypes.c_int), ("norm1", ctypes.c_double)]
+
+
+# ---- cg_unitary (include/calaman_cg.h) ----
+CG_MINIMIZE, CG_MAXIMIZE = 0, 1
+CG_SEARCH_POLYNOMIAL, CG_SEARCH_DFT = 0, 1
+CG_CONVERGED, CG_MAX_ITERATIONS, CG_LINE_SEARCH_FAILED, CG_NUMERICAL_FAILURE = 0, 1, 2, 3
+CG_MAX_SAMPLES = 32
+
+
+class CgSearchOptions(ctypes.Structure):
+    _fields_ = [("poly_order", ctypes.c_int), ("dft_factor", ctypes.c_int),
+                ("dft_periods", ctypes.c_int), ("use_spectral_radius", ctypes.c_int)]
+
+
+class CgOptions(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int), ("tol", ctypes.c_double),
+                ("method", ctypes.c_int), ("search", CgSearchOptions),
+                ("reset_period", ctypes.c_int), ("reorthogonalize_period", ctypes.c_int)]
+
+
+class CgInfo(ctypes.Structure):
+    _fields_ = [("reason", ctypes.c_int), ("iterations", ctypes.c_int), ("resets", ctypes.c_int),
+                ("line_search_failures", ctypes.c_int), ("gradient_norm_sq", ctypes.c_double),
+                ("last_step", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class CgSearchInfo(ctypes.Structure):
+    _fields_ = [("omega_max", ctypes.c_double), ("interval", ctypes.c_double),
+                ("num_samples", ctypes.c_int), ("num_roots", ctypes.c_int),
+                ("found", ctypes.c_int), ("method_used", ctypes.c_int),
+                ("samples", ctypes.c_double * CG_MAX_SAMPLES)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["samples"] = list(self.samples)[:self.num_samples]
+        return d
+
+
+CG_BUFFER_SIZE_FN = ctypes.CFUNCTYPE(ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int)
+CG_GRADIENT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.c_void_p, ctypes.c_size_t)
+CG_VALUE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                               ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+
+
+class CgCost(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("buffer_size", CG_BUFFER_SIZE_FN),
+                ("gradient", CG_GRADIENT_FN), ("value", CG_VALUE_FN), ("user", ctypes.c_void_p)]
+
+
+class CgBrockett(ctypes.Structure):
+    _fields_ = [("dR", ctypes.c_void_p), ("ldr", ctypes.c_int), ("dN", ctypes.c_void_p)]
 
 
 class CalamanError(RuntimeError):
@@ -219,14 +277,33 @@ _SIGNATURES = {
     "clm_expm_batched_ad_plan_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchAdEntry),
                                              _i]),
     "clm_expm_batched_backward": (_i, [_vp, _vp, _vp, _vp, _sz]),
+    # cg_unitary
+    "clm_cg_default_options_s": (None, [ctypes.POINTER(CgOptions)]),
+    "clm_cg_default_options_d": (None, [ctypes.POINTER(CgOptions)]),
 }
+for _t in "sdcz":
+    _SIGNATURES[f"clm_cg_brockett_cost_{_t}"] = (_i, [ctypes.POINTER(CgBrockett),
+                                                      ctypes.POINTER(CgCost)])
+    _SIGNATURES[f"clm_cg_unitary_buffer_size_{_t}"] = (_i, [_i, _sz, ctypes.POINTER(_sz)])
+    _SIGNATURES[f"clm_cg_unitary_{_t}"] = (_i, [_vp, _vp, _i, _vp, ctypes.POINTER(CgCost), _i, _vp,
+                                                _sz, ctypes.POINTER(CgOptions),
+                                                ctypes.POINTER(CgInfo)])
+    _SIGNATURES[f"clm_cg_geodesic_search_{_t}"] = (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp,
+                                                        ctypes.POINTER(CgCost), _i, _i, _vp, _sz,
+                                                        ctypes.POINTER(CgSearchOptions),
+                                                        ctypes.POINTER(_d),
+                                                        ctypes.POINTER(CgSearchInfo)])
+    _SIGNATURES[f"clm_cg_geodesic_step_{_t}"] = (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _sz])
 
 
-def header_symbols(path: str = HEADER_PATH):
-    """Every function the public C header declares."""
-    text = open(path).read()
+def header_symbols(path: str | None = None):
+    """Every function the public C headers declare."""
+    paths = [path] if path else [HEADER_PATH, CG_HEADER_PATH]
+    text = "".join(open(p).read() for p in paths)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(clm_[a-z0-9_]+)\s*\(", text)))
+    # a type name before a function-pointer declarator, "clm_status_t (*f)(...)", is no symbol
+    return sorted(set(n for n in re.findall(r"\b(clm_[a-z0-9_]+)\s*\(", text)
+                      if not n.endswith("_t")))
 
 
 def lib():

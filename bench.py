@@ -322,22 +322,26 @@ def adjoint_leg(dev, steps, warmup, b=512, n=128, t=0.5, seed=5):
         grad = step()
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
-    # algorithmic FLOP from the executed plans: forward num_gemms products, backward
-    # two per squaring, two per Horner step and power, plus recomputed squarings
-    from calaman_g_amd.expm import expm_plan
-    from calaman_g_amd import _lib
-    import calaman_g_amd as cg
-    norms = [float(np.abs(m).sum(0).max()) for m in a_np]
+    # algorithmic FLOP from the executed plans (a forward launch of the same members
+    # reports them): forward num_gemms products; backward two per squaring, two per
+    # Horner step and per power of the executed rung's bank, plus recomputed squarings
+    ents = []
+    for i in range(b):
+        acm = a[i].detach().t().contiguous()
+        ents.append((n, acm, n, torch.empty_like(acm), n))
+    fplan = BatchPlan(ents)
+    res = fplan.results_buffer()
+    check_status = fplan.launch(fplan.workspace(), res)
+    if check_status != 0:
+        raise RuntimeError("batched forward for the plan report failed")
     flop = 0.0
-    for nm in norms:
-        p = expm_plan(nm, torch.complex64)
-        ps = cg.ps_plan(p["m"])
-        sq = p["s"]
+    for r in BatchPlan.decode_results(res):
+        ps = cg.ps_plan(r["m"])
+        sq = r["s"]
         cstep = max(1, -(-sq // 4))
         recompute = sum(k % cstep for k in range(sq))
-        fwd = p["num_gemms"]
         bwd = 2 * sq + 2 * ps["r"] + 2 * ps["num_powers"] + recompute
-        flop += (fwd + bwd) * 8.0 * n ** 3
+        flop += (r["num_gemms"] + bwd) * 8.0 * n ** 3
     achieved = flop / sec / 1e12
     return {"fwd_bwd_ms": round(sec * 1e3, 3), "batches_per_s": round(1.0 / sec, 2),
             "matrices": b, "n": n, "dtype": "c64", "checkpoints": 4,
@@ -437,17 +441,47 @@ def fp8tail_leg(dev, steps, warmup):
     return out
 
 
-def cpu_baseline(mats):
+def _median_runs(fn, runs):
+    """1 warm-up call, then the median wall time of `runs` calls; returns (seconds, last output)"""
+    out = fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], out
+
+
+def cpu_baseline(mats, runs=5):
+    """The oracle (the C restatement of the reference's Pade expm, OpenMP over every
+    core) timed as planned for the host: 1 warm-up, then the median of `runs` calls,
+    for each n = 2048 real input (the GPU workload), one complex128 n = 2048 input
+    and the n = 256 inputs. `value` is the n = 2048 real FP64 rate: the number of
+    norms over the sum of the per-norm medians."""
     import oracle
     threads = oracle.num_threads()
-    t0 = time.perf_counter()
-    outs = [oracle.expm(a) for a in mats]
-    dt = time.perf_counter() - t0
-    return {"value": round(len(mats) / dt, 4), "unit": "expm/s", "cores": threads,
+    t_start = time.perf_counter()
+    med, outs = [], []
+    for a in mats:
+        t, e = _median_runs(lambda a=a: oracle.expm(a), runs)
+        med.append(t)
+        outs.append(e)
+    small = make_inputs(256, cplx=False)
+    small_med = [_median_runs(lambda a=a: oracle.expm(a), runs)[0] for a in small]
+    zc = make_inputs(N, cplx=True)[1]
+    z_med = _median_runs(lambda: oracle.expm(zc), runs)[0]
+    total = time.perf_counter() - t_start
+    return {"value": round(len(mats) / sum(med), 4), "unit": "expm/s", "cores": threads,
             "kind": "port",
-            "sample": f"one n={N} FP64 expm at each ||A||_1 in {list(NORMS)} ({len(mats)} expms, "
-                      f"{dt:.1f} s): the oracle's C restatement of the reference Pade expm, "
-                      f"OpenMP {threads} threads"}, outs
+            "per_norm_median_s": {str(t): round(m, 4) for t, m in zip(NORMS, med)},
+            "n256_expm_per_s": round(len(small) / sum(small_med), 2),
+            "complex128_n2048_norm8_median_s": round(z_med, 4),
+            "compiler": "gcc -O3 -march=x86-64-v3 -ffp-contract=fast -fopenmp",
+            "sample": f"1 warm-up + median of {runs} runs of the oracle's C restatement of the "
+                      f"reference Pade expm (OpenMP {threads} threads) for each n={N} FP64 "
+                      f"input at ||A||_1 in {list(NORMS)}, for the n=256 inputs and one "
+                      f"complex128 n={N} input at ||A||_1=8; {total:.1f} s of CPU work"}, outs
 
 
 def main():

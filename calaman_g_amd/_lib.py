@@ -282,6 +282,13 @@ _SIGNATURES = {
     "clm_cg_default_options_d": (None, [ctypes.POINTER(CgOptions)]),
 }
 for _t in "sdcz":
+    _SIGNATURES[f"clm_pade_buffer_size_{_t}"] = (_i, [_i, _i, ctypes.POINTER(_sz)])
+    _SIGNATURES[f"clm_pade_{_t}"] = (_i, [_vp, _vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz, _vp])
+    _SIGNATURES[f"clm_expm_pade_buffer_size_{_t}"] = (_i, [_i, ctypes.POINTER(_sz)])
+    _SIGNATURES[f"clm_expm_pade_{_t}"] = (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
+                                               ctypes.POINTER(ExpmPlan)])
+    _SIGNATURES[f"clm_getrf_{_t}"] = (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp])
+    _SIGNATURES[f"clm_getrs_{_t}"] = (_i, [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp])
     _SIGNATURES[f"clm_cg_brockett_cost_{_t}"] = (_i, [ctypes.POINTER(CgBrockett),
                                                       ctypes.POINTER(CgCost)])
     _SIGNATURES[f"clm_cg_unitary_buffer_size_{_t}"] = (_i, [_i, _sz, ctypes.POINTER(_sz)])
@@ -294,6 +301,22 @@ for _t in "sdcz":
                                                         ctypes.POINTER(_d),
                                                         ctypes.POINTER(CgSearchInfo)])
     _SIGNATURES[f"clm_cg_geodesic_step_{_t}"] = (_i, [_vp, _vp, _i, _vp, _i, _d, _vp, _vp, _sz])
+    # balancing, the balanced exponential
+    _SIGNATURES[f"clm_gebal_{_t}"] = (_i, [_vp, _i, _i, _vp, _i, ctypes.POINTER(_i),
+                                           ctypes.POINTER(_i), _vp, _vp, _i])
+    _SIGNATURES[f"clm_expm_balanced_buffer_size_{_t}"] = (_i, [_i, ctypes.POINTER(_sz)])
+    _SIGNATURES[f"clm_expm_balanced_{_t}"] = (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _vp, _sz, _vp,
+                                                   ctypes.POINTER(ExpmPlan)])
+    # the Hermitian route
+    _SIGNATURES[f"clm_expm_herm_buffer_size_{_t}"] = (_i, [_i, ctypes.POINTER(_sz)])
+    _SIGNATURES[f"clm_expm_herm_{_t}"] = (_i, [_vp, _vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz, _vp])
+_SIGNATURES["clm_gebal_buffer_size"] = (_i, [_i, ctypes.POINTER(_i)])
+_SIGNATURES["clm_lacn2_buffer_size_s"] = (_sz, [_i])
+_SIGNATURES["clm_lacn2_buffer_size_d"] = (_sz, [_i])
+_SIGNATURES["clm_lacn2_s"] = (_i, [_vp, _i, _vp, _vp, _vp, _sz, ctypes.POINTER(_f),
+                                   ctypes.POINTER(_i), ctypes.POINTER(_i)])
+_SIGNATURES["clm_lacn2_d"] = (_i, [_vp, _i, _vp, _vp, _vp, _sz, ctypes.POINTER(_d),
+                                   ctypes.POINTER(_i), ctypes.POINTER(_i)])
 
 
 def header_symbols(path: str | None = None):

@@ -13,6 +13,7 @@
 // Paterson-Stockmeyer instead of the reference's Pade approximant + LU: same
 // accuracy target (backward error <= u), no LU, and every product is an MFMA GEMM.
 // Argument checks come before any HIP call, so rejected calls never touch a device.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -20,6 +21,7 @@
 #include <limits>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "clm_internal.hpp"
 #include "expm_plan.hpp"
@@ -945,6 +947,233 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   return CLM_STATUS_SUCCESS;
 }
 
+// ---- the reference's Pade route on the GPU (src/expm/pade.cppm:77-191,
+//      src/expm/expm.cppm:87-174) -------------------------------------------------------------
+// b_j = (2m-j)! m! / ((2m)! j! (m-j)!), b_0 = 1 (src/expm/expm_bridge.h:58-113): each entry
+// the correctly rounded quotient of the two integers
+const double kPadeB3[4] = {1.0, 1.0 / 2.0, 1.0 / 10.0, 1.0 / 120.0};
+const double kPadeB5[6] = {1.0, 1.0 / 2.0, 1.0 / 9.0, 1.0 / 72.0, 1.0 / 1008.0, 1.0 / 30240.0};
+const double kPadeB7[8] = {1.0,          1.0 / 2.0,     3.0 / 26.0,     5.0 / 312.0,
+                           5.0 / 3432.0, 1.0 / 11440.0, 1.0 / 308880.0, 1.0 / 17297280.0};
+const double kPadeB9[10] = {1.0,          1.0 / 2.0,         2.0 / 17.0,          7.0 / 408.0,
+                            7.0 / 4080.0, 1.0 / 8160.0,      1.0 / 159120.0,      1.0 / 4455360.0,
+                            1.0 / 196035840.0, 1.0 / 17643225600.0};
+const double kPadeB13[14] = {1.0,
+                             1.0 / 2.0,
+                             3.0 / 25.0,
+                             11.0 / 600.0,
+                             11.0 / 5520.0,
+                             3.0 / 18400.0,
+                             1.0 / 96600.0,
+                             1.0 / 1932000.0,
+                             1.0 / 48944000.0,
+                             1.0 / 1585785600.0,
+                             1.0 / 67395888000.0,
+                             1.0 / 3953892096000.0,
+                             1.0 / 355850288640000.0,
+                             1.0 / 64764752532480000.0};
+const double *pade_coeffs(int m) {
+  switch (m) {
+  case 3: return kPadeB3;
+  case 5: return kPadeB5;
+  case 7: return kPadeB7;
+  case 9: return kPadeB9;
+  case 13: return kPadeB13;
+  default: return nullptr;
+  }
+}
+int pade_num_powers(int m) { return m == 13 ? 3 : (m - 1) / 2; }
+
+// [P0 .. P(np-1) | X (m = 13) | V | W | ipiv (n ints) | info (2 ints)]: A^2, A^4, A^6,
+// A^8 as the degree needs them; U = A W reuses P0 and q_m(A) = V - U reuses V
+template <typename T>
+struct PadeWs {
+  T *P[4] = {nullptr, nullptr, nullptr, nullptr};
+  T *X = nullptr, *V = nullptr, *W = nullptr;
+  int *ipiv = nullptr, *info = nullptr;
+  size_t carve(void *base, int n, int m) {
+    Layout L(base);
+    const size_t nn = static_cast<size_t>(n) * n;
+    const int np = pade_num_powers(m);
+    for (int k = 0; k < np; ++k) P[k] = L.take<T>(nn);
+    if (m == 13) X = L.take<T>(nn);
+    V = L.take<T>(nn);
+    W = L.take<T>(nn);
+    ipiv = L.take<int>(static_cast<size_t>(n));
+    info = L.take<int>(2);
+    return L.total();
+  }
+};
+
+template <typename T>
+size_t pade_buffer_size(int n, int m) {
+  PadeWs<T> w;
+  return w.carve(nullptr, n, m);
+}
+template <typename T>
+size_t pade_worst_buffer_size(int n) {
+  size_t worst = 0;
+  for (int m : {3, 5, 7, 9, 13}) worst = std::max(worst, pade_buffer_size<T>(n, m));
+  return worst;
+}
+// [As | n*n T] [colsum | n reals] [the Pade region for the worst degree]; the squarings
+// ping-pong through As once the Pade evaluation has consumed it
+template <typename T>
+size_t expm_pade_head(int n) {
+  Layout L(nullptr);
+  (void)L.take<T>(static_cast<size_t>(n) * n);
+  (void)L.take<real_t<T>>(static_cast<size_t>(n));
+  return L.total();
+}
+template <typename T>
+size_t expm_pade_buffer_size(int n) {
+  return expm_pade_head<T>(n) + pade_worst_buffer_size<T>(n);
+}
+
+template <typename T>
+hipError_t product(hipStream_t st, int n, const T *X, long ldx, const T *Y, long ldy, T *Z, long ldz,
+                   bool accumulate) {
+  GemmArgs<T> g;
+  g.M = g.N = g.K = n;
+  g.A = X;
+  g.lda = ldx;
+  g.B = Y;
+  g.ldb = ldy;
+  g.C = Z;
+  g.ldc = ldz;
+  g.alpha = one<T>();
+  g.beta = accumulate ? one<T>() : ops<T>::zero();
+  g.Cin = accumulate ? Z : nullptr;
+  g.ldcin = ldz;
+  return launch_gemm<T>(st, g);
+}
+
+// r_m(A) = q_m(A)^-1 p_m(A) into R: powers, the even and odd halves, U = A W, the
+// split, then getrf / getrs of (V - U) X = (V + U); info[0] getrf's, info[1] = 0
+template <typename T>
+clm_status_t pade_run(hipStream_t st, int m, int n, const T *A, long lda, T *R, long ldr,
+                      const PadeWs<T> &ws, int *info) {
+  const double *b = pade_coeffs(m);
+  const int np = pade_num_powers(m);
+  T *const *P = ws.P;
+  CLM_TRY_HIP(product<T>(st, n, A, lda, A, lda, P[0], n, false));
+  if (np >= 2) CLM_TRY_HIP(product<T>(st, n, P[0], n, P[0], n, P[1], n, false));
+  if (np >= 3) CLM_TRY_HIP(product<T>(st, n, P[1], n, P[0], n, P[2], n, false));
+  if (np >= 4) CLM_TRY_HIP(product<T>(st, n, P[1], n, P[1], n, P[3], n, false));
+  const T *Pc[4] = {P[0], P[1], P[2], P[3]};
+  double cv[5] = {}, cw[5] = {};
+  if (m != 13) {
+    for (int i = 0; i <= np; ++i) {
+      cv[i] = b[2 * i];
+      cw[i] = b[2 * i + 1];
+    }
+    CLM_TRY_HIP(launch_pade_even_odd<T>(st, n, np, Pc, cv, cw, ws.V, ws.W));
+  } else {
+    // the degree-13 nested form: W = A6 (b13 A6 + b11 A4 + b9 A2) + b7 A6 + b5 A4 + b3 A2 + b1 I,
+    // V likewise with the even coefficients
+    const double odd_v[4] = {0.0, b[9], b[11], b[13]}, odd_w[4] = {b[1], b[3], b[5], b[7]};
+    CLM_TRY_HIP(launch_pade_even_odd<T>(st, n, 3, Pc, odd_v, odd_w, ws.X, ws.W));
+    CLM_TRY_HIP(product<T>(st, n, P[2], n, ws.X, n, ws.W, n, true));
+    const double even_v[4] = {0.0, b[8], b[10], b[12]}, even_w[4] = {b[0], b[2], b[4], b[6]};
+    CLM_TRY_HIP(launch_pade_even_odd<T>(st, n, 3, Pc, even_v, even_w, ws.X, ws.V));
+    CLM_TRY_HIP(product<T>(st, n, P[2], n, ws.X, n, ws.V, n, true));
+  }
+  T *U = P[0];  // the powers are dead from here
+  CLM_TRY_HIP(product<T>(st, n, A, lda, ws.W, n, U, n, false));
+  CLM_TRY_HIP(launch_pade_split<T>(st, n, U, ws.V, R, ldr, ws.V));
+  CLM_TRY_HIP(launch_getrf<T>(st, n, ws.V, n, ws.ipiv, info));
+  CLM_TRY_HIP(launch_getrs<T>(st, n, n, ws.V, n, ws.ipiv, R, ldr));
+  CLM_TRY_HIP(launch_fill_int(st, info + 1, 1, 0));
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t pade_impl(clm_ctx *ctx, hipStream_t st, int m, int n, const T *dA, int lda, T *dR,
+                       int ldr, void *work, size_t bytes, int *dInfo) {
+  if (pade_coeffs(m) == nullptr || n < 1 || n > kMaxDim || lda < n || ldr < n || dA == nullptr ||
+      dR == nullptr || work == nullptr)
+    return CLM_STATUS_INVALID_VALUE;
+  if (bytes < pade_buffer_size<T>(n, m)) return CLM_STATUS_ALLOC_FAILED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  PadeWs<T> ws;
+  ws.carve(work, n, m);
+  return pade_run<T>(st, m, n, dA, lda, dR, ldr, ws, dInfo ? dInfo : ws.info);
+}
+
+// expm<T> as the reference evaluates it: the 1-norm (one synchronization), the Pade
+// plan, 2^-s A, r_m, s squarings, the odd-s copy
+template <typename T>
+clm_status_t expm_pade_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda, T *dE,
+                            int lde, void *work, size_t bytes, int *dInfo, clm_expm_plan *plan) {
+  if (n < 1 || n > kMaxDim || lda < n || lde < n || work == nullptr || dA == nullptr ||
+      dE == nullptr)
+    return CLM_STATUS_INVALID_VALUE;
+  if (bytes < expm_pade_buffer_size<T>(n)) return CLM_STATUS_ALLOC_FAILED;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  Layout L(work);
+  T *As = L.take<T>(static_cast<size_t>(n) * n);
+  real_t<T> *colsum = L.take<real_t<T>>(static_cast<size_t>(n));
+  void *pade_base = static_cast<char *>(work) + L.total();
+  real_t<T> norm = 0;
+  CLM_TRY(read_norm1(st, n, dA, lda, colsum, &norm, c->device, true));
+  if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
+  clm_expm_plan p = is_single<T>() ? pade_plan_single(static_cast<float>(norm))
+                                   : pade_plan(static_cast<double>(norm));
+  p.dtype = dtype_code<T>();
+  p.norm_powers = 1;
+  if (plan) *plan = p;
+  const T *src = dA;
+  long ld_src = lda;
+  if (p.s > 0) {
+    CLM_TRY_HIP(launch_scale_copy<T>(st, n, std::ldexp(1.0, -p.s), dA, lda, As, n));
+    src = As;
+    ld_src = n;
+  }
+  PadeWs<T> ws;
+  ws.carve(pade_base, n, p.m);
+  CLM_TRY(pade_run<T>(st, p.m, n, src, ld_src, dE, lde, ws, dInfo ? dInfo : ws.info));
+  T *cur = dE, *other = As;
+  long ld_cur = lde, ld_other = n;
+  for (int k = 0; k < p.s; ++k) {
+    CLM_TRY_HIP(product<T>(st, n, cur, ld_cur, cur, ld_cur, other, ld_other, false));
+    std::swap(cur, other);
+    std::swap(ld_cur, ld_other);
+  }
+  if (cur != dE) CLM_TRY_HIP(launch_copy<T>(st, n, cur, ld_cur, dE, lde));
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t getrf_impl(clm_ctx *ctx, hipStream_t st, int n, T *dA, int lda, int *dIpiv,
+                        int *dInfo) {
+  if (n < 0 || n > kMaxDim || lda < (n > 1 ? n : 1) || dInfo == nullptr ||
+      (n > 0 && (dA == nullptr || dIpiv == nullptr)))
+    return CLM_STATUS_INVALID_VALUE;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  if (n == 0) {
+    CLM_TRY_HIP(launch_fill_int(st, dInfo, 1, 0));
+    return CLM_STATUS_SUCCESS;
+  }
+  CLM_TRY_HIP(launch_getrf<T>(st, n, dA, lda, dIpiv, dInfo));
+  return CLM_STATUS_SUCCESS;
+}
+
+template <typename T>
+clm_status_t getrs_impl(clm_ctx *ctx, hipStream_t st, int n, int nrhs, const T *dLU, int lda,
+                        const int *dIpiv, T *dB, int ldb, int *dInfo) {
+  if (n < 0 || nrhs < 0 || n > kMaxDim || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) ||
+      (n > 0 && nrhs > 0 && (dLU == nullptr || dIpiv == nullptr || dB == nullptr)))
+    return CLM_STATUS_INVALID_VALUE;
+  clm_ctx *c = nullptr;
+  CLM_TRY(resolve_ctx(ctx, &c));
+  CLM_TRY_HIP(launch_getrs<T>(st, n, nrhs, dLU, lda, dIpiv, dB, ldb));
+  if (dInfo) CLM_TRY_HIP(launch_fill_int(st, dInfo, 1, 0));
+  return CLM_STATUS_SUCCESS;
+}
+
 template <typename T>
 clm_status_t ps_impl(clm_ctx *ctx, int n, const T *dCoeffs, int degree, const T *dA, int lda,
                      T *dP, int ldp, void *work, size_t work_bytes, int s_requested) {
@@ -1610,6 +1839,138 @@ clm_status_t clm_gemm_c(clm_ctx *ctx, int m, int n, int k, clm_float_complex alp
   return clm::gemm_impl<zc>(ctx, m, n, k, zc{alpha.re, alpha.im}, reinterpret_cast<const zc *>(dA),
                             lda, reinterpret_cast<const zc *>(dB), ldb, zc{beta.re, beta.im},
                             reinterpret_cast<zc *>(dC), ldc);
+}
+
+clm_status_t clm_pade_buffer_size_s(int m, int n, size_t *bytes) {
+  if (bytes == nullptr || clm::pade_coeffs(m) == nullptr || n < 1 || n > clm::kMaxDim)
+    return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::pade_buffer_size<float>(n, m);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_pade_s(clm_ctx *ctx, clm_stream_t stream, int m, int n, const float *dA, int lda,
+                        float *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo) {
+  return clm::pade_impl<float>(ctx, static_cast<hipStream_t>(stream), m, n, dA, lda,
+                             dR, ldr, dWork, lwork_bytes, dInfo);
+}
+clm_status_t clm_expm_pade_buffer_size_s(int n, size_t *bytes) {
+  if (bytes == nullptr || n < 1 || n > clm::kMaxDim) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_pade_buffer_size<float>(n);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_expm_pade_s(clm_ctx *ctx, clm_stream_t stream, int n, const float *dA, int lda,
+                             float *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan) {
+  return clm::expm_pade_impl<float>(ctx, static_cast<hipStream_t>(stream), n, dA, lda,
+                                  dE, lde, dWork, lwork_bytes, dInfo, plan);
+}
+clm_status_t clm_getrf_s(clm_ctx *ctx, clm_stream_t stream, int n, float *dA, int lda, int *dIpiv,
+                         int *dInfo) {
+  return clm::getrf_impl<float>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dIpiv,
+                              dInfo);
+}
+clm_status_t clm_getrs_s(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const float *dLU,
+                         int lda, const int *dIpiv, float *dB, int ldb, int *dInfo) {
+  return clm::getrs_impl<float>(ctx, static_cast<hipStream_t>(stream), n, nrhs, dLU, lda,
+                              dIpiv, dB, ldb, dInfo);
+}
+
+clm_status_t clm_pade_buffer_size_d(int m, int n, size_t *bytes) {
+  if (bytes == nullptr || clm::pade_coeffs(m) == nullptr || n < 1 || n > clm::kMaxDim)
+    return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::pade_buffer_size<double>(n, m);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_pade_d(clm_ctx *ctx, clm_stream_t stream, int m, int n, const double *dA, int lda,
+                        double *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo) {
+  return clm::pade_impl<double>(ctx, static_cast<hipStream_t>(stream), m, n, dA, lda,
+                             dR, ldr, dWork, lwork_bytes, dInfo);
+}
+clm_status_t clm_expm_pade_buffer_size_d(int n, size_t *bytes) {
+  if (bytes == nullptr || n < 1 || n > clm::kMaxDim) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_pade_buffer_size<double>(n);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_expm_pade_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                             double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan) {
+  return clm::expm_pade_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda,
+                                  dE, lde, dWork, lwork_bytes, dInfo, plan);
+}
+clm_status_t clm_getrf_d(clm_ctx *ctx, clm_stream_t stream, int n, double *dA, int lda, int *dIpiv,
+                         int *dInfo) {
+  return clm::getrf_impl<double>(ctx, static_cast<hipStream_t>(stream), n, dA, lda, dIpiv,
+                              dInfo);
+}
+clm_status_t clm_getrs_d(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const double *dLU,
+                         int lda, const int *dIpiv, double *dB, int ldb, int *dInfo) {
+  return clm::getrs_impl<double>(ctx, static_cast<hipStream_t>(stream), n, nrhs, dLU, lda,
+                              dIpiv, dB, ldb, dInfo);
+}
+
+clm_status_t clm_pade_buffer_size_c(int m, int n, size_t *bytes) {
+  if (bytes == nullptr || clm::pade_coeffs(m) == nullptr || n < 1 || n > clm::kMaxDim)
+    return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::pade_buffer_size<zc>(n, m);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_pade_c(clm_ctx *ctx, clm_stream_t stream, int m, int n, const clm_float_complex *dA, int lda,
+                        clm_float_complex *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo) {
+  return clm::pade_impl<zc>(ctx, static_cast<hipStream_t>(stream), m, n, reinterpret_cast<const zc *>(dA), lda,
+                             reinterpret_cast<zc *>(dR), ldr, dWork, lwork_bytes, dInfo);
+}
+clm_status_t clm_expm_pade_buffer_size_c(int n, size_t *bytes) {
+  if (bytes == nullptr || n < 1 || n > clm::kMaxDim) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_pade_buffer_size<zc>(n);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_expm_pade_c(clm_ctx *ctx, clm_stream_t stream, int n, const clm_float_complex *dA, int lda,
+                             clm_float_complex *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan) {
+  return clm::expm_pade_impl<zc>(ctx, static_cast<hipStream_t>(stream), n, reinterpret_cast<const zc *>(dA), lda,
+                                  reinterpret_cast<zc *>(dE), lde, dWork, lwork_bytes, dInfo, plan);
+}
+clm_status_t clm_getrf_c(clm_ctx *ctx, clm_stream_t stream, int n, clm_float_complex *dA, int lda, int *dIpiv,
+                         int *dInfo) {
+  return clm::getrf_impl<zc>(ctx, static_cast<hipStream_t>(stream), n, reinterpret_cast<zc *>(dA), lda, dIpiv,
+                              dInfo);
+}
+clm_status_t clm_getrs_c(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const clm_float_complex *dLU,
+                         int lda, const int *dIpiv, clm_float_complex *dB, int ldb, int *dInfo) {
+  return clm::getrs_impl<zc>(ctx, static_cast<hipStream_t>(stream), n, nrhs, reinterpret_cast<const zc *>(dLU), lda,
+                              dIpiv, reinterpret_cast<zc *>(dB), ldb, dInfo);
+}
+
+clm_status_t clm_pade_buffer_size_z(int m, int n, size_t *bytes) {
+  if (bytes == nullptr || clm::pade_coeffs(m) == nullptr || n < 1 || n > clm::kMaxDim)
+    return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::pade_buffer_size<zd>(n, m);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_pade_z(clm_ctx *ctx, clm_stream_t stream, int m, int n, const clm_double_complex *dA, int lda,
+                        clm_double_complex *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo) {
+  return clm::pade_impl<zd>(ctx, static_cast<hipStream_t>(stream), m, n, reinterpret_cast<const zd *>(dA), lda,
+                             reinterpret_cast<zd *>(dR), ldr, dWork, lwork_bytes, dInfo);
+}
+clm_status_t clm_expm_pade_buffer_size_z(int n, size_t *bytes) {
+  if (bytes == nullptr || n < 1 || n > clm::kMaxDim) return CLM_STATUS_INVALID_VALUE;
+  *bytes = clm::expm_pade_buffer_size<zd>(n);
+  return CLM_STATUS_SUCCESS;
+}
+clm_status_t clm_expm_pade_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_double_complex *dA, int lda,
+                             clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan) {
+  return clm::expm_pade_impl<zd>(ctx, static_cast<hipStream_t>(stream), n, reinterpret_cast<const zd *>(dA), lda,
+                                  reinterpret_cast<zd *>(dE), lde, dWork, lwork_bytes, dInfo, plan);
+}
+clm_status_t clm_getrf_z(clm_ctx *ctx, clm_stream_t stream, int n, clm_double_complex *dA, int lda, int *dIpiv,
+                         int *dInfo) {
+  return clm::getrf_impl<zd>(ctx, static_cast<hipStream_t>(stream), n, reinterpret_cast<zd *>(dA), lda, dIpiv,
+                              dInfo);
+}
+clm_status_t clm_getrs_z(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const clm_double_complex *dLU,
+                         int lda, const int *dIpiv, clm_double_complex *dB, int ldb, int *dInfo) {
+  return clm::getrs_impl<zd>(ctx, static_cast<hipStream_t>(stream), n, nrhs, reinterpret_cast<const zd *>(dLU), lda,
+                              dIpiv, reinterpret_cast<zd *>(dB), ldb, dInfo);
 }
 
 }  // extern "C"

@@ -255,6 +255,24 @@ hipError_t launch_copy(hipStream_t stream, int n, const T *src, long lds, T *dst
 // d[0..count) = v
 hipError_t launch_fill_int(hipStream_t stream, int *d, int count, int v);
 
+// ---- the Pade route: LU with partial pivoting, triangular solves, numerator passes ----------
+// A = P L U in place (unit-lower L below the diagonal, U on and above), ipiv 1-based
+// row interchanges; *info = first zero pivot's column (1-based) or 0
+template <typename T>
+hipError_t launch_getrf(hipStream_t st, int n, T *A, long lda, int *ipiv, int *info);
+// B <- A^-1 B from getrf's factors (nrhs columns)
+template <typename T>
+hipError_t launch_getrs(hipStream_t st, int n, int nrhs, const T *LU, long lda, const int *ipiv,
+                        T *B, long ldb);
+// V = cv[0] I + sum_k cv[k+1] P[k], W = cw[0] I + sum_k cw[k+1] P[k], k < np <= 4 (n x n
+// blocks, leading dimension n)
+template <typename T>
+hipError_t launch_pade_even_odd(hipStream_t st, int n, int np, const T *const *P,
+                                const double *cv, const double *cw, T *V, T *W);
+// R = V + U (leading dimension ldr), Q = V - U (Q may be V)
+template <typename T>
+hipError_t launch_pade_split(hipStream_t st, int n, const T *U, const T *V, T *R, long ldr, T *Q);
+
 // src[0 .. count-1] (count <= 5) once the stream reaches them: a flag polled in
 // host-mapped memory, then a blocking synchronization after 2 ms
 template <typename R>

@@ -552,6 +552,254 @@ def expm_batched(mats, stream: Optional[torch.cuda.Stream] = None):
         return [hand_over(e).t() for e in outs]
 
 
+# ---- the reference's Pade route (clm_pade_*, clm_expm_pade_*, clm_getrf_*, clm_getrs_*) ----------
+def pade_buffer_size(m: int, n: int, dtype: torch.dtype = _F64) -> int:
+    out = ctypes.c_size_t()
+    check(getattr(lib(), f"clm_pade_buffer_size_{_suffix(dtype)}")(int(m), int(n), ctypes.byref(out)),
+          "clm_pade_buffer_size")
+    return int(out.value)
+
+
+def expm_pade_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    out = ctypes.c_size_t()
+    check(getattr(lib(), f"clm_expm_pade_buffer_size_{_suffix(dtype)}")(int(n), ctypes.byref(out)),
+          "clm_expm_pade_buffer_size")
+    return int(out.value)
+
+
+def pade_cm(m: int, n: int, dA: torch.Tensor, lda: int, dR: torch.Tensor, ldr: int,
+            dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None,
+            stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> int:
+    """clm_pade_{s,d,c,z}: R = r_m(A) (pade<T>, src/expm/pade.cppm:77-191)"""
+    f = getattr(lib(), f"clm_pade_{_suffix(dA.dtype)}")
+    return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(m), int(n),
+                 _ptr(dA), int(lda), _ptr(dR), int(ldr), _ptr(dWork),
+                 dWork.numel() * dWork.element_size(), _ptr(dInfo)))
+
+
+def expm_pade_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
+                 dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None,
+                 stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> tuple:
+    """clm_expm_pade_{s,d,c,z}: the reference's expm<T> (Pade + LU); (status, plan dict)"""
+    plan = ExpmPlan()
+    f = getattr(lib(), f"clm_expm_pade_{_suffix(dA.dtype)}")
+    st = f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA),
+           int(lda), _ptr(dE), int(lde), _ptr(dWork), dWork.numel() * dWork.element_size(),
+           _ptr(dInfo), ctypes.byref(plan))
+    return int(st), plan.as_dict()
+
+
+def getrf_cm(n: int, dA: torch.Tensor, lda: int, dIpiv: torch.Tensor, dInfo: torch.Tensor,
+             stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> int:
+    """clm_getrf_{s,d,c,z}: A = P L U in place, ipiv 1-based"""
+    f = getattr(lib(), f"clm_getrf_{_suffix(dA.dtype)}")
+    return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n),
+                 _ptr(dA), int(lda), _ptr(dIpiv), _ptr(dInfo)))
+
+
+def getrs_cm(n: int, nrhs: int, dLU: torch.Tensor, lda: int, dIpiv: torch.Tensor,
+             dB: torch.Tensor, ldb: int, dInfo: Optional[torch.Tensor] = None,
+             stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> int:
+    """clm_getrs_{s,d,c,z}: B <- A^-1 B from getrf's factors"""
+    f = getattr(lib(), f"clm_getrs_{_suffix(dLU.dtype)}")
+    return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n),
+                 int(nrhs), _ptr(dLU), int(lda), _ptr(dIpiv), _ptr(dB), int(ldb), _ptr(dInfo)))
+
+
+def expm_pade(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
+              return_plan: bool = False):
+    """exp(a) by the reference's own algorithm: the Pade ladder, LU solve and squarings"""
+    _require_device(a, "a")
+    n = a.shape[-1]
+    if a.dim() != 2 or a.shape[0] != n:
+        raise ValueError("expm_pade expects a square matrix")
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_pade_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+        info = torch.empty(2, dtype=torch.int32, device=a.device)
+        st, plan = expm_pade_cm(n, acm, n, ecm, n, work, info, stream)
+        check(st, "expm_pade")
+        e = hand_over(ecm).t()
+    return (e, plan) if return_plan else e
+
+
+# ---- exp(A) for a symmetric / Hermitian A by its eigendecomposition ----------------------------
+FILL_MODES = {"L": 0, "U": 1}
+
+
+def _fill_mode(uplo) -> int:
+    if isinstance(uplo, str):
+        if uplo.upper() not in FILL_MODES:
+            raise ValueError(f"uplo {uplo!r}: 'L' or 'U'")
+        return FILL_MODES[uplo.upper()]
+    return int(uplo)
+
+
+def expm_herm_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    out = ctypes.c_size_t()
+    check(getattr(lib(), f"clm_expm_herm_buffer_size_{_suffix(dtype)}")(int(n), ctypes.byref(out)),
+          "clm_expm_herm_buffer_size")
+    return int(out.value)
+
+
+def expm_herm_cm(uplo, n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
+                 dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None,
+                 stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> int:
+    """clm_expm_herm_{s,d,c,z} (src/expm/expm_herm.cppm:145): exp(A) from the uplo triangle"""
+    f = getattr(lib(), f"clm_expm_herm_{_suffix(dA.dtype)}")
+    return int(f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), _fill_mode(uplo),
+                 int(n), _ptr(dA), int(lda), _ptr(dE), int(lde), _ptr(dWork),
+                 dWork.numel() * dWork.element_size(), _ptr(dInfo)))
+
+
+def expm_herm(a: torch.Tensor, uplo="U", stream: Optional[torch.cuda.Stream] = None,
+              return_info: bool = False):
+    """exp(a) for a symmetric / Hermitian device matrix given by its uplo triangle"""
+    _require_device(a, "a")
+    n = a.shape[-1]
+    if a.dim() != 2 or a.shape[0] != n:
+        raise ValueError("expm_herm expects a square matrix")
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_herm_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+        info = torch.zeros(1, dtype=torch.int32, device=a.device)
+        check(expm_herm_cm(uplo, n, acm, n, ecm, n, work, info, stream), "expm_herm")
+        e = hand_over(ecm).t()
+        info = hand_over(info)
+    return (e, int(info.item())) if return_info else e
+
+
+# ---- balancing (?gebal), the balanced exponential, the 1-norm estimator (?lacn2) ---------------
+GEBAL_JOBS = {"N": 0, "P": 1, "S": 2, "B": 3}
+
+
+def _gebal_job(job) -> int:
+    if isinstance(job, str):
+        if job.upper() not in GEBAL_JOBS:
+            raise ValueError(f"gebal job {job!r}: one of N, P, S, B")
+        return GEBAL_JOBS[job.upper()]
+    return int(job)
+
+
+def gebal_buffer_size(n: int) -> int:
+    """clm_gebal_buffer_size: the int workspace gebal needs (gebal_bufferSize)"""
+    out = ctypes.c_int()
+    check(lib().clm_gebal_buffer_size(int(n), ctypes.byref(out)), "clm_gebal_buffer_size")
+    return int(out.value)
+
+
+def gebal_cm(job, n: int, dA: torch.Tensor, lda: int, dScale: torch.Tensor, dWork: torch.Tensor,
+             max_sweeps: int = 0, stream: Optional[torch.cuda.Stream] = None) -> tuple:
+    """clm_gebal_{s,d,c,z} (src/lapack/gebal/interface.cppm:128): A <- D^-1 P^T A P D in place;
+    (status, ilo, ihi), both 1-based"""
+    ilo, ihi = ctypes.c_int(), ctypes.c_int()
+    f = getattr(lib(), f"clm_gebal_{_suffix(dA.dtype)}")
+    st = f(ctypes.c_void_p(_stream_ptr(stream)), _gebal_job(job), int(n), _ptr(dA), int(lda),
+           ctypes.byref(ilo), ctypes.byref(ihi), _ptr(dScale), _ptr(dWork), int(max_sweeps))
+    return int(st), int(ilo.value), int(ihi.value)
+
+
+def gebal(a: torch.Tensor, job="B", max_sweeps: int = 0,
+          stream: Optional[torch.cuda.Stream] = None) -> tuple:
+    """(B, ilo, ihi, scale) with B = D^-1 P^T a P D for a square device matrix"""
+    _require_device(a, "a")
+    n = a.shape[-1]
+    if a.dim() != 2 or a.shape[0] != n:
+        raise ValueError("gebal expects a square matrix")
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        scale = torch.empty(max(n, 1), dtype=_real_dtype(a.dtype), device=a.device)
+        work = torch.empty(max(gebal_buffer_size(n), 1), dtype=torch.int32, device=a.device)
+        st, ilo, ihi = gebal_cm(job, n, acm, n, scale, work, max_sweeps, stream)
+        check(st, "gebal")
+        b = hand_over(acm).t()
+        scale = hand_over(scale)
+    return b, ilo, ihi, scale[:n]
+
+
+def expm_balanced_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    out = ctypes.c_size_t()
+    check(getattr(lib(), f"clm_expm_balanced_buffer_size_{_suffix(dtype)}")(int(n), ctypes.byref(out)),
+          "clm_expm_balanced_buffer_size")
+    return int(out.value)
+
+
+def expm_balanced_cm(n: int, dA: torch.Tensor, lda: int, dE: torch.Tensor, lde: int,
+                     dWork: torch.Tensor, dInfo: Optional[torch.Tensor] = None,
+                     stream: Optional[torch.cuda.Stream] = None, ctx: Optional[Context] = None) -> tuple:
+    """clm_expm_balanced_{s,d,c,z}: D exp(D^-1 A D) D^-1 (src/expm/README.md:100-122);
+    (status, plan dict of the balanced matrix)"""
+    plan = ExpmPlan()
+    f = getattr(lib(), f"clm_expm_balanced_{_suffix(dA.dtype)}")
+    st = f(ctx.handle if ctx else None, ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dA),
+           int(lda), _ptr(dE), int(lde), _ptr(dWork), dWork.numel() * dWork.element_size(),
+           _ptr(dInfo), ctypes.byref(plan))
+    return int(st), plan.as_dict()
+
+
+def expm_balanced(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
+                  return_plan: bool = False):
+    """exp(a) with gebal's scaling around the exponential: one squaring fewer for every
+    factor of two the balancing takes off the 1-norm"""
+    _require_device(a, "a")
+    n = a.shape[-1]
+    if a.dim() != 2 or a.shape[0] != n:
+        raise ValueError("expm_balanced expects a square matrix")
+    with _on_stream(stream) as hand_over:
+        acm = to_cm(a)
+        ecm = torch.empty_like(acm)
+        work = torch.empty(expm_balanced_buffer_size(n, a.dtype), dtype=torch.uint8, device=a.device)
+        info = torch.empty(2, dtype=torch.int32, device=a.device)
+        st, plan = expm_balanced_cm(n, acm, n, ecm, n, work, info, stream)
+        check(st, "expm_balanced")
+        e = hand_over(ecm).t()
+    return (e, plan) if return_plan else e
+
+
+def lacn2_buffer_size(n: int, dtype: torch.dtype = _F64) -> int:
+    return int(getattr(lib(), f"clm_lacn2_buffer_size_{_suffix(dtype)}")(int(n)))
+
+
+class Lacn2State:
+    """The host state xLACN2 carries between calls: est, kase, isave[3]"""
+
+    def __init__(self, dtype: torch.dtype = _F64):
+        self.est = (ctypes.c_float if _is_single(dtype) else ctypes.c_double)(0.0)
+        self.kase = ctypes.c_int(0)
+        self.isave = (ctypes.c_int * 3)(0, 0, 0)
+
+
+def lacn2_cm(n: int, dV: torch.Tensor, dX: torch.Tensor, dWork: torch.Tensor, state: Lacn2State,
+             stream: Optional[torch.cuda.Stream] = None) -> int:
+    """clm_lacn2_{s,d}: one reverse-communication step (src/lapack/lacn2/interface.cppm:95)"""
+    f = getattr(lib(), f"clm_lacn2_{_suffix(dX.dtype)}")
+    return int(f(ctypes.c_void_p(_stream_ptr(stream)), int(n), _ptr(dV), _ptr(dX), _ptr(dWork),
+                 dWork.numel() * dWork.element_size(), ctypes.byref(state.est),
+                 ctypes.byref(state.kase), state.isave))
+
+
+def norm1_estimate(matvec, rmatvec, n: int, dtype: torch.dtype = _F64,
+                   device: Optional[torch.device] = None) -> tuple:
+    """||A||_1 estimated by xLACN2 from x -> A x and x -> A^T x on device vectors (the
+    callables return the product, on the current stream); (est, v with ||v||_1 = est ||w||_1,
+    the sequence of kase values)"""
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    v = torch.empty(n, dtype=dtype, device=device)
+    x = torch.empty(n, dtype=dtype, device=device)
+    work = torch.empty(lacn2_buffer_size(n, dtype), dtype=torch.uint8, device=device)
+    state = Lacn2State(dtype)
+    kases = []
+    while True:
+        check(lacn2_cm(n, v, x, work, state), "lacn2")
+        if state.kase.value == 0:
+            break
+        kases.append(state.kase.value)
+        x.copy_(matvec(x) if state.kase.value == 1 else rmatvec(x))
+    return float(state.est.value), v, kases
+
+
 # ---- convenience layer (row-major torch matrices) ----------------------------------------------
 def expm(a: torch.Tensor, stream: Optional[torch.cuda.Stream] = None,
          return_plan: bool = False):
@@ -694,4 +942,9 @@ __all__ = [
     "gemm_cm", "ps_plan", "ps_cost", "ps_block_size", "ps_buffer_size", "ps_power_stride",
     "horner_buffer_size", "to_cm", "from_cm", "BatchPlan", "expm_batched", "batched_tuning",
     "set_batched_tuning", "batched_autotune", "ExpmBatchedFunction", "expm_batched_autograd",
+    "pade_buffer_size", "pade_cm", "expm_pade_buffer_size", "expm_pade_cm", "expm_pade",
+    "getrf_cm", "getrs_cm", "GEBAL_JOBS", "gebal_buffer_size", "gebal_cm", "gebal",
+    "expm_balanced_buffer_size", "expm_balanced_cm", "expm_balanced", "lacn2_buffer_size",
+    "Lacn2State", "lacn2_cm", "norm1_estimate", "FILL_MODES", "expm_herm_buffer_size",
+    "expm_herm_cm", "expm_herm",
 ]

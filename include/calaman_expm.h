@@ -174,6 +174,166 @@ clm_status_t clm_expm_z(clm_ctx *ctx, clm_stream_t stream, int n, const clm_doub
                         int lda, clm_double_complex *dE, int lde, void *dWork, size_t lwork_bytes,
                         int *dInfo, clm_expm_plan *plan);
 
+/* ---- the reference's own route: Pade approximant + LU --------------------------
+ * pade<T> (src/expm/pade.cppm:77-191): R = r_m(A) = q_m(A)^-1 p_m(A) for m in
+ * {3, 5, 7, 9, 13}, unscaled (accurate while ||A||_1 <= clm_pade_theta_*(m)):
+ * powers A^2 .. A^8 (A^6 and the nested form at m = 13), the even and odd halves
+ * of p_m in one pass, U = A W, then (V - U) R = (V + U) by clm_getrf_* /
+ * clm_getrs_*. Never synchronizes. dInfo: device int[2] = getrf info, getrs info
+ * (may be NULL). pade_bufferSize<T> (src/expm/buffer_size.cppm:36-48):
+ * clm_pade_buffer_size_*. */
+clm_status_t clm_pade_buffer_size_s(int m, int n, size_t *bytes);
+clm_status_t clm_pade_buffer_size_d(int m, int n, size_t *bytes);
+clm_status_t clm_pade_buffer_size_c(int m, int n, size_t *bytes);
+clm_status_t clm_pade_buffer_size_z(int m, int n, size_t *bytes);
+clm_status_t clm_pade_s(clm_ctx *ctx, clm_stream_t stream, int m, int n, const float *dA, int lda,
+                        float *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo);
+clm_status_t clm_pade_d(clm_ctx *ctx, clm_stream_t stream, int m, int n, const double *dA, int lda,
+                        double *dR, int ldr, void *dWork, size_t lwork_bytes, int *dInfo);
+clm_status_t clm_pade_c(clm_ctx *ctx, clm_stream_t stream, int m, int n,
+                        const clm_float_complex *dA, int lda, clm_float_complex *dR, int ldr,
+                        void *dWork, size_t lwork_bytes, int *dInfo);
+clm_status_t clm_pade_z(clm_ctx *ctx, clm_stream_t stream, int m, int n,
+                        const clm_double_complex *dA, int lda, clm_double_complex *dR, int ldr,
+                        void *dWork, size_t lwork_bytes, int *dInfo);
+/* expm<T> exactly as the reference evaluates it (src/expm/expm.cppm:87-174): the
+ * 1-norm (one synchronization), the Pade plan (clm_pade_plan_*), 2^-s A, r_m,
+ * s squarings; dInfo (may be NULL) as clm_pade_*; plan->method =
+ * CLM_METHOD_PADE. expm_bufferSize<T>: clm_expm_pade_buffer_size_*. */
+clm_status_t clm_expm_pade_buffer_size_s(int n, size_t *bytes);
+clm_status_t clm_expm_pade_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_pade_buffer_size_c(int n, size_t *bytes);
+clm_status_t clm_expm_pade_buffer_size_z(int n, size_t *bytes);
+clm_status_t clm_expm_pade_s(clm_ctx *ctx, clm_stream_t stream, int n, const float *dA, int lda,
+                             float *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan);
+clm_status_t clm_expm_pade_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                             double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                             clm_expm_plan *plan);
+clm_status_t clm_expm_pade_c(clm_ctx *ctx, clm_stream_t stream, int n,
+                             const clm_float_complex *dA, int lda, clm_float_complex *dE, int lde,
+                             void *dWork, size_t lwork_bytes, int *dInfo, clm_expm_plan *plan);
+clm_status_t clm_expm_pade_z(clm_ctx *ctx, clm_stream_t stream, int n,
+                             const clm_double_complex *dA, int lda, clm_double_complex *dE, int lde,
+                             void *dWork, size_t lwork_bytes, int *dInfo, clm_expm_plan *plan);
+/* The LU the reference takes from the vendor solver (getrf/getrs,
+ * src/expm/pade.cppm:186-188): A = P L U in place with partial pivoting (ipiv
+ * 1-based; *dInfo = the first zero pivot's column, 1-based, or 0), then
+ * B <- A^-1 B for nrhs right-hand sides (dInfo may be NULL; written 0). Square
+ * matrices, n <= 46340. No workspace. */
+clm_status_t clm_getrf_s(clm_ctx *ctx, clm_stream_t stream, int n, float *dA, int lda, int *dIpiv,
+                         int *dInfo);
+clm_status_t clm_getrf_d(clm_ctx *ctx, clm_stream_t stream, int n, double *dA, int lda, int *dIpiv,
+                         int *dInfo);
+clm_status_t clm_getrf_c(clm_ctx *ctx, clm_stream_t stream, int n, clm_float_complex *dA, int lda,
+                         int *dIpiv, int *dInfo);
+clm_status_t clm_getrf_z(clm_ctx *ctx, clm_stream_t stream, int n, clm_double_complex *dA, int lda,
+                         int *dIpiv, int *dInfo);
+clm_status_t clm_getrs_s(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const float *dLU,
+                         int lda, const int *dIpiv, float *dB, int ldb, int *dInfo);
+clm_status_t clm_getrs_d(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs, const double *dLU,
+                         int lda, const int *dIpiv, double *dB, int ldb, int *dInfo);
+clm_status_t clm_getrs_c(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs,
+                         const clm_float_complex *dLU, int lda, const int *dIpiv,
+                         clm_float_complex *dB, int ldb, int *dInfo);
+clm_status_t clm_getrs_z(clm_ctx *ctx, clm_stream_t stream, int n, int nrhs,
+                         const clm_double_complex *dLU, int lda, const int *dIpiv,
+                         clm_double_complex *dB, int ldb, int *dInfo);
+
+/* ---- exp(A) for a symmetric / Hermitian A by its eigendecomposition --------------------
+ * calaman::expm_herm<T> (src/expm/expm_herm.cppm:145-199): only the uplo triangle of A
+ * is read (the other assumed to mirror it; the diagonal's imaginary part ignored),
+ * A = U diag(w) U^H, exp(A) = U diag(exp(w)) U^H. The reference takes the
+ * eigendecomposition from the vendor syevd / heevd; this library's is a blocked
+ * two-sided Jacobi method on the device (off-diagonal Frobenius norm driven below
+ * u ||A||_F). uplo: CLM_FILL_MODE_LOWER / CLM_FILL_MODE_UPPER (the BLAS fill-mode
+ * values). dInfo (device int, may be NULL): 0, or 1 when the Jacobi sweeps did not
+ * converge within 30 (the result is still written). Synchronizes once per sweep.
+ * Bad arguments: CLM_STATUS_INVALID_VALUE; a short workspace: CLM_STATUS_ALLOC_FAILED.
+ * expm_herm_bufferSize<T> (:97-109): clm_expm_herm_buffer_size_* (three blocks of
+ * order n rounded up to 64, where the reference holds two and the solver's scratch). */
+#define CLM_FILL_MODE_LOWER 0
+#define CLM_FILL_MODE_UPPER 1
+clm_status_t clm_expm_herm_buffer_size_s(int n, size_t *bytes);
+clm_status_t clm_expm_herm_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_herm_buffer_size_c(int n, size_t *bytes);
+clm_status_t clm_expm_herm_buffer_size_z(int n, size_t *bytes);
+clm_status_t clm_expm_herm_s(clm_ctx *ctx, clm_stream_t stream, int uplo, int n, const float *dA,
+                             int lda, float *dE, int lde, void *dWork, size_t lwork_bytes,
+                             int *dInfo);
+clm_status_t clm_expm_herm_d(clm_ctx *ctx, clm_stream_t stream, int uplo, int n, const double *dA,
+                             int lda, double *dE, int lde, void *dWork, size_t lwork_bytes,
+                             int *dInfo);
+clm_status_t clm_expm_herm_c(clm_ctx *ctx, clm_stream_t stream, int uplo, int n,
+                             const clm_float_complex *dA, int lda, clm_float_complex *dE, int lde,
+                             void *dWork, size_t lwork_bytes, int *dInfo);
+clm_status_t clm_expm_herm_z(clm_ctx *ctx, clm_stream_t stream, int uplo, int n,
+                             const clm_double_complex *dA, int lda, clm_double_complex *dE, int lde,
+                             void *dWork, size_t lwork_bytes, int *dInfo);
+
+/* ---- balancing (?gebal), the balanced exponential, the 1-norm estimator (?lacn2) -----
+ * calaman::gebal<T> (src/lapack/gebal/interface.cppm:128-262): A overwritten with
+ * B = D^-1 P^T A P D as LAPACK's xGEBAL through 3.4 does it -- the permutation
+ * isolating eigenvalues, then Parlett-Reinsch sweeps of powers of two (exact, so
+ * undoing it gives A back bit for bit). job: CLM_GEBAL_* (GebalJob, :76). ilo,
+ * ihi: host outputs, 1-based. dScale: device, n reals (real also for complex
+ * T): the diagonal of D inside [ilo, ihi], the 1-based exchange index outside.
+ * dWork: device ints, clm_gebal_buffer_size (gebal_bufferSize, :89-92). max_sweeps
+ * <= 0: the reference's default, 1000. Synchronizes the stream (every step branches
+ * on device data) unless job is CLM_GEBAL_NONE. Bad arguments:
+ * CLM_STATUS_RUNTIME_INVALID_VALUE; a failed launch or copy: CLM_STATUS_RUNTIME_ERROR. */
+#define CLM_GEBAL_NONE 0
+#define CLM_GEBAL_PERMUTE 1
+#define CLM_GEBAL_SCALE 2
+#define CLM_GEBAL_BOTH 3
+clm_status_t clm_gebal_buffer_size(int n, int *lwork);
+clm_status_t clm_gebal_s(clm_stream_t stream, int job, int n, float *dA, int lda, int *ilo,
+                         int *ihi, float *dScale, int *dWork, int max_sweeps);
+clm_status_t clm_gebal_d(clm_stream_t stream, int job, int n, double *dA, int lda, int *ilo,
+                         int *ihi, double *dScale, int *dWork, int max_sweeps);
+clm_status_t clm_gebal_c(clm_stream_t stream, int job, int n, clm_float_complex *dA, int lda,
+                         int *ilo, int *ihi, float *dScale, int *dWork, int max_sweeps);
+clm_status_t clm_gebal_z(clm_stream_t stream, int job, int n, clm_double_complex *dA, int lda,
+                         int *ilo, int *ihi, double *dScale, int *dWork, int max_sweeps);
+/* The balancing recipe of src/expm/README.md:100-122 as one call: B = D^-1 A D by
+ * gebal's scaling half (no permutation), exp(B) by clm_expm_*, then
+ * exp(A) = D exp(B) D^-1 (D of powers of two: the last step is exact). A is not
+ * modified. Workspace: clm_expm_balanced_buffer_size_* (B, D, the gebal ints and
+ * clm_expm_*'s own). Synchronizes (the sweeps, the 1-norm, the norms of powers). */
+clm_status_t clm_expm_balanced_buffer_size_s(int n, size_t *bytes);
+clm_status_t clm_expm_balanced_buffer_size_d(int n, size_t *bytes);
+clm_status_t clm_expm_balanced_buffer_size_c(int n, size_t *bytes);
+clm_status_t clm_expm_balanced_buffer_size_z(int n, size_t *bytes);
+clm_status_t clm_expm_balanced_s(clm_ctx *ctx, clm_stream_t stream, int n, const float *dA, int lda,
+                                 float *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                                 clm_expm_plan *plan);
+clm_status_t clm_expm_balanced_d(clm_ctx *ctx, clm_stream_t stream, int n, const double *dA, int lda,
+                                 double *dE, int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                                 clm_expm_plan *plan);
+clm_status_t clm_expm_balanced_c(clm_ctx *ctx, clm_stream_t stream, int n,
+                                 const clm_float_complex *dA, int lda, clm_float_complex *dE,
+                                 int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                                 clm_expm_plan *plan);
+clm_status_t clm_expm_balanced_z(clm_ctx *ctx, clm_stream_t stream, int n,
+                                 const clm_double_complex *dA, int lda, clm_double_complex *dE,
+                                 int lde, void *dWork, size_t lwork_bytes, int *dInfo,
+                                 clm_expm_plan *plan);
+/* calaman::lacn2<T> (src/lapack/lacn2/interface.cppm:73-206): one reverse-
+ * communication step of LAPACK's xLACN2 (Hager's method, Higham's refinements).
+ * Start with *kase = 0; on a return with *kase 1 (2), overwrite dX with A x (A^T x)
+ * and call again with everything else unchanged; *kase 0 on return: done, *est <=
+ * ||A||_1 and dV = A w with est = ||v||_1 / ||w||_1. est, kase, isave[3]: host state
+ * carried between calls (isave 1-based, as the reference's). dWork: device, at least
+ * clm_lacn2_buffer_size_* bytes, kept across calls. The reference binds a BLAS handle
+ * for its stream; this takes the stream. Bad arguments or state:
+ * CLM_STATUS_INVALID_VALUE. Synchronizes on every step but the first. */
+size_t clm_lacn2_buffer_size_s(int n);
+size_t clm_lacn2_buffer_size_d(int n);
+clm_status_t clm_lacn2_s(clm_stream_t stream, int n, float *dV, float *dX, void *dWork,
+                         size_t work_bytes, float *est, int *kase, int *isave);
+clm_status_t clm_lacn2_d(clm_stream_t stream, int n, double *dV, double *dX, void *dWork,
+                         size_t work_bytes, double *est, int *kase, int *isave);
+
 /* clm_expm_d without a workspace argument, for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). No counterpart in the reference. The workspace is a file-scope

@@ -237,6 +237,18 @@ def lu_solve(A, B):
     return X, int(info)
 
 
+def getrf(A):
+    """The oracle's partial-pivot LU (?getrf semantics): (LU, ipiv 1-based, info)"""
+    c = _is_complex(A)
+    dt = np.complex128 if c else np.float64
+    LU = _fortran(A, dt).copy(order="F")
+    n = LU.shape[0]
+    ipiv = (ctypes.c_int * max(n, 1))()
+    f = lib().orc_getrf_z if c else lib().orc_getrf_d
+    info = f(n, _ptr(LU), n, ipiv)
+    return LU, np.array(ipiv[:n], dtype=np.int64), int(info)
+
+
 # ---- Pade / expm ------------------------------------------------------------------
 def pade(m: int, A):
     c = _is_complex(A)

@@ -1,7 +1,8 @@
 // Host layer of libcalaman_expm under AddressSanitizer and UndefinedBehaviorSanitizer.
 //
-// Built from the library's own sources compiled host-only (no device code, no GPU
-// needed) by `make -C calaman_g_amd/csrc host-sanitize`, and run by
+// Built from the library's own sources with the sanitizers on their host half (the
+// device code is never launched; no GPU needed) by
+// `make -C calaman_g_amd/csrc host-sanitize`, and run by
 // tests/test_sanitizers.py. It drives every host path that returns before a
 // device call: the plan rules (Taylor ladder from the 1-norm and from the norms
 // of powers, the split schedule, the Pade ladder), the workspace sizes, the
@@ -89,6 +90,15 @@ int main() {
     for (size_t cb : {size_t(0), size_t(1) << 20})
       (void)clm_cg_unitary_buffer_size_z(n, cb, &bytes);
   }
+  for (int m = 0; m <= 14; ++m)
+    for (int n : {-1, 0, 1, 7, 2048, 46341}) {
+      (void)clm_pade_buffer_size_d(m, n, &bytes);
+      (void)clm_pade_buffer_size_z(m, n, &bytes);
+      (void)clm_pade_buffer_size_s(m, n, &bytes);
+      (void)clm_pade_buffer_size_c(m, n, &bytes);
+      (void)clm_expm_pade_buffer_size_d(n, &bytes);
+      (void)clm_expm_pade_buffer_size_c(n, &bytes);
+    }
   expect(clm_expm_buffer_size_d(8, nullptr) != CLM_STATUS_SUCCESS, "null size pointer");
 
   // argument checks that return before any device call
@@ -124,6 +134,43 @@ int main() {
   expect(clm_gemm_split_d(nullptr, 4, 4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4) ==
              CLM_STATUS_INVALID_VALUE,
          "split gemm bad split");
+
+  expect(clm_pade_d(nullptr, nullptr, 11, 4, p, 4, p, 4, p, 1 << 20, nullptr) ==
+             CLM_STATUS_INVALID_VALUE,
+         "pade bad degree");
+  expect(clm_pade_d(nullptr, nullptr, 13, 4, p, 4, p, 4, p, 64, nullptr) == CLM_STATUS_ALLOC_FAILED,
+         "pade short workspace");
+  expect(clm_expm_pade_d(nullptr, nullptr, 4, p, 2, p, 4, p, 1 << 20, nullptr, nullptr) ==
+             CLM_STATUS_INVALID_VALUE,
+         "expm_pade lda < n");
+  expect(clm_getrf_d(nullptr, nullptr, 4, p, 4, nullptr, nullptr) == CLM_STATUS_INVALID_VALUE,
+         "getrf null info");
+  expect(clm_getrs_d(nullptr, nullptr, 4, 1, p, 4, nullptr, p, 4, nullptr) ==
+             CLM_STATUS_INVALID_VALUE,
+         "getrs null ipiv");
+
+  int lw = 0, ilo = 0, ihi = 0;
+  expect(clm_gebal_buffer_size(7, &lw) == CLM_STATUS_SUCCESS && lw == 15, "gebal lwork");
+  expect(clm_gebal_d(nullptr, 9, 4, p, 4, &ilo, &ihi, p, reinterpret_cast<int *>(p), 0) ==
+             CLM_STATUS_RUNTIME_INVALID_VALUE,
+         "gebal bad job");
+  expect(clm_gebal_d(nullptr, CLM_GEBAL_BOTH, 0, p, 1, &ilo, &ihi, p, nullptr, 0) ==
+                 CLM_STATUS_SUCCESS &&
+             ilo == 1 && ihi == 0,
+         "gebal n = 0");
+  for (int n : {-1, 0, 1, 2048, 46341}) {
+    (void)clm_expm_balanced_buffer_size_d(n, &bytes);
+    (void)clm_expm_balanced_buffer_size_c(n, &bytes);
+    (void)clm_lacn2_buffer_size_d(n);
+  }
+  expect(clm_expm_balanced_z(nullptr, nullptr, 4, reinterpret_cast<clm_double_complex *>(p), 4,
+                             reinterpret_cast<clm_double_complex *>(p), 4, p, 16, nullptr,
+                             nullptr) == CLM_STATUS_ALLOC_FAILED,
+         "expm_balanced short workspace");
+  double est = 0.0;
+  int kase = 1, isave[3] = {7, 0, 0};
+  expect(clm_lacn2_d(nullptr, 8, p, p, p, 1 << 10, &est, &kase, isave) == CLM_STATUS_INVALID_VALUE,
+         "lacn2 bad state");
 
   clm_batch_plan *bp = nullptr;
   clm_batch_entry e{0, CLM_DTYPE_F64, p, 1, p, 1};

@@ -359,3 +359,75 @@ def test_split_power_plan_rules():
     small, large = _split_plan_powers(64, norms), _split_plan_powers(65, norms)
     assert small.ps_split == _lib.SPLIT_BF16X6 and large.ps_split == _lib.SPLIT_FP16X3
     assert small.s <= 5 and large.s <= 5
+
+
+def test_pade_route_sizes_and_argument_checks():
+    """clm_pade_buffer_size_*: the powers the degree needs (1, 2, 3, 4, and 3 plus the
+    nested X at m = 13) and V, W; expm's covers the worst degree plus A / 2^s;
+    argument checks return before any device work"""
+    out = ctypes.c_size_t()
+    for sfx, size in (("d", 8), ("z", 16), ("s", 4), ("c", 8)):
+        f = getattr(L, f"clm_pade_buffer_size_{sfx}")
+        n = 100
+        blocks = {}
+        for m, nb in ((3, 3), (5, 4), (7, 5), (9, 6), (13, 6)):
+            assert f(m, n, ctypes.byref(out)) == 0
+            blocks[m] = out.value
+            assert out.value >= nb * n * n * size
+        assert blocks[3] < blocks[5] < blocks[7] < blocks[9]
+        assert f(4, n, ctypes.byref(out)) == ST.INVALID_VALUE
+        assert f(13, 0, ctypes.byref(out)) == ST.INVALID_VALUE
+        assert getattr(L, f"clm_expm_pade_buffer_size_{sfx}")(n, ctypes.byref(out)) == 0
+        assert out.value >= max(blocks.values()) + n * n * size
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert L.clm_pade_d(None, None, 11, 4, p, 4, p, 4, p, 1 << 20, None) == ST.INVALID_VALUE
+    assert L.clm_pade_d(None, None, 3, 4, p, 3, p, 4, p, 1 << 20, None) == ST.INVALID_VALUE
+    assert L.clm_pade_d(None, None, 3, 4, p, 4, p, 4, p, 16, None) == ST.ALLOC_FAILED
+    assert L.clm_expm_pade_d(None, None, 0, p, 1, p, 1, p, 1 << 20, None, None) == ST.INVALID_VALUE
+    assert L.clm_expm_pade_d(None, None, 4, p, 4, p, 4, p, 16, None, None) == ST.ALLOC_FAILED
+    assert L.clm_getrf_d(None, None, -1, p, 1, p, p) == ST.INVALID_VALUE
+    assert L.clm_getrf_d(None, None, 4, p, 3, p, p) == ST.INVALID_VALUE
+    assert L.clm_getrf_d(None, None, 4, p, 4, p, None) == ST.INVALID_VALUE
+    assert L.clm_getrs_d(None, None, 4, 2, p, 4, p, p, 3, None) == ST.INVALID_VALUE
+
+
+def test_balancing_and_lacn2_sizes_and_argument_checks():
+    """gebal_bufferSize is 2n + 1 ints; the balanced expm's workspace holds B, D, the
+    gebal ints and clm_expm_*'s own; lacn2's holds the sign record, a flag and the
+    reductions. Every check below returns before any device work."""
+    lw = ctypes.c_int()
+    assert L.clm_gebal_buffer_size(5, ctypes.byref(lw)) == 0 and lw.value == 11
+    assert L.clm_gebal_buffer_size(-1, ctypes.byref(lw)) == ST.RUNTIME_INVALID_VALUE
+    out, inner = ctypes.c_size_t(), ctypes.c_size_t()
+    for sfx, size in (("d", 8), ("z", 16), ("s", 4), ("c", 8)):
+        assert getattr(L, f"clm_expm_balanced_buffer_size_{sfx}")(100, ctypes.byref(out)) == 0
+        assert getattr(L, f"clm_expm_buffer_size_{sfx}")(100, ctypes.byref(inner)) == 0
+        assert out.value >= inner.value + 100 * 100 * size + 100 * (size // (1 + (sfx in "zc")))
+        assert getattr(L, f"clm_expm_balanced_buffer_size_{sfx}")(0, ctypes.byref(out)) == \
+            ST.INVALID_VALUE
+    assert L.clm_lacn2_buffer_size_d(1000) >= 1000 * 4 + 4 + 32
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    ilo, ihi = ctypes.c_int(), ctypes.c_int()
+    for job in (-1, 4):
+        assert L.clm_gebal_d(None, job, 4, p, 4, ctypes.byref(ilo), ctypes.byref(ihi), p, p, 0) == \
+            ST.RUNTIME_INVALID_VALUE
+    assert L.clm_gebal_d(None, 3, 4, p, 3, ctypes.byref(ilo), ctypes.byref(ihi), p, p, 0) == \
+        ST.RUNTIME_INVALID_VALUE
+    assert L.clm_gebal_z(None, 3, 4, p, 4, ctypes.byref(ilo), ctypes.byref(ihi), p, None, 0) == \
+        ST.RUNTIME_INVALID_VALUE
+    # n = 0: ilo = 1, ihi = 0 without a launch
+    assert L.clm_gebal_d(None, 3, 0, p, 1, ctypes.byref(ilo), ctypes.byref(ihi), p, p, 0) == 0
+    assert (ilo.value, ihi.value) == (1, 0)
+    assert L.clm_expm_balanced_d(None, None, 4, p, 3, p, 4, p, 1 << 20, None, None) == \
+        ST.INVALID_VALUE
+    assert L.clm_expm_balanced_d(None, None, 4, p, 4, p, 4, p, 16, None, None) == ST.ALLOC_FAILED
+    est, kase, isave = ctypes.c_double(), ctypes.c_int(0), (ctypes.c_int * 3)(0, 0, 0)
+    assert L.clm_lacn2_d(None, 0, p, p, p, 1 << 10, ctypes.byref(est), ctypes.byref(kase), isave) == \
+        ST.INVALID_VALUE
+    assert L.clm_lacn2_d(None, 8, p, p, p, 8, ctypes.byref(est), ctypes.byref(kase), isave) == \
+        ST.INVALID_VALUE
+    kase.value, isave[0] = 1, 9  # a state no step leaves behind
+    assert L.clm_lacn2_d(None, 8, p, p, p, 1 << 10, ctypes.byref(est), ctypes.byref(kase), isave) == \
+        ST.INVALID_VALUE

@@ -9,7 +9,7 @@
 //     n x n block the result keeps, so the block is exp(A) exactly as before);
 //   * H is cut into 2k blocks of 32; a sweep pairs every block with every other once
 //     over 2k - 1 rounds (the circle method). A round is two launches: one workgroup
-//     per pair diagonalizes its 64 x 64 subproblem in LDS by cyclic Jacobi sweeps
+//     per pair runs one cyclic Jacobi sweep over its 64 x 64 subproblem in LDS
 //     (32 disjoint rotations at a time, circle order again), keeping the unitary Q;
 //     then one launch applies Q_t^H H_tu Q_u to every off-diagonal tile (t < u, the
 //     mirror written as its adjoint) and V <- V Q_u to the eigenvector tiles, skipping
@@ -31,8 +31,11 @@ namespace {
 constexpr int kB = 32;             // block width
 constexpr int kP = 2 * kB;         // a pair of blocks: the subproblem order
 constexpr int kLd = kP + 1;        // LDS row pitch
-constexpr int kThreads = 256;      // one workgroup per subproblem or tile
-constexpr int kInnerSweeps = 10;   // cap on the Jacobi sweeps of one subproblem
+constexpr int kThreads = 256;      // one workgroup per tile (a lane a 4 x 4 block)
+constexpr int kSolveThreads = 1024;  // one workgroup per subproblem (32 lanes a rotation)
+constexpr int kInnerSweeps = 1;    // Jacobi sweeps of one subproblem per round: more
+                                   // sweeps save no outer sweep at n >= 128 and every
+                                   // rotation adds its rounding to V
 constexpr int kMaxSweeps = 30;     // cap on the sweeps over the whole matrix
 
 __device__ inline double conj_(double x) { return x; }
@@ -44,17 +47,37 @@ __device__ inline float re_(float x) { return x; }
 __device__ inline double re_(zd x) { return x.re; }
 __device__ inline float re_(zc x) { return x.re; }
 __device__ inline double abs_(double x) { return fabs(x); }
-__device__ inline float abs_(float x) { return fabsf(x); }
 __device__ inline double abs_(zd x) { return hypot(x.re, x.im); }
-__device__ inline float abs_(zc x) { return hypotf(x.re, x.im); }
 __device__ inline double abs2_(double x) { return x * x; }
-__device__ inline float abs2_(float x) { return x * x; }
 __device__ inline double abs2_(zd x) { return x.re * x.re + x.im * x.im; }
-__device__ inline float abs2_(zc x) { return x.re * x.re + x.im * x.im; }
 __device__ inline double exp_(double x) { return exp(x); }
-__device__ inline float exp_(float x) { return expf(x); }
-__device__ inline double hyp_(double a, double b) { return hypot(a, b); }
-__device__ inline float hyp_(float a, float b) { return hypotf(a, b); }
+// The working type: the Jacobi sweeps of the single-precision types run in double
+// (their rotations' rounding would otherwise reach 1e-5 of exp(A) at n = 128)
+template <typename T>
+struct work_type {
+  using type = T;
+};
+template <>
+struct work_type<float> {
+  using type = double;
+};
+template <>
+struct work_type<zc> {
+  using type = zd;
+};
+template <typename T>
+using work_t = typename work_type<T>::type;
+__device__ inline double widen(double x) { return x; }
+__device__ inline double widen(float x) { return x; }
+__device__ inline zd widen(zd x) { return x; }
+__device__ inline zd widen(zc x) { return zd{x.re, x.im}; }
+template <typename T>
+__device__ inline T narrow(work_t<T> x);  // defined for the single-precision types
+template <>
+__device__ inline float narrow<float>(double x) { return static_cast<float>(x); }
+template <>
+__device__ inline zc narrow<zc>(zd x) { return zc{static_cast<float>(x.re), static_cast<float>(x.im)}; }
+
 template <typename T>
 __device__ inline T sub_(T a, T b) {
   return ops<T>::add(a, ops<T>::scale(b, -1.0));
@@ -76,20 +99,29 @@ __device__ inline void pair_of(int r, int t, int np2, int &p, int &q) {
   }
 }
 
-// H = the uplo triangle of A mirrored (real diagonal), zero-padded to np x np
-template <typename T>
+// H = the uplo triangle of A mirrored (real diagonal), zero-padded to np x np, in the
+// working type W
+template <typename T, typename W>
 __global__ void __launch_bounds__(kThreads)
-    mirror_kernel(int n, int np, int upper, const T *__restrict__ A, long lda, T *__restrict__ H) {
+    mirror_kernel(int n, int np, int upper, const T *__restrict__ A, long lda, W *__restrict__ H) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= np) return;
-  T v = ops<T>::zero();
+  W v = ops<W>::zero();
   if (i < n && j < n) {
-    if (i == j) v = ops<T>::from_real(re_(A[i + static_cast<long>(i) * lda]));
-    else if ((i < j) == (upper != 0)) v = A[i + static_cast<long>(j) * lda];
-    else v = conj_(A[j + static_cast<long>(i) * lda]);
+    if (i == j) v = ops<W>::from_real(re_(A[i + static_cast<long>(i) * lda]));
+    else if ((i < j) == (upper != 0)) v = widen(A[i + static_cast<long>(j) * lda]);
+    else v = widen(conj_(A[j + static_cast<long>(i) * lda]));
   }
   H[i + static_cast<long>(j) * np] = v;
+}
+// E = the n x n result rounded to the caller's type
+template <typename T, typename W>
+__global__ void __launch_bounds__(kThreads)
+    narrow_kernel(int n, const W *__restrict__ X, T *__restrict__ E, long lde) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) E[i + static_cast<long>(j) * lde] = narrow<T>(X[i + static_cast<long>(j) * n]);
 }
 
 template <typename T>
@@ -139,98 +171,91 @@ __global__ void clear_count_kernel(unsigned *c) {
 // local index l of a 64-wide pair (p, q) -> global index
 __device__ inline int glob(int l, int p, int q) { return l < kB ? p * kB + l : q * kB + (l - kB); }
 
-// The subproblem of pair t in round r: S = H[{p,q},{p,q}] diagonalized in LDS,
-// S <- Q^H S Q by rotations J = [[c, e s], [-conj(e) s, c]] (e = S_ij / |S_ij|,
-// t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = (S_jj - S_ii) / (2 |S_ij|),
-// c = 1 / sqrt(1 + t^2), s = t c: the real symmetric Jacobi rotation conjugated by
-// the phase of S_ij). Writes S back and Q to Qbuf[t] when anything rotated.
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    solve_kernel(int np, T *H, const real_t<T> *delta, int round, T *Qbuf, int *flags,
+// The subproblem of pair t in round r: S = H[{p,q},{p,q}] in LDS, one cyclic sweep of
+// rotations J = [[c, e s], [-conj(e) s, c]] (e = S_ij / |S_ij|, t = sign(tau) /
+// (|tau| + sqrt(1 + tau^2)), tau = (S_jj - S_ii) / (2 |S_ij|), c = 1 / sqrt(1 + t^2),
+// s = t c: the real symmetric Jacobi rotation conjugated by the phase of S_ij),
+// S <- J^H S J and Q <- Q J. 32 lanes serve one rotation of a round: each computes
+// the rotation itself from the same three entries (no broadcast), updates two of
+// the pair's 64 columns of rows i, j, then -- after every row update -- two rows of
+// columns i, j of S and of Q. Two barriers a round. Writes S back and Q to Qbuf[t]
+// when anything rotated.
+template <typename W>
+__global__ void __launch_bounds__(kSolveThreads)
+    solve_kernel(int np, W *H, const real_t<W> *delta, int round, W *Qbuf, int *flags,
                  unsigned *count) {
-  using R = real_t<T>;
-  __shared__ T S[kP][kLd];
-  __shared__ T Q[kP][kLd];
-  __shared__ R rc[kP / 2], rs[kP / 2], rdi[kP / 2], rdj[kP / 2];
-  __shared__ T rph[kP / 2];
-  __shared__ int ron[kP / 2];
+  using R = real_t<W>;
+  __shared__ W S[kP][kLd];
+  __shared__ W Q[kP][kLd];
   __shared__ int sweep_rot;
   const int np2 = np / kB;
   const int t = blockIdx.x;
   int p, q;
   pair_of(round, t, np2, p, q);
   const long ld = np;
-  for (int e = threadIdx.x; e < kP * kP; e += kThreads) {
+  for (int e = threadIdx.x; e < kP * kP; e += kSolveThreads) {
     const int r = e % kP, c = e / kP;
     S[r][c] = H[glob(r, p, q) + glob(c, p, q) * ld];
-    Q[r][c] = ops<T>::from_real(r == c ? 1.0 : 0.0);
+    Q[r][c] = ops<W>::from_real(r == c ? 1.0 : 0.0);
   }
+  const int m = threadIdx.x >> 5;  // the rotation this lane serves (32 lanes each)
+  const int sl = threadIdx.x & 31;
   const R d = *delta;
   bool any = false;
   for (int sw = 0; sw < kInnerSweeps; ++sw) {
     if (threadIdx.x == 0) sweep_rot = 0;
     __syncthreads();
     for (int r = 0; r < kP - 1; ++r) {
-      if (threadIdx.x < kP / 2) {
-        const int m = threadIdx.x;
-        int a, b;
-        pair_of(r, m, kP, a, b);
-        const int i = min(a, b), j = max(a, b);
-        const T c = S[i][j];
-        const R absc = abs_(c);
-        const int on = absc > d;  // false for NaN
-        if (on) {
-          const R aii = re_(S[i][i]), ajj = re_(S[j][j]);
-          const R tau = (ajj - aii) / (R(2) * absc);
-          const R tt = (tau >= R(0) ? R(1) : R(-1)) / (fabs(tau) + hyp_(R(1), tau));
-          const R cs = R(1) / sqrt(R(1) + tt * tt);
-          rc[m] = cs;
-          rs[m] = tt * cs;
-          rph[m] = ops<T>::scale(c, R(1) / absc);
-          rdi[m] = aii - tt * absc;
-          rdj[m] = ajj + tt * absc;
-          atomicAdd(&sweep_rot, 1);
+      int a, b;
+      pair_of(r, m, kP, a, b);
+      const int i = min(a, b), j = max(a, b);
+      // rows i and j are this pair's alone until the barrier: the three entries are
+      // read by all 32 lanes before any of them writes
+      const W c = S[i][j];
+      const R absc = abs_(c);
+      const bool on = absc > d;  // false for NaN
+      R cs = R(1), di = R(0), dj = R(0);
+      W es = ops<W>::zero();
+      if (on) {
+        const R aii = re_(S[i][i]), ajj = re_(S[j][j]);
+        const R tau = (ajj - aii) / (R(2) * absc);
+        const R at = fabs(tau);
+        const R tt = at > R(1e150) ? R(0.5) / tau
+                                   : (tau >= R(0) ? R(1) : R(-1)) / (at + sqrt(R(1) + tau * tau));
+        cs = R(1) / sqrt(R(1) + tt * tt);
+        es = ops<W>::scale(c, tt * cs / absc);
+        di = aii - tt * absc;
+        dj = ajj + tt * absc;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // rows i, j: S <- J^H S
+          const int col = sl + 32 * h;
+          const W xi = S[i][col], xj = S[j][col];
+          S[i][col] = sub_(ops<W>::scale(xi, cs), ops<W>::mul(es, xj));
+          S[j][col] = mac_(ops<W>::scale(xj, cs), conj_(es), xi);
         }
-        ron[m] = on;
+        if (sl == 0) atomicAdd(&sweep_rot, 1);
       }
       __syncthreads();
-      // rows i, j: S <- J^H S (a wave a pair, a lane a column)
-      for (int e = threadIdx.x; e < (kP / 2) * kP; e += kThreads) {
-        const int m = e / kP, col = e % kP;
-        if (!ron[m]) continue;
-        int a, b;
-        pair_of(r, m, kP, a, b);
-        const int i = min(a, b), j = max(a, b);
-        const T xi = S[i][col], xj = S[j][col];
-        const T es = ops<T>::scale(rph[m], rs[m]);
-        S[i][col] = sub_(ops<T>::scale(xi, rc[m]), ops<T>::mul(es, xj));
-        S[j][col] = mac_(ops<T>::scale(xj, rc[m]), conj_(es), xi);
-      }
-      __syncthreads();
-      // columns i, j: S <- S J, Q <- Q J; the pair's 2 x 2 block set to its values
-      for (int e = threadIdx.x; e < (kP / 2) * kP; e += kThreads) {
-        const int m = e / kP, row = e % kP;
-        if (!ron[m]) continue;
-        int a, b;
-        pair_of(r, m, kP, a, b);
-        const int i = min(a, b), j = max(a, b);
-        const R cs = rc[m];
-        const T es = ops<T>::scale(rph[m], rs[m]);
-        const T xi = S[row][i], xj = S[row][j];
-        T ni = sub_(ops<T>::scale(xi, cs), ops<T>::mul(conj_(es), xj));
-        T nj = mac_(ops<T>::scale(xj, cs), es, xi);
-        if (row == i) {
-          ni = ops<T>::from_real(rdi[m]);
-          nj = ops<T>::zero();
-        } else if (row == j) {
-          ni = ops<T>::zero();
-          nj = ops<T>::from_real(rdj[m]);
+      if (on) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // columns i, j: S <- S J, Q <- Q J
+          const int row = sl + 32 * h;
+          const W xi = S[row][i], xj = S[row][j];
+          W ni = sub_(ops<W>::scale(xi, cs), ops<W>::mul(conj_(es), xj));
+          W nj = mac_(ops<W>::scale(xj, cs), es, xi);
+          if (row == i) {
+            ni = ops<W>::from_real(di);
+            nj = ops<W>::zero();
+          } else if (row == j) {
+            ni = ops<W>::zero();
+            nj = ops<W>::from_real(dj);
+          }
+          S[row][i] = ni;
+          S[row][j] = nj;
+          const W qi = Q[row][i], qj = Q[row][j];
+          Q[row][i] = sub_(ops<W>::scale(qi, cs), ops<W>::mul(conj_(es), qj));
+          Q[row][j] = mac_(ops<W>::scale(qj, cs), es, qi);
         }
-        S[row][i] = ni;
-        S[row][j] = nj;
-        const T qi = Q[row][i], qj = Q[row][j];
-        Q[row][i] = sub_(ops<T>::scale(qi, cs), ops<T>::mul(conj_(es), qj));
-        Q[row][j] = mac_(ops<T>::scale(qj, cs), es, qi);
       }
       __syncthreads();
     }
@@ -240,8 +265,8 @@ __global__ void __launch_bounds__(kThreads)
     any = true;
   }
   if (any) {
-    T *Qt = Qbuf + static_cast<long>(t) * kP * kP;
-    for (int e = threadIdx.x; e < kP * kP; e += kThreads) {
+    W *Qt = Qbuf + static_cast<long>(t) * kP * kP;
+    for (int e = threadIdx.x; e < kP * kP; e += kSolveThreads) {
       const int r = e % kP, c = e / kP;
       H[glob(r, p, q) + glob(c, p, q) * ld] = S[r][c];
       Qt[r + c * kP] = Q[r][c];
@@ -406,14 +431,9 @@ struct HermWs {
 };
 
 template <typename T>
-constexpr double unit_roundoff() {
-  return std::is_same_v<real_t<T>, float> ? 5.9604644775390625e-08 : 1.1102230246251565e-16;
-}
-
-template <typename T>
 clm_status_t herm_bytes(int n, size_t *bytes) {
   if (bytes == nullptr || n < 1 || n > kMaxDim) return CLM_STATUS_INVALID_VALUE;
-  HermWs<T> ws;
+  HermWs<work_t<T>> ws;
   *bytes = ws.carve(nullptr, n);
   return CLM_STATUS_SUCCESS;
 }
@@ -421,10 +441,12 @@ clm_status_t herm_bytes(int n, size_t *bytes) {
 template <typename T>
 clm_status_t expm_herm_impl(clm_ctx *ctx, hipStream_t st, int uplo, int n, const T *A, int lda,
                             T *E, int lde, void *work, size_t bytes, int *info) {
+  using W = work_t<T>;
+  using R = real_t<W>;
   if (n < 1 || n > kMaxDim || lda < n || lde < n || A == nullptr || E == nullptr ||
       work == nullptr || (uplo != CLM_FILL_MODE_LOWER && uplo != CLM_FILL_MODE_UPPER))
     return CLM_STATUS_INVALID_VALUE;
-  HermWs<T> ws;
+  HermWs<W> ws;
   if (bytes < ws.carve(nullptr, n)) return CLM_STATUS_ALLOC_FAILED;
   clm_ctx *c = nullptr;
   CLM_TRY(resolve_ctx(ctx, &c));
@@ -432,19 +454,20 @@ clm_status_t expm_herm_impl(clm_ctx *ctx, hipStream_t st, int uplo, int n, const
   const int np = (n + kP - 1) / kP * kP;
   const int np2 = np / kB, k = np2 / 2;
   const dim3 grid_sq((np + kThreads - 1) / kThreads, np);
-  mirror_kernel<T><<<grid_sq, kThreads, 0, st>>>(n, np, uplo == CLM_FILL_MODE_UPPER, A, lda, ws.H);
-  identity_kernel<T><<<grid_sq, kThreads, 0, st>>>(np, ws.V);
-  colsq_kernel<T><<<np, kThreads, 0, st>>>(np, ws.H, ws.colsq);
-  delta_kernel<T><<<1, kThreads, 0, st>>>(np, ws.colsq, static_cast<real_t<T>>(unit_roundoff<T>()),
-                                          ws.delta);
+  mirror_kernel<T, W><<<grid_sq, kThreads, 0, st>>>(n, np, uplo == CLM_FILL_MODE_UPPER, A, lda,
+                                                    ws.H);
+  identity_kernel<W><<<grid_sq, kThreads, 0, st>>>(np, ws.V);
+  colsq_kernel<W><<<np, kThreads, 0, st>>>(np, ws.H, ws.colsq);
+  // the unit roundoff of the working precision, 2^-53
+  delta_kernel<W><<<1, kThreads, 0, st>>>(np, ws.colsq, R(1.1102230246251565e-16), ws.delta);
   CLM_TRY_HIP(hipGetLastError());
   bool converged = false;
   const int jobs = k * (k - 1) / 2 + k * k;
   for (int sweep = 0; sweep < kMaxSweeps && !converged; ++sweep) {
     clear_count_kernel<<<1, 64, 0, st>>>(ws.count);
     for (int r = 0; r < np2 - 1; ++r) {
-      solve_kernel<T><<<k, kThreads, 0, st>>>(np, ws.H, ws.delta, r, ws.Q, ws.flags, ws.count);
-      update_kernel<T><<<jobs, kThreads, 0, st>>>(np, ws.H, ws.V, r, ws.Q, ws.flags);
+      solve_kernel<W><<<k, kSolveThreads, 0, st>>>(np, ws.H, ws.delta, r, ws.Q, ws.flags, ws.count);
+      update_kernel<W><<<jobs, kThreads, 0, st>>>(np, ws.H, ws.V, r, ws.Q, ws.flags);
     }
     CLM_TRY_HIP(hipGetLastError());
     unsigned rotated = 0;
@@ -456,28 +479,41 @@ clm_status_t expm_herm_impl(clm_ctx *ctx, hipStream_t st, int uplo, int n, const
     set_info_kernel<<<1, 64, 0, st>>>(info, converged ? 0 : 1);
     CLM_TRY_HIP(hipGetLastError());
   }
-  // M = V diag(exp(w)) (rows < n) over W's storage first, then W = V(0:n,:)^H in H's
-  T *M = ws.W;
-  exp_scale_kernel<T><<<dim3((n + kThreads - 1) / kThreads, np), kThreads, 0, st>>>(n, np, ws.H,
+  // M = V diag(exp(w)) (rows < n) over W's storage first, then V(0:n,:)^H over H's (the
+  // eigenvalues have been read by then), then exp(A) = M V^H: into E directly, or, for
+  // the single-precision types, into V's storage and rounded into E
+  W *M = ws.W;
+  exp_scale_kernel<W><<<dim3((n + kThreads - 1) / kThreads, np), kThreads, 0, st>>>(n, np, ws.H,
                                                                                    ws.V, M);
   CLM_TRY_HIP(hipGetLastError());
-  T *Wt = ws.H;  // the eigenvalues have been read
-  adjoint_rows_kernel<T><<<dim3((np + kThreads - 1) / kThreads, n), kThreads, 0, st>>>(n, np, ws.V,
-                                                                                       Wt);
+  W *Vh = ws.H;
+  adjoint_rows_kernel<W><<<dim3((np + kThreads - 1) / kThreads, n), kThreads, 0, st>>>(n, np, ws.V,
+                                                                                       Vh);
   CLM_TRY_HIP(hipGetLastError());
-  GemmArgs<T> g;
+  constexpr bool kSame = std::is_same_v<T, W>;
+  GemmArgs<W> g;
   g.M = n;
   g.N = n;
   g.K = np;
   g.A = M;
   g.lda = n;
-  g.B = Wt;
+  g.B = Vh;
   g.ldb = np;
-  g.C = E;
-  g.ldc = lde;
-  g.alpha = ops<T>::from_real(1.0);
-  g.beta = ops<T>::zero();
-  CLM_TRY_HIP(launch_gemm<T>(st, g));
+  if constexpr (kSame) {
+    g.C = E;
+    g.ldc = lde;
+  } else {
+    g.C = ws.V;
+    g.ldc = n;
+  }
+  g.alpha = ops<W>::from_real(1.0);
+  g.beta = ops<W>::zero();
+  CLM_TRY_HIP(launch_gemm<W>(st, g));
+  if constexpr (!kSame) {
+    narrow_kernel<T, W><<<dim3((n + kThreads - 1) / kThreads, n), kThreads, 0, st>>>(n, ws.V, E,
+                                                                                     lde);
+    CLM_TRY_HIP(hipGetLastError());
+  }
   return CLM_STATUS_SUCCESS;
 }
 

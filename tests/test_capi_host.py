@@ -431,3 +431,25 @@ def test_balancing_and_lacn2_sizes_and_argument_checks():
     kase.value, isave[0] = 1, 9  # a state no step leaves behind
     assert L.clm_lacn2_d(None, 8, p, p, p, 1 << 10, ctypes.byref(est), ctypes.byref(kase), isave) == \
         ST.INVALID_VALUE
+
+
+def test_expm_herm_sizes_and_argument_checks():
+    """test/expm/expm_tests.cpp:780-800: n < 1, lda < n, lde < n and a null workspace are
+    INVALID_VALUE, so is a size query at n < 1; a bad fill mode too; a short workspace
+    ALLOC_FAILED. The workspace holds three blocks of order n rounded up to 64."""
+    out = ctypes.c_size_t()
+    for sfx, size in (("d", 8), ("z", 16), ("s", 4), ("c", 8)):
+        f = getattr(L, f"clm_expm_herm_buffer_size_{sfx}")
+        assert f(0, ctypes.byref(out)) == ST.INVALID_VALUE
+        assert f(100, ctypes.byref(out)) == 0
+        assert out.value >= 3 * 128 * 128 * size
+        assert f(1, ctypes.byref(out)) == 0 and out.value >= 3 * 64 * 64 * size
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    call = L.clm_expm_herm_d
+    assert call(None, None, 1, 0, p, 4, p, 4, p, 1 << 30, None) == ST.INVALID_VALUE
+    assert call(None, None, 1, 4, p, 3, p, 4, p, 1 << 30, None) == ST.INVALID_VALUE
+    assert call(None, None, 1, 4, p, 4, p, 3, p, 1 << 30, None) == ST.INVALID_VALUE
+    assert call(None, None, 1, 4, p, 4, p, 4, None, 1 << 30, None) == ST.INVALID_VALUE
+    assert call(None, None, 2, 4, p, 4, p, 4, p, 1 << 30, None) == ST.INVALID_VALUE
+    assert call(None, None, 0, 4, p, 4, p, 4, p, 64, None) == ST.ALLOC_FAILED

@@ -177,9 +177,12 @@ def gemm_roofline(dev, n=N, reps=20):
 
 def traffic_probe(dev, n=N, reps=5):
     """A calibration copy of exactly 256 MiB each way, then the GEMM launches whose
-    HBM bytes rocprofv3's FETCH_SIZE / WRITE_SIZE counters report per dispatch."""
+    HBM bytes rocprofv3's FETCH_SIZE / WRITE_SIZE counters report per dispatch, then two
+    split expms of the ||A||_1 = 8 workload matrix (their plane products carry the LDS
+    counters of the split variant)."""
+    import numpy as np
     import torch
-    from calaman_g_amd.expm import Context, gemm_cm
+    from calaman_g_amd.expm import Context, gemm_cm, expm_split
 
     src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
@@ -194,7 +197,12 @@ def traffic_probe(dev, n=N, reps=5):
     for _ in range(reps):
         gemm_cm(ctx, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
     torch.cuda.synchronize()
-    log(f"traffic probe done: copy 268435456 B each way, {reps} GEMM launches n={n}")
+    a8 = torch.from_numpy(np.ascontiguousarray(make_inputs(n, cplx=False)[1])).to(dev)
+    for _ in range(2):
+        expm_split(a8)
+    torch.cuda.synchronize()
+    log(f"traffic probe done: copy 268435456 B each way, {reps} GEMM launches n={n}, "
+        "2 split expms at ||A||_1 = 8")
 
 
 PEAK_FP32_TFLOPS = 157.3

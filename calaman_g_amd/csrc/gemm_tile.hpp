@@ -84,7 +84,12 @@ template <typename S, int BM, int BN, int NT, int BK>
 struct RTile {
   static constexpr int EV = Mfma<S>::EV;
   static constexpr int LDA_S = BM + 16;  // As[k][m]
-  static constexpr int LDB_S = BK + EV;  // Bs[n][k]
+  // Bs[n][k]: a row pitch of 2 (mod 32) LDS banks, BK + 1 doubles or BK + 2 floats, so
+  // the 16 rows one k-step reads (lane % 16) land in 16 distinct bank pairs; a pitch
+  // of BK + one 16-byte vector put rows n and n + 8 on the same banks (the hardware
+  // counted 17M conflict cycles per n = 2048 FP64 product, 3.2 per LDS instruction).
+  // The rows are no longer 16-byte aligned, so B is staged element by element.
+  static constexpr int LDB_S = BK + (sizeof(S) == 8 ? 1 : 2);
   static constexpr int A_V = BM * BK / EV / NT;  // 16-byte vectors per lane per slice
   static constexpr int B_V = BN * BK / EV / NT;
   static constexpr int A_STAGE = BK * LDA_S;
@@ -150,7 +155,8 @@ __device__ __forceinline__ void rgemm_sstore(
   for (int it = 0; it < TL::B_V; ++it) {
     const int idx = tid + it * NT;
     const int kk = (idx % (BK / EV)) * EV, nn = idx / (BK / EV);
-    *reinterpret_cast<V *>(&Bs[nn * TL::LDB_S + kk]) = rb[it];
+#pragma unroll
+    for (int e = 0; e < EV; ++e) Bs[nn * TL::LDB_S + kk + e] = reinterpret_cast<const S *>(&rb[it])[e];
   }
 }
 
@@ -195,7 +201,7 @@ template <typename C, int BM, int BN, int NT, int BK>
 struct CTile {
   using S = typename CxReal<C>::type;
   static constexpr int LDA_S = BM + 16;
-  static constexpr int LDB_S = BK + Mfma<S>::EV;
+  static constexpr int LDB_S = BK + (sizeof(S) == 8 ? 1 : 2);  // as RTile's: 2 banks (mod 32)
   static constexpr int A_E = BM * BK / NT;  // complex elements per lane per slice
   static constexpr int B_E = BN * BK / NT;
   static constexpr int A_PLANE = BK * LDA_S;

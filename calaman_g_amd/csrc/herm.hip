@@ -278,11 +278,13 @@ __global__ void __launch_bounds__(kSolveThreads)
   }
 }
 
-// acc = X * Y over 64 x 64 tiles in LDS (adj: X^H * Y); a lane a 4 x 4 block of
-// the result, rows 4 (lane % 16), columns 4 (lane / 16)
+// acc = X * Y over 64 x 64 tiles in LDS (adj: X^H * Y); a lane a 4 x 4 block of the
+// result: rows lane % 16 + 16 a (16 consecutive rows per k across a lane group, so
+// the X reads fall on distinct banks), columns 4 (lane / 16) + b (one Y element a
+// lane group: a broadcast)
 template <typename T, bool kAdjX>
 __device__ inline void tile_product(const T (*X)[kLd], const T (*Y)[kLd], T (&acc)[4][4]) {
-  const int r0 = 4 * (threadIdx.x % 16), c0 = 4 * (threadIdx.x / 16);
+  const int r0 = threadIdx.x % 16, c0 = 4 * (threadIdx.x / 16);
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -290,7 +292,7 @@ __device__ inline void tile_product(const T (*X)[kLd], const T (*Y)[kLd], T (&ac
   for (int k = 0; k < kP; ++k) {
     T x[4], y[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) x[a] = kAdjX ? conj_(X[k][r0 + a]) : X[r0 + a][k];
+    for (int a = 0; a < 4; ++a) x[a] = kAdjX ? conj_(X[k][r0 + 16 * a]) : X[r0 + 16 * a][k];
 #pragma unroll
     for (int b = 0; b < 4; ++b) y[b] = Y[k][c0 + b];
 #pragma unroll
@@ -301,11 +303,11 @@ __device__ inline void tile_product(const T (*X)[kLd], const T (*Y)[kLd], T (&ac
 }
 template <typename T>
 __device__ inline void tile_store(T (*X)[kLd], const T (&acc)[4][4]) {
-  const int r0 = 4 * (threadIdx.x % 16), c0 = 4 * (threadIdx.x / 16);
+  const int r0 = threadIdx.x % 16, c0 = 4 * (threadIdx.x / 16);
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) X[r0 + a][c0 + b] = acc[a][b];
+    for (int b = 0; b < 4; ++b) X[r0 + 16 * a][c0 + b] = acc[a][b];
 }
 
 // One launch per round: jobs [0, k(k-1)/2) are the tiles (t, u), t < u, of H:

@@ -420,9 +420,14 @@ struct SplitWs {
   PlaneSet PR, PCA, PCX;
   long pstride = 0;
   mutable int generation = 0;  // splits so far in this run (exponent tags)
+  // the generation of the input matrix's row exponents in PR.e and column exponents in
+  // PCA.e when they were found ahead of its first split (0: not, or no longer, there)
+  mutable int a_gen = 0;
+  const double *a_src = nullptr;
   // the run's first step when it uses the half format: every exponent to generation 0
   hipError_t start(hipStream_t st, int n) const {
     generation = 0;
+    a_gen = 0;
     return launch_clear_exponents(st, n, PR.e, PCA.e, PCX.e);
   }
   size_t carve(void *base, int n) {
@@ -447,11 +452,17 @@ size_t expm_split_buffer_size(int n) {
 clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
                           const PlaneSet *pc, const PlaneSet *pr, const SplitWs &sp) {
   if (split == CLM_SPLIT_FP16X3) {
-    const int gen = ++sp.generation;
+    // the input matrix's exponents, when found ahead and still in place (its first
+    // split, into PR and / or PCA)
+    const bool pre = sp.a_gen > 0 && X == sp.a_src && (!pr || pr == &sp.PR) &&
+                     (!pc || pc == &sp.PCA);
+    const int gen = pre ? sp.a_gen : ++sp.generation;
+    sp.a_gen = 0;
     if (pc) pc->gen = gen;
     if (pr) pr->gen = gen;
-    CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr, pc ? pc->e : nullptr,
-                                      gen));
+    if (!pre)
+      CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr,
+                                        pc ? pc->e : nullptr, gen));
     CLM_TRY_HIP(launch_split_planes_half(st, n, n, X, ldx, pc ? pc->p : nullptr,
                                          pr ? pr->p : nullptr, sp.pstride, pr ? pr->e : nullptr,
                                          pc ? pc->e : nullptr, gen, gen));
@@ -724,13 +735,20 @@ ReadbackRing *ring_of(int device) {
 // out[0 .. count-1] = the device values src[0 .. count-1] once the stream reaches
 // them. pinned = false takes a pageable copy and a stream synchronization and
 // allocates nothing.
-template <typename R>
+// `after` (when given) enqueues work that need not precede the values: it runs on the
+// device while the host waits for them.
+template <typename R, typename After = std::nullptr_t>
 clm_status_t read_back(hipStream_t st, int device, const R *src, int count, double *out,
-                       bool pinned) {
+                       bool pinned, After after = nullptr) {
+  auto launch_after = [&]() -> hipError_t {
+    if constexpr (std::is_same_v<After, std::nullptr_t>) return hipSuccess;
+    else return after();
+  };
   ReadbackRing *ring = pinned ? ring_of(device) : nullptr;
   if (!ring) {
     R host[kMaxNormPower] = {};
     CLM_TRY_HIP(hipMemcpyAsync(host, src, sizeof(R) * count, hipMemcpyDeviceToHost, st));
+    CLM_TRY_HIP(launch_after());
     CLM_TRY_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < count; ++i) out[i] = static_cast<double>(host[i]);
     return CLM_STATUS_SUCCESS;
@@ -740,6 +758,7 @@ clm_status_t read_back(hipStream_t st, int device, const R *src, int count, doub
   const int slot = static_cast<int>(ticket % kRingSlots);
   ReadbackSlot *h = ring->host + slot;
   CLM_TRY_HIP(launch_publish(st, src, count, ring->dev[slot].vals, &ring->dev[slot].seq, seq));
+  CLM_TRY_HIP(launch_after());
   const auto t0 = std::chrono::steady_clock::now();
   while (__atomic_load_n(&h->seq, __ATOMIC_ACQUIRE) != seq) {
     if (std::chrono::steady_clock::now() - t0 > kSpinLimit) {
@@ -793,10 +812,26 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
     if (variant == kFp8Tail) f8.carve(static_cast<char *>(work) + fp64_bytes + split_bytes, n);
   }
 
-  real_t<T> norm = 0;
-  CLM_TRY(read_norm1(st, n, dA, lda, ws.colsum, &norm, c->device, may_allocate));
+  // the 1-norm, read back; behind it, while the host waits, the split variants clear
+  // their exponent arrays and find A's row and column exponents for the half format,
+  // which A's first split then takes instead of a pass of its own (unused when the
+  // plan picks the bfloat16 format)
+  CLM_TRY_HIP(launch_norm1(st, n, dA, lda, ws.colsum));
+  double norm_v = 0;
+  auto ahead = [&]() -> hipError_t {
+    if (!use_split) return hipSuccess;
+    const hipError_t e = sp.start(st, n);
+    if (e != hipSuccess) return e;
+    if constexpr (std::is_same_v<T, double>) {
+      sp.a_gen = ++sp.generation;
+      sp.a_src = dA;
+      return launch_half_exponents(st, n, n, dA, lda, sp.PR.e, sp.PCA.e, sp.a_gen);
+    }
+    return hipSuccess;
+  };
+  CLM_TRY(read_back(st, c->device, ws.colsum, 1, &norm_v, may_allocate, ahead));
+  const real_t<T> norm = static_cast<real_t<T>>(norm_v);
   if (!std::isfinite(norm)) return CLM_STATUS_INVALID_VALUE;
-  if (use_split) CLM_TRY_HIP(sp.start(st, n));
 
   // The plan from the 1-norm alone (the degree ladder at the element type's unit
   // roundoff, 2^-53 or 2^-24) names the power bank; when that bank holds A^3 the

@@ -149,6 +149,19 @@ int main() {
              CLM_STATUS_INVALID_VALUE,
          "getrs null ipiv");
 
+  for (int n : {-1, 0, 1, 63, 64, 65, 2048, 46341}) {
+    (void)clm_expm_herm_buffer_size_d(n, &bytes);
+    (void)clm_expm_herm_buffer_size_c(n, &bytes);
+  }
+  expect(clm_expm_herm_d(nullptr, nullptr, 5, 4, p, 4, p, 4, p, 1 << 20, nullptr) ==
+             CLM_STATUS_INVALID_VALUE,
+         "expm_herm bad fill mode");
+  expect(clm_expm_herm_z(nullptr, nullptr, CLM_FILL_MODE_UPPER, 4,
+                         reinterpret_cast<clm_double_complex *>(p), 4,
+                         reinterpret_cast<clm_double_complex *>(p), 4, p, 64,
+                         nullptr) == CLM_STATUS_ALLOC_FAILED,
+         "expm_herm short workspace");
+
   int lw = 0, ilo = 0, ihi = 0;
   expect(clm_gebal_buffer_size(7, &lw) == CLM_STATUS_SUCCESS && lw == 15, "gebal lwork");
   expect(clm_gebal_d(nullptr, 9, 4, p, 4, &ilo, &ihi, p, reinterpret_cast<int *>(p), 0) ==

@@ -193,3 +193,35 @@ def test_lacn2_exact_on_nonnegative(dev, n):
     est32, _, kases32 = _drive(a, torch.float32, dev)
     assert kases32 == [1, 2, 1, 1]
     assert est32 == pytest.approx(a.sum(0).max(), rel=1e-5)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_gebal_non_finite_and_sweep_cap(dev, dtype):
+    """an index whose norms are not finite is skipped (src/lapack/gebal/README.md
+    "Differences"), exactly as the oracle skips it; max_sweeps caps the sweeps as the
+    oracle's cap does"""
+    n = 40
+    a = make("scaled", n, 77, CPLX[dtype])
+    a[3, 7] = np.inf
+    a[11, 2] = np.nan
+    real = torch.float64
+    for sweeps in (1, 2, 0):
+        da = cm(a, dtype, dev, n + 3)
+        scale = torch.empty(n, dtype=real, device=dev)
+        work = torch.zeros(cg.gebal_buffer_size(n), dtype=torch.int32, device=dev)
+        st, ilo, ihi = cg.gebal_cm("B", n, da, n + 3, scale, work, sweeps)
+        assert st == 0
+        torch.cuda.synchronize()
+        ob_b, o_ilo, o_ihi, o_d = ob.gebal(a, ob.BOTH, max_sweeps=sweeps if sweeps > 0 else 1000)
+        assert (ilo, ihi) == (o_ilo, o_ihi)
+        np.testing.assert_array_equal(scale.cpu().numpy(), o_d)
+        np.testing.assert_array_equal(from_cm(da, n), ob_b)
+
+
+def test_lacn2_single(dev):
+    n = 100
+    a = np.random.default_rng(5).standard_normal((n, n)).astype(np.float32).astype(np.float64)
+    est, v, kases = _drive(a, torch.float32, dev)
+    o_est, o_v, o_seq = ob.lacn2(a)
+    assert kases == o_seq
+    assert est == pytest.approx(o_est, rel=1e-5)

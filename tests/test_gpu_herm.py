@@ -135,3 +135,12 @@ def test_expm_herm_convenience(dev):
     t = torch.from_numpy(a).to(dev)
     e, info = cg.expm_herm(t, "U", return_info=True)
     assert info == 0 and rel(e.cpu().numpy(), oh.expm_herm(a)) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_expm_herm_n1000(dev, dtype):
+    a = hermitian(1000, 99, CPLX[dtype], scale=0.3)
+    e, info = run(a, dtype, dev, "L")
+    err = rel(e, oh.expm_herm(a, oh.LOWER))
+    print(f"  rel err {err:.3e}")
+    assert info == 0 and err <= 1e-12

@@ -432,6 +432,9 @@ clm_status_t lacn2_step(hipStream_t st, int n, T *v, T *x, void *work, size_t by
   if (n < 1 || v == nullptr || x == nullptr || work == nullptr || est == nullptr ||
       kase == nullptr || isave == nullptr || (*kase != 0 && (isave[0] < 1 || isave[0] > 5)))
     return CLM_STATUS_INVALID_VALUE;
+  // the column index the later steps carry (1-based) is read on the device: a state
+  // that was not left by a previous step is refused before it can address past x
+  if (*kase != 0 && isave[0] >= 3 && (isave[1] < 1 || isave[1] > n)) return CLM_STATUS_INVALID_VALUE;
   Lacn2Ws ws;
   if (bytes < ws.carve(work, n)) return CLM_STATUS_INVALID_VALUE;
   const int blocks = std::min((n + kBlk - 1) / kBlk, 1024);

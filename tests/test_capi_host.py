@@ -431,6 +431,9 @@ def test_balancing_and_lacn2_sizes_and_argument_checks():
     kase.value, isave[0] = 1, 9  # a state no step leaves behind
     assert L.clm_lacn2_d(None, 8, p, p, p, 1 << 10, ctypes.byref(est), ctypes.byref(kase), isave) == \
         ST.INVALID_VALUE
+    kase.value, isave[0], isave[1] = 2, 4, 9  # a column index past n
+    assert L.clm_lacn2_d(None, 8, p, p, p, 1 << 10, ctypes.byref(est), ctypes.byref(kase), isave) == \
+        ST.INVALID_VALUE
 
 
 def test_expm_herm_sizes_and_argument_checks():

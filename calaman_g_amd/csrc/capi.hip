@@ -413,22 +413,32 @@ clm_status_t ps_evaluate(hipStream_t st, int n, const T *coef, int degree, const
 //                              of the Paterson-Stockmeyer Horner steps)
 struct PlaneSet {
   unsigned short *p = nullptr;
-  int *e = nullptr;     // half format: per-row (row-major set) or per-column exponents,
-  mutable int gen = 0;  // tagged with the generation of the split that wrote them
+  mutable int *e = nullptr;  // half format: per-row (row-major set) or per-column exponents,
+  mutable int gen = 0;       // tagged with the generation of the split that wrote them
+};
+// exponents of a matrix found before its split: by a pass ahead of time (the input's
+// rows in PR.e and columns in PCA.e) or by the epilogue of the product that wrote it
+// (its rows, in the spare row array); gen = 0: none
+struct PreExponents {
+  const double *X = nullptr;
+  int *row_e = nullptr;
+  int *col_e = nullptr;
+  int gen = 0;
 };
 struct SplitWs {
   PlaneSet PR, PCA, PCX;
   long pstride = 0;
   mutable int generation = 0;  // splits so far in this run (exponent tags)
-  // the generation of the input matrix's row exponents in PR.e and column exponents in
-  // PCA.e when they were found ahead of its first split (0: not, or no longer, there)
-  mutable int a_gen = 0;
-  const double *a_src = nullptr;
+  // the spare row-exponent array: a product's epilogue writes its output's row
+  // exponents there while PR.e still serves the product's own left operand; the
+  // next split of that output swaps it with PR.e
+  mutable int *e_alt = nullptr;
+  mutable PreExponents pre;
   // the run's first step when it uses the half format: every exponent to generation 0
   hipError_t start(hipStream_t st, int n) const {
     generation = 0;
-    a_gen = 0;
-    return launch_clear_exponents(st, n, PR.e, PCA.e, PCX.e);
+    pre = PreExponents{};
+    return launch_clear_exponents(st, n, PR.e, PCA.e, PCX.e, e_alt);
   }
   size_t carve(void *base, int n) {
     Layout l(base);
@@ -437,6 +447,7 @@ struct SplitWs {
       ps->p = l.take<unsigned short>(3 * static_cast<size_t>(pstride));
       ps->e = l.take<int>(static_cast<size_t>(n));
     }
+    e_alt = l.take<int>(static_cast<size_t>(n));
     return l.total();
   }
 };
@@ -452,15 +463,22 @@ size_t expm_split_buffer_size(int n) {
 clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
                           const PlaneSet *pc, const PlaneSet *pr, const SplitWs &sp) {
   if (split == CLM_SPLIT_FP16X3) {
-    // the input matrix's exponents, when found ahead and still in place (its first
-    // split, into PR and / or PCA)
-    const bool pre = sp.a_gen > 0 && X == sp.a_src && (!pr || pr == &sp.PR) &&
-                     (!pc || pc == &sp.PCA);
-    const int gen = pre ? sp.a_gen : ++sp.generation;
-    sp.a_gen = 0;
+    // X's exponents, when found before (ahead of time or by the product that wrote X)
+    // for every side this split needs; any split of the half format retires them
+    const PreExponents q = sp.pre;
+    sp.pre = PreExponents{};
+    const bool reuse = q.gen > 0 && X == q.X && (!pr || (pr == &sp.PR && q.row_e)) &&
+                       (!pc || (pc == &sp.PCA && q.col_e == sp.PCA.e));
+    int gen;
+    if (reuse) {
+      gen = q.gen;
+      if (pr && q.row_e != sp.PR.e) std::swap(sp.PR.e, sp.e_alt);  // the epilogue's rows
+    } else {
+      gen = ++sp.generation;
+    }
     if (pc) pc->gen = gen;
     if (pr) pr->gen = gen;
-    if (!pre)
+    if (!reuse)
       CLM_TRY_HIP(launch_half_exponents(st, n, n, X, ldx, pr ? pr->e : nullptr,
                                         pc ? pc->e : nullptr, gen));
     CLM_TRY_HIP(launch_split_planes_half(st, n, n, X, ldx, pc ? pc->p : nullptr,
@@ -473,10 +491,16 @@ clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, lon
   return CLM_STATUS_SUCCESS;
 }
 
-// the split product C = PR-planes x PC-planes (+ epilogue in g)
+// the split product C = PR-planes x PC-planes (+ epilogue in g); with rows_next (half
+// format), the epilogue also finds C's row exponents for C's next split as a left operand
 clm_status_t gemm_planes(hipStream_t st, GemmArgs<double> g, int split, const PlaneSet &a,
-                         const PlaneSet &b, long pstride) {
+                         const PlaneSet &b, long pstride, const SplitWs *rows_next = nullptr) {
+  if (rows_next && split == CLM_SPLIT_FP16X3) {
+    g.row_e_out = rows_next->e_alt;
+    g.e_gen_out = ++rows_next->generation;
+  }
   CLM_TRY_HIP(launch_gemm_split_planes(st, g, split, a.p, b.p, pstride, a.e, b.e, a.gen, b.gen));
+  if (g.row_e_out) rows_next->pre = PreExponents{g.C, g.row_e_out, nullptr, g.e_gen_out};
   return CLM_STATUS_SUCCESS;
 }
 
@@ -506,7 +530,7 @@ clm_status_t bank_powers_split(hipStream_t st, int split, int n, const double *A
     }
     *pca_is_a = true;
     CLM_TRY(gemm_planes(st, plain_args(n, bank + static_cast<long>(k - 2) * stride, n), split,
-                        sp.PR, sp.PCA, sp.pstride));
+                        sp.PR, sp.PCA, sp.pstride, &sp));
   }
   return CLM_STATUS_SUCCESS;
 }
@@ -552,7 +576,7 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
     g.bank = bank;
     g.bank_stride = stride;
     g.ldbank = n;
-    CLM_TRY(gemm_planes(st, g, split, sp.PR, *pcx, sp.pstride));
+    CLM_TRY(gemm_planes(st, g, split, sp.PR, *pcx, sp.pstride, j > 0 ? &sp : nullptr));
     cur = next;
   }
   return CLM_STATUS_SUCCESS;
@@ -823,9 +847,8 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
     const hipError_t e = sp.start(st, n);
     if (e != hipSuccess) return e;
     if constexpr (std::is_same_v<T, double>) {
-      sp.a_gen = ++sp.generation;
-      sp.a_src = dA;
-      return launch_half_exponents(st, n, n, dA, lda, sp.PR.e, sp.PCA.e, sp.a_gen);
+      sp.pre = PreExponents{dA, sp.PR.e, sp.PCA.e, ++sp.generation};
+      return launch_half_exponents(st, n, n, dA, lda, sp.PR.e, sp.PCA.e, sp.pre.gen);
     }
     return hipSuccess;
   };

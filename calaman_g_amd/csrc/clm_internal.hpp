@@ -161,6 +161,11 @@ struct GemmArgs {
   const T *bank = nullptr;
   long bank_stride = 0;
   long ldbank = 0;
+  // the split products only: when set, the epilogue also leaves each output row's
+  // largest-magnitude exponent in row_e_out, tagged with generation e_gen_out (the
+  // half-format split's own tags), so the next split of C needs no exponent pass
+  int *row_e_out = nullptr;
+  int e_gen_out = 0;
 };
 
 hipError_t launch_gemm_d(hipStream_t stream, const GemmArgs<double> &g);
@@ -182,7 +187,8 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
 // IEEE-half planes (split = CLM_SPLIT_FP16X3). Exponent arrays hold values tagged
 // with the generation (>= 1) of the split that wrote them, so they are cleared
 // (to generation 0) once per run of splits, not before each: ...
-hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c);
+hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c,
+                                  int *d = nullptr);
 // ... the largest-magnitude exponent of every row (row_e) and column (col_e) of
 // X, as generation gen, either array may be null ...
 hipError_t launch_half_exponents(hipStream_t stream, int rows, int cols, const double *X, long ldx,

@@ -462,9 +462,6 @@ constexpr int kNoExponent = -(1 << 30);  // an all-zero row / column
 // over e within a generation, and an entry an earlier generation left reads as
 // an all-zero row's or column's -- so the arrays need one clearing per run of
 // splits, not one before every split.
-__device__ __forceinline__ int exp_tag(int gen, int e) {
-  return (gen << 12) | (min(max(e, -2047), 2047) + 2048);
-}
 __device__ __forceinline__ int exp_of(int tagged, int gen) {
   return (tagged >> 12) == gen ? (tagged & 4095) - 2048 : kNoExponent;
 }
@@ -507,7 +504,7 @@ __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, in
   cm[0] = fmax(cm[0], __shfl_xor(cm[0], 8, 64));
   if (col_e && (lane & 12) == 0) {
     const int j = tl.col(2 * (lane & 1) + ((lane >> 1) & 1));
-    if (j < cols && cm[0] > 0.0) atomicMax(&col_e[j], exp_tag(gen, ilogb(cm[0])));
+    if (j < cols && cm[0] > 0.0) atomicMax(&col_e[j], tile::exp_tag(gen, ilogb(cm[0])));
   }
   // rows: lanes l, l ^ 16, l ^ 32, l ^ 48 share rows; after bits 4, 5 a lane keeps
   // row 2 b4 + b5 of its four
@@ -528,7 +525,7 @@ __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, in
   if (row_e && wave == 0) {
     const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
     const int i = tl.i0 + lane;
-    if (i < rows && m > 0.0) atomicMax(&row_e[i], exp_tag(gen, ilogb(m)));
+    if (i < rows && m > 0.0) atomicMax(&row_e[i], tile::exp_tag(gen, ilogb(m)));
   }
 }
 
@@ -626,12 +623,13 @@ __global__ void __launch_bounds__(256)
                            st);
 }
 
-__global__ void clear_exponents_kernel(int *a, int *b, int *c, int n) {
+__global__ void clear_exponents_kernel(int *a, int *b, int *c, int *d, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     if (a) a[i] = 0;
     if (b) b[i] = 0;
     if (c) c[i] = 0;
+    if (d) d[i] = 0;
   }
 }
 // the 16x16x32 MFMA of a plane format: bf16 or (FMT = 1) IEEE half
@@ -796,14 +794,17 @@ template <int FMT, int NW, bool kFast>
 __device__ __forceinline__ void tile_epilogue(const GemmArgs<double> &g, const float *sum, int m0,
                                               int n0, const int *__restrict__ row_e,
                                               const int *__restrict__ col_e, int row_gen,
-                                              int col_gen) {
-  tile::lds_tile_epilogue<double, NW, kFast>(g, m0, n0, [&](int c, int r) {
-    double a = static_cast<double>(sum[c * kSumLd + r]);
-    if constexpr (FMT == 1)
-      a = ldexp(a, -(half_scale(exp_of(row_e[m0 + r], row_gen)) +
-                     half_scale(exp_of(col_e[n0 + c], col_gen))));
-    return a;
-  });
+                                              int col_gen, int *row_tags = nullptr) {
+  tile::lds_tile_epilogue<double, NW, kFast>(
+      g, m0, n0,
+      [&](int c, int r) {
+        double a = static_cast<double>(sum[c * kSumLd + r]);
+        if constexpr (FMT == 1)
+          a = ldexp(a, -(half_scale(exp_of(row_e[m0 + r], row_gen)) +
+                         half_scale(exp_of(col_e[n0 + c], col_gen))));
+        return a;
+      },
+      row_tags);
 }
 
 template <int SPLIT, int FMT, int NW, int KS, bool kFast>
@@ -980,7 +981,14 @@ __global__ void __launch_bounds__(512)
     sum_tile_put<4, 4>(sum, lane, wm0, wn0, acc);
   }
   __syncthreads();
-  tile_epilogue<FMT, 8, kFast>(g, sum, m0, n0, row_e, col_e, row_gen, col_gen);
+  // the output rows' exponents, when asked for, gather in the (now free) reduction buffer
+  int *row_tags = nullptr;
+  if (g.row_e_out) {
+    row_tags = reinterpret_cast<int *>(red);
+    if (tid < 128) row_tags[tid] = 0;
+    __syncthreads();
+  }
+  tile_epilogue<FMT, 8, kFast>(g, sum, m0, n0, row_e, col_e, row_gen, col_gen, row_tags);
 }
 
 inline bool aligned16s(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -1021,6 +1029,8 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
                                     long pstride, const int *row_e, const int *col_e, int row_gen,
                                     int col_gen) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  // the output rows' exponents come from the 3-product kernel's epilogue only
+  if (g.row_e_out && split == CLM_SPLIT_BF16X6) return hipErrorInvalidValue;
   const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
   // 3-product formats: 8 waves, 64-deep slices (two LDS stages of two planes);
   // 6 products: 8 waves, 32-deep slices (three planes)
@@ -1049,9 +1059,10 @@ hipError_t launch_gemm_split_planes(hipStream_t stream, const GemmArgs<double> &
   return hipGetLastError();
 }
 
-hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c) {
-  if (n <= 0 || (!a && !b && !c)) return hipSuccess;
-  hipLaunchKernelGGL(clear_exponents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, b, c, n);
+hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int *c, int *d) {
+  if (n <= 0 || (!a && !b && !c && !d)) return hipSuccess;
+  hipLaunchKernelGGL(clear_exponents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, b, c, d,
+                     n);
   return hipGetLastError();
 }
 

@@ -62,8 +62,15 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
   tn = r / gsize;
 }
 
+// tagged exponent of the half-format splits: (gen << 12) | (e + 2048), e clamped to
+// [-2047, 2047]; atomicMax over tags is the max over e within a generation
+__device__ __forceinline__ int exp_tag(int gen, int e) {
+  return (gen << 12) | (min(max(e, -2047), 2047) + 2048);
+}
+
+// the stored value is returned (the split products' exponent tracking reads it)
 template <typename T>
-__device__ __forceinline__ void epilogue_store(const GemmArgs<T> &g, int row, int col, T acc) {
+__device__ __forceinline__ T epilogue_store(const GemmArgs<T> &g, int row, int col, T acc) {
   T v = ops<T>::mul(g.alpha, acc);
   if (g.Cin) v = ops<T>::add(v, ops<T>::mul(g.beta, g.Cin[row + (long)col * g.ldcin]));
   if (g.nterms > 0) {
@@ -75,6 +82,7 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs<T> &g, int row, in
     v = ops<T>::add(v, e);
   }
   g.C[row + (long)col * g.ldc] = v;
+  return v;
 }
 
 // ------------------------------------------------------------------------------------
@@ -320,11 +328,17 @@ __device__ __forceinline__ bool epilogue_vec2_ok(const GemmArgs<S> &g) {
          (g.nterms < 3 || ((g.ldbank & 1) == 0 && (g.bank_stride & 1) == 0 && al(g.bank)));
 }
 
+// row_tags (FP64 only; LDS, 128 ints zeroed by the caller before a barrier): when given
+// and g.row_e_out is set, the largest |value| stored in each of the tile's rows is
+// folded into its tagged exponent (an LDS max over the waves, then one global max per
+// row) -- the split's exponent pass, done on the values as they are written.
 template <typename S, int NW, bool kFast, int MAXCH = 8, typename Fetch>
 __device__ __forceinline__ void lds_tile_epilogue(const GemmArgs<S> &g, int m0, int n0,
-                                                  Fetch fetch) {
+                                                  Fetch fetch, int *row_tags = nullptr) {
   using V2 = typename Vec2<S>::type;
   using O = ops<S>;
+  const bool track = row_tags != nullptr && g.row_e_out != nullptr;
+  S rmax0 = S(0), rmax1 = S(0);  // this lane's rows, over its columns
   constexpr int CPW = 128 / NW;                // columns per wave
   constexpr int CH = CPW < MAXCH ? CPW : MAXCH;  // columns per chunk of reads in flight
   static_assert(128 % NW == 0 && CPW % CH == 0, "column chunks");
@@ -393,13 +407,39 @@ __device__ __forceinline__ void lds_tile_epilogue(const GemmArgs<S> &g, int m0, 
 #pragma unroll
       for (int k = 0; k < CH; ++k)
         *const_cast<V2 *>(at(g.C, g.ldc, c0, k)) = v[k];
+      if constexpr (std::is_same_v<S, double>) {
+        if (track) {
+#pragma unroll
+          for (int k = 0; k < CH; ++k) {
+            rmax0 = fmax(rmax0, fabs(v[k].x));
+            rmax1 = fmax(rmax1, fabs(v[k].y));
+          }
+        }
+      }
     }
   } else {
     for (int k = 0; k < CPW; ++k) {
       const int c = wave + NW * k, col = n0 + c;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        if (kFast || (row + h < g.M && col < g.N)) epilogue_store(g, row + h, col, fetch(c, rl + h));
+        if (kFast || (row + h < g.M && col < g.N)) {
+          const S v = epilogue_store(g, row + h, col, fetch(c, rl + h));
+          if constexpr (std::is_same_v<S, double>) {
+            if (track) {
+              if (h == 0) rmax0 = fmax(rmax0, fabs(v));
+              else rmax1 = fmax(rmax1, fabs(v));
+            }
+          }
+        }
+    }
+  }
+  if constexpr (std::is_same_v<S, double>) {
+    if (track) {  // uniform: every lane reaches both barriers
+      if (rmax0 > 0.0) atomicMax(&row_tags[rl], exp_tag(g.e_gen_out, ilogb(rmax0)));
+      if (rmax1 > 0.0) atomicMax(&row_tags[rl + 1], exp_tag(g.e_gen_out, ilogb(rmax1)));
+      __syncthreads();
+      const int t = threadIdx.x;
+      if (t < 128 && m0 + t < g.M && row_tags[t] > 0) atomicMax(&g.row_e_out[m0 + t], row_tags[t]);
     }
   }
 }

@@ -542,9 +542,11 @@ clm_status_t bank_powers_split(hipStream_t st, int split, int n, const double *A
 clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int degree,
                                const double *A, long lda, double *P, long ldp, double *bank,
                                long stride, double *scratch, long ld_scratch, int s_requested,
-                               int split, const SplitWs &sp, int have = 0, double x_scale = 1.0) {
+                               int split, const SplitWs &sp, int have = 0, double x_scale = 1.0,
+                               bool pca_is_a = false) {
+  // pca_is_a: sp.PCA already holds A's column planes in this format (the powers formed
+  // before the plan), so neither they nor their exponents are made again
   const clm_ps_plan p = ps_plan(degree, s_requested);
-  bool pca_is_a = false;
   CLM_TRY(bank_powers_split(st, split, n, A, lda, bank, stride, have + 2, p.num_powers + 1, sp,
                             &pca_is_a));
   const PlaneSet *pcx = &sp.PCA;
@@ -962,9 +964,13 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
                                   p.ps_split, fp8_min_degree, sp, f8, &count, q - 1));
       p.fp8_gemms = fp8_count + count;
     } else if (use_split) {
+      // A's column planes from the powers formed before the plan still serve when the
+      // polynomial multiplies by A itself (the scaling folded, not applied to a copy) in
+      // the same format
+      const bool pca_valid = q > 1 && src == dA && p.ps_split == p0.ps_split;
       CLM_TRY(ps_evaluate_split(st, n, pcoef, p.m, src, ld_src, target, ld_target, ws.bank,
                                 static_cast<long>(ws.stride), other, ld_other, p.ps_block,
-                                p.ps_split, sp, q - 1, x_scale));
+                                p.ps_split, sp, q - 1, x_scale, pca_valid));
     } else {
       CLM_TRY(ps_evaluate<T>(st, n, pcoef, p.m, src, ld_src, target, ld_target, ws.bank,
                              static_cast<long>(ws.stride), other, ld_other, p.ps_block, q - 1,

@@ -877,14 +877,14 @@ __global__ void __launch_bounds__(NW * 64)
 // sums meet in LDS: group 1 writes its accumulators, group 0 adds them (even-step
 // sum + odd-step sum) and runs the epilogue.
 // ------------------------------------------------------------------------------------
-template <int SPLIT, int FMT>
-__device__ __forceinline__ void planes_kg_compute(const unsigned short *S, int lane, int wm0,
-                                                  int wn0, int ks, f4 (&acc)[4][4]) {
-  constexpr int NPL = SplitPlanes<SPLIT>::NPL;
+// a wave's fragments of one 32-deep k-step: rows wm0.. of the A planes, wn0.. of the B planes
+template <int NPL, int FMT>
+__device__ __forceinline__ void planes_kg_read(const unsigned short *S, int lane, int wm0, int wn0,
+                                               int ks, typename PlaneMfma<FMT>::vec (&a)[NPL][4],
+                                               typename PlaneMfma<FMT>::vec (&b)[NPL][4]) {
   using PT = PlaneTile<NPL, 2>;
   using V = typename PlaneMfma<FMT>::vec;
   const int koff = ks * 32 + 8 * (lane >> 4);
-  V a[NPL][4], b[NPL][4];
 #pragma unroll
   for (int t = 0; t < NPL; ++t)
 #pragma unroll
@@ -894,14 +894,31 @@ __device__ __forceinline__ void planes_kg_compute(const unsigned short *S, int l
       b[t][i] = *reinterpret_cast<const V *>(
           &S[(NPL + t) * PT::PLANE + PT::off(wn0 + i * 16 + (lane & 15), koff)]);
     }
+}
+
+// the first two products of a k-step (the small terms a1 b0 and a0 b1), every tile
+template <int FMT>
+__device__ __forceinline__ void planes_kg_mma_small(const typename PlaneMfma<FMT>::vec (&a)[2][4],
+                                                    const typename PlaneMfma<FMT>::vec (&b)[2][4],
+                                                    f4 (&acc)[4][4]) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       acc[i][j] = PlaneMfma<FMT>::mma(a[1][i], b[0][j], acc[i][j]);
       acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[1][j], acc[i][j]);
-      acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
     }
+}
+
+// the last product of a k-step (a0 b0), every tile
+template <int FMT>
+__device__ __forceinline__ void planes_kg_mma_lead(const typename PlaneMfma<FMT>::vec (&a)[2][4],
+                                                   const typename PlaneMfma<FMT>::vec (&b)[2][4],
+                                                   f4 (&acc)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = PlaneMfma<FMT>::mma(a[0][i], b[0][j], acc[i][j]);
 }
 
 template <int SPLIT, int FMT, bool kFast>
@@ -938,23 +955,51 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
+  // Software pipeline over the 64-deep slices: a wave holds slice kt's fragments in
+  // registers; it runs the two small products of every tile, stages slice kt + 1
+  // (loaded from memory one slice earlier) and issues the loads of slice kt + 2,
+  // meets the barrier, then issues its LDS reads of slice kt + 1 and runs the lead
+  // products of slice kt behind them, so the read latency hides under MFMAs. The
+  // stage written at slice kt held slice kt - 1, whose fragments every wave had in
+  // registers before the previous barrier (its small products read all of them).
+  // Every accumulator still takes a1 b0, a0 b1, a0 b0 of each k-step in k order.
+  using V = typename PlaneMfma<FMT>::vec;
   u32x4 ra[NPL][IT], rb[NPL][IT];
   const int KT = (g.K + PT::BK - 1) / PT::BK;
   if (KT > 0) {
     planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, 0, tid, ra, rb);
     planes2_sstore<NPL, KS, NT>(L, tid, ra, rb);
+    if (KT > 1) planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, PT::BK, tid, ra, rb);
     __syncthreads();
+    V a[NPL][4], b[NPL][4];
+    planes_kg_read<NPL, FMT>(L, lane, wm0, wn0, grp, a, b);
     for (int kt = 0; kt < KT - 1; ++kt) {
-      const int cur = kt & 1;
-      planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 1) * PT::BK, tid, ra, rb);
-      // the next slice's loads go out before this slice's MFMAs, whose time covers
-      // their latency (the scheduler would otherwise sink them next to the LDS stores)
-      __builtin_amdgcn_sched_barrier(0);
-      planes_kg_compute<SPLIT, FMT>(L + cur * PT::STAGE, lane, wm0, wn0, grp, acc);
-      planes2_sstore<NPL, KS, NT>(L + (cur ^ 1) * PT::STAGE, tid, ra, rb);
+      const int nxt = (kt + 1) & 1;
+      planes_kg_mma_small<FMT>(a, b, acc);
+      planes2_sstore<NPL, KS, NT>(L + nxt * PT::STAGE, tid, ra, rb);
+      // the LDS stores spread over the small products (four MFMAs, one store)
+#pragma unroll
+      for (int q = 0; q < 2 * NPL * IT; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      }
+      if (kt + 2 < KT)
+        planes2_gload<NPL, KS, NT, kFast>(g, PA, PB, pstride, m0, n0, (kt + 2) * PT::BK, tid, ra, rb);
       __syncthreads();
+      V na[NPL][4], nb[NPL][4];
+      planes_kg_read<NPL, FMT>(L + nxt * PT::STAGE, lane, wm0, wn0, grp, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+      planes_kg_mma_lead<FMT>(a, b, acc);
+#pragma unroll
+      for (int t = 0; t < NPL; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a[t][i] = na[t][i];
+          b[t][i] = nb[t][i];
+        }
     }
-    planes_kg_compute<SPLIT, FMT>(L + ((KT - 1) & 1) * PT::STAGE, lane, wm0, wn0, grp, acc);
+    planes_kg_mma_small<FMT>(a, b, acc);
+    planes_kg_mma_lead<FMT>(a, b, acc);
   }
 
   __syncthreads();  // every wave is done with the stages

@@ -199,17 +199,22 @@ __device__ __forceinline__ i8v lds_i8v(const unsigned char *S, int row, int k) {
   return v;
 }
 
-// one 128-deep k-tile: 4 x 4 scaled MFMAs per wave
-__device__ __forceinline__ void fp8w_compute(const unsigned char *S, int lane, int wm0, int wn0,
-                                             int sa, int sb, f4 (&acc)[4][4]) {
+// a wave's fragments of one 128-deep k-tile
+__device__ __forceinline__ void fp8w_read(const unsigned char *S, int lane, int wm0, int wn0,
+                                          i8v (&a)[4], i8v (&b)[4]) {
   const int koff = 32 * (lane >> 4);
-  i8v a[4], b[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) a[i] = lds_i8v(S, wm0 + i * 16 + (lane & 15), koff);
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = lds_i8v(S + kStageA, wn0 + j * 16 + (lane & 15), koff);
+}
+
+// the scaled MFMAs of rows I0 .. I1 - 1 (16-row sub-tiles) of a wave's tile, every column
+template <int I0, int I1>
+__device__ __forceinline__ void fp8w_mma(const i8v (&a)[4], const i8v (&b)[4], int sa, int sb,
+                                         f4 (&acc)[4][4]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = I0; i < I1; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0,
@@ -242,20 +247,49 @@ __global__ void __launch_bounds__(512)
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  // Software pipeline over the k-tiles: a wave holds k-tile kt's fragments in
+  // registers; it runs the MFMAs of its first two row sub-tiles, stages k-tile kt + 1
+  // (loaded one k-tile earlier, the stores spread over those MFMAs) and loads
+  // k-tile kt + 2, meets the barrier, then issues its LDS reads of k-tile kt + 1 and
+  // runs the last two row sub-tiles behind them. The stage written at k-tile kt held
+  // k-tile kt - 1, read into registers before the previous barrier (which waits for
+  // every LDS access of the wave). Each accumulator takes the k-tiles in order.
   u4v ra[4], rb[2];
   const int KT = tiles_k;
   if (KT > 0) {
     fp8w_gload<kFast>(g, PA, PB, m0, n0, 0, tid, ra, rb);
     fp8w_sstore(L, tid, ra, rb);
+    if (KT > 1) fp8w_gload<kFast>(g, PA, PB, m0, n0, kT, tid, ra, rb);
     __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-      const int cur = kt & 1;
-      if (kt + 1 < KT) fp8w_gload<kFast>(g, PA, PB, m0, n0, (kt + 1) * kT, tid, ra, rb);
+    i8v a[4], b[4];
+    fp8w_read(L, lane, wm0, wn0, a, b);
+    for (int kt = 0; kt < KT - 1; ++kt) {
+      const int nxt = (kt + 1) & 1;
       const int sa = e8m0_of(eA[qm + kt * qtiles_m]), sb = e8m0_of(eB[kt + tn * tiles_k]);
-      fp8w_compute(L + cur * kStageW, lane, wm0, wn0, sa, sb, acc);
-      if (kt + 1 < KT) fp8w_sstore(L + (cur ^ 1) * kStageW, tid, ra, rb);
+      fp8w_mma<0, 2>(a, b, sa, sb, acc);
+      fp8w_sstore(L + nxt * kStageW, tid, ra, rb);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      }
+      // the first half's results exist here: its MFMAs stay ahead of the barrier
+      asm volatile("" : "+v"(acc[0][0]), "+v"(acc[0][1]), "+v"(acc[0][2]), "+v"(acc[0][3]),
+                   "+v"(acc[1][0]), "+v"(acc[1][1]), "+v"(acc[1][2]), "+v"(acc[1][3]));
+      if (kt + 2 < KT) fp8w_gload<kFast>(g, PA, PB, m0, n0, (kt + 2) * kT, tid, ra, rb);
       __syncthreads();
+      i8v na[4], nb[4];
+      fp8w_read(L + nxt * kStageW, lane, wm0, wn0, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+      fp8w_mma<2, 4>(a, b, sa, sb, acc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = na[i];
+        b[i] = nb[i];
+      }
     }
+    const int sa = e8m0_of(eA[qm + (KT - 1) * qtiles_m]), sb = e8m0_of(eB[KT - 1 + tn * tiles_k]);
+    fp8w_mma<0, 4>(a, b, sa, sb, acc);
   }
   // the epilogue through LDS, one 128-row half of the tile at a time: the four waves
   // holding the half lay their sums down column-major, then every wave writes C (and

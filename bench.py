@@ -26,6 +26,10 @@ Also reported, on the same JSON line:
                 stream, against the 78.6 TFLOP/s FP64 matrix peak;
   cpu_baseline  the CPU oracle (the C restatement of the reference's Pade expm)
                 timed on this host's cores on one expm per norm.
+  breakdown.library_gemm_16bit
+                torch.mm (hipBLASLt) in FP16 and BF16 at 2048 x 2048 x 6144 (the
+                work of the split expm's three n = 2048 plane products) and at
+                2048^3, for scale against the plane product's trace durations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --traffic-probe   (for rocprofv3 --pmc: one 256 MiB device copy
@@ -108,10 +112,10 @@ def time_expms(mats, dev, steps, warmup, split=False):
     return max(wall, ev) / steps, [dict(p) for p in plans], results
 
 
-def per_norm_split(mats, dev, reps=5):
+def per_norm_split(mats, dev, reps=10, warm=3):
     """FP64 and split-BF16 expm time per matrix (one norm at a time): the speedup of
     the precision variant depends on s, so the per-norm ratios are reported beside
-    the whole-step one."""
+    the whole-step one. A few untimed calls first, so the clocks have ramped."""
     import numpy as np
     import torch
     import calaman_g_amd as cg
@@ -125,7 +129,8 @@ def per_norm_split(mats, dev, reps=5):
         de = torch.empty_like(da)
         row = {}
         for key, fn in (("fp64_ms", expm_cm), ("split_ms", expm_split_cm)):
-            fn(n, da, n, de, n, work)
+            for _ in range(warm):
+                fn(n, da, n, de, n, work)
             torch.cuda.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
@@ -203,6 +208,33 @@ def traffic_probe(dev, n=N, reps=5):
     torch.cuda.synchronize()
     log(f"traffic probe done: copy 268435456 B each way, {reps} GEMM launches n={n}, "
         "2 split expms at ||A||_1 = 8")
+
+
+def library_gemm_leg(dev, reps=50):
+    """The vendor library's 16-bit GEMM (torch.mm, hipBLASLt underneath) at the plane
+    product's shape, for scale: the split product's three n = 2048 plane products are
+    the work of one 2048 x 2048 x 6144 16-bit GEMM with FP32 sums (the library's output
+    here is 16-bit, a smaller write than the plane product's FP64 epilogue)."""
+    import torch
+    out = {}
+    for name, dt in (("f16", torch.float16), ("bf16", torch.bfloat16)):
+        for (m, n, k) in ((2048, 2048, 6144), (2048, 2048, 2048)):
+            a = torch.randn(m, k, device=dev).to(dt)
+            b = torch.randn(k, n, device=dev).to(dt)
+            for _ in range(20):
+                c = a @ b
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                c = a @ b
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / reps * 1e3
+            out[f"{name}_{m}x{n}x{k}"] = {"us": round(us, 1),
+                                          "tflops": round(2.0 * m * n * k / us / 1e6, 1)}
+            del a, b, c
+    return out
 
 
 PEAK_FP32_TFLOPS = 157.3
@@ -537,6 +569,8 @@ def main():
     adjoint = adjoint_leg(dev, max(3, args.steps // 4), 1)
     log(f"adjoint: {adjoint}")
     fp8tail = fp8tail_leg(dev, max(2, args.steps // 8), 1)
+    libgemm = library_gemm_leg(dev)
+    log(f"library 16-bit gemm: {libgemm}")
 
     cpu = None
     errs = None
@@ -584,7 +618,8 @@ def main():
             "complex128_ms_per_step": round(zsec * 1e3, 4),
             "batched": batched,
             "adjoint_fwd_bwd": adjoint,
-            "mixed_precision_n4096": fp8tail},
+            "mixed_precision_n4096": fp8tail,
+            "library_gemm_16bit": libgemm},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

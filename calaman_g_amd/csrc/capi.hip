@@ -434,10 +434,21 @@ struct SplitWs {
   // next split of that output swaps it with PR.e
   mutable int *e_alt = nullptr;
   mutable PreExponents pre;
+  // Planes split ahead of need, while the host waits on a read-back: PR (PCA) holds
+  // the planes of spec_pr (spec_pca) in format spec_*_fmt. The next split asked of that
+  // plane set either finds them there (and skips its pass) or replaces them; either way
+  // the note is spent. Anything that rewrites the matrix in place drops it (drop_spec).
+  mutable const double *spec_pr = nullptr, *spec_pca = nullptr;
+  mutable int spec_pr_fmt = 0, spec_pca_fmt = 0;
+  void drop_spec() const {
+    spec_pr = spec_pca = nullptr;
+    spec_pr_fmt = spec_pca_fmt = 0;
+  }
   // the run's first step when it uses the half format: every exponent to generation 0
   hipError_t start(hipStream_t st, int n) const {
     generation = 0;
     pre = PreExponents{};
+    drop_spec();
     return launch_clear_exponents(st, n, PR.e, PCA.e, PCX.e, e_alt);
   }
   size_t carve(void *base, int n) {
@@ -462,6 +473,16 @@ size_t expm_split_buffer_size(int n) {
 // row-major); the half format first finds the row / column exponents
 clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, long ldx,
                           const PlaneSet *pc, const PlaneSet *pr, const SplitWs &sp) {
+  // planes already split ahead of need for exactly this matrix and format
+  if (pr == &sp.PR) {
+    if (sp.spec_pr == X && sp.spec_pr_fmt == split) pr = nullptr;
+    sp.spec_pr = nullptr;
+  }
+  if (pc == &sp.PCA) {
+    if (sp.spec_pca == X && sp.spec_pca_fmt == split) pc = nullptr;
+    sp.spec_pca = nullptr;
+  }
+  if (!pr && !pc) return CLM_STATUS_SUCCESS;
   if (split == CLM_SPLIT_FP16X3) {
     // X's exponents, when found before (ahead of time or by the product that wrote X)
     // for every side this split needs; any split of the half format retires them
@@ -487,6 +508,19 @@ clm_status_t split_planes(hipStream_t st, int split, int n, const double *X, lon
   } else {
     CLM_TRY_HIP(launch_split_planes(st, split, n, n, X, ldx, pc ? pc->p : nullptr,
                                     pr ? pr->p : nullptr, sp.pstride));
+  }
+  return CLM_STATUS_SUCCESS;
+}
+
+// split X into PR (and PCA) ahead of need, noting it for the next split asked of them
+clm_status_t split_ahead(hipStream_t st, int split, int n, const double *X, long ldx, bool pca,
+                         const SplitWs &sp) {
+  CLM_TRY(split_planes(st, split, n, X, ldx, pca ? &sp.PCA : nullptr, &sp.PR, sp));
+  sp.spec_pr = X;
+  sp.spec_pr_fmt = split;
+  if (pca) {
+    sp.spec_pca = X;
+    sp.spec_pca_fmt = split;
   }
   return CLM_STATUS_SUCCESS;
 }
@@ -909,7 +943,25 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
           CLM_TRY_HIP(launch_norm1(st, n, ws.bank + static_cast<long>(k - 2) * stride, n,
                                    ws.colsum, dnorms + (k - 2)));
       }
-      CLM_TRY(read_back(st, c->device, dnorms + (formed - 1), fresh, pn + formed, may_allocate));
+      // behind this wait, the split variant splits A^q as a left operand when the plan
+      // from the norms known so far has a block above q (the next power A^(q+1) is then
+      // likely wanted and starts from those planes; otherwise they are replaced): the
+      // device would idle through the wait anyway
+      auto power_ahead = [&]() -> hipError_t {
+        if (variant != kSplit) return hipSuccess;
+        if constexpr (std::is_same_v<T, double>) {
+          const clm_expm_plan known =
+              formed == 1 ? p0 : split_plan_powers(pn, formed, n, split);
+          if (known.ps_block <= q) return hipSuccess;
+          return split_ahead(st, p0.ps_split, n, ws.bank + static_cast<long>(q - 2) * stride, n,
+                             false, sp) == CLM_STATUS_SUCCESS
+                     ? hipSuccess
+                     : hipErrorLaunchFailure;
+        }
+        return hipSuccess;
+      };
+      CLM_TRY(read_back(st, c->device, dnorms + (formed - 1), fresh, pn + formed, may_allocate,
+                        power_ahead));
       formed = q;
       const bool more = use_split ? split_worth_next_power(pn, q, n, split, w_new)
                                   : worth_next_power(pn, q, is_single<T>());
@@ -945,6 +997,7 @@ clm_status_t expm_impl(clm_ctx *ctx, hipStream_t st, int n, const T *dA, int lda
   } else if (p.s > 0) {
     CLM_TRY_HIP(launch_scale_copy(st, n, std::ldexp(1.0, -p.s), dA, lda, ws.As, n));
     CLM_TRY_HIP(launch_scale_powers(st, n, ws.bank, static_cast<long>(ws.stride), q - 1, p.s));
+    sp.drop_spec();  // the powers changed in place
     src = ws.As;
     ld_src = n;
   }

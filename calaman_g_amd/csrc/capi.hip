@@ -598,8 +598,18 @@ clm_status_t ps_evaluate_split(hipStream_t st, int n, const double *coef, int de
     const int remaining = degree - j * p.s + 1;
     return remaining < p.s ? remaining : p.s;
   };
-  CLM_TRY_HIP(launch_build_block(st, n, buf[cur], ld[cur], coef + static_cast<long>(p.r) * p.s,
-                                 terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
+  if (split == CLM_SPLIT_FP16X3 && p.r >= 1) {
+    // the top block's row exponents come out of its own pass (into the spare row
+    // array, free here) and serve its first split as a left operand
+    const int gen = ++sp.generation;
+    CLM_TRY_HIP(launch_build_block_rows(st, n, buf[cur], ld[cur],
+                                        coef + static_cast<long>(p.r) * p.s, terms(p.r), A, lda,
+                                        bank, stride, static_cast<long>(n), sp.e_alt, gen));
+    sp.pre = PreExponents{buf[cur], sp.e_alt, nullptr, gen};
+  } else {
+    CLM_TRY_HIP(launch_build_block(st, n, buf[cur], ld[cur], coef + static_cast<long>(p.r) * p.s,
+                                   terms(p.r), A, lda, bank, stride, static_cast<long>(n)));
+  }
   for (int j = p.r - 1; j >= 0; --j) {
     const int next = 1 - cur;
     CLM_TRY(split_planes(st, split, n, buf[cur], ld[cur], nullptr, &sp.PR, sp));

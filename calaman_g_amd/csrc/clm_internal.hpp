@@ -248,6 +248,12 @@ hipError_t launch_scale_copy(hipStream_t stream, int n, double f, const T *in, l
 template <typename T>
 hipError_t launch_lascl(hipStream_t stream, size_t m, size_t n, double mul, T *A, size_t lda);
 // B = coef[0] I + coef[1] X1 + sum_t coef[t] bank[t-2]
+// launch_build_block for FP64 on the 64 x 64 element-wise tiling, with each row's
+// tagged exponent (generation gen, the half-format splits' encoding) into row_e
+hipError_t launch_build_block_rows(hipStream_t stream, int n, double *Bout, long ldb,
+                                   const double *coef, int nterms, const double *X1, long ldx1,
+                                   const double *bank, long stride, long ldbank, int *row_e,
+                                   int gen);
 template <typename T>
 hipError_t launch_build_block(hipStream_t stream, int n, T *Bout, long ldb, const T *coef,
                               int nterms, const T *X1, long ldx1, const T *bank, long stride,

@@ -529,6 +529,100 @@ __device__ __forceinline__ void max_exponent_body(const Tile64 &tl, int rows, in
   }
 }
 
+// The Paterson-Stockmeyer block B = c0 I + c1 X1 + sum_t c_t bank[t-2] (each entry as
+// build_block_kernel forms it, the same multiply-adds in the same order) on the
+// element-wise tiling, with the exponent of each row's largest |B| found on the way
+// (max_exponent_body's row half): the block's first split as a left operand then needs
+// no exponent pass of its own.
+template <bool kFull>
+__device__ __forceinline__ void build_block_rows_body(const Tile64 &tl, int n, double *Bout,
+                                                      long ldb, const double *coef, int nterms,
+                                                      const double *X1, long ldx1,
+                                                      const double *bank, long stride,
+                                                      long ldbank, int *row_e, int gen,
+                                                      double (&red)[4][64]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[q][r] = (tl.r0 + r == tl.col(q)) ? coef[0] : 0.0;
+  if (nterms > 1) {
+    double x[4][4];
+    tile64_load<kFull>(tl, n, n, X1, ldx1, x);
+    const double c = coef[1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[q][r] = ops<double>::mad(c, x[q][r], v[q][r]);
+  }
+  for (int t = 2; t < nterms; ++t) {
+    double x[4][4];
+    tile64_load<kFull>(tl, n, n, bank + static_cast<long>(t - 2) * stride, ldbank, x);
+    const double c = coef[t];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[q][r] = ops<double>::mad(c, x[q][r], v[q][r]);
+  }
+  double rm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = tl.col(q);
+    if constexpr (kFull) {
+      double *p = Bout + tl.r0 + static_cast<long>(j) * ldb;
+      *reinterpret_cast<double2 *>(p) = double2{v[q][0], v[q][1]};
+      *reinterpret_cast<double2 *>(p + 2) = double2{v[q][2], v[q][3]};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) rm[r] = fmax(rm[r], fabs(v[q][r]));
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (j < n && tl.r0 + r < n) {
+          Bout[tl.r0 + r + static_cast<long>(j) * ldb] = v[q][r];
+          rm[r] = fmax(rm[r], fabs(v[q][r]));
+        }
+    }
+  }
+  // lanes l, l ^ 16, l ^ 32, l ^ 48 share rows; after bits 4, 5 a lane keeps row
+  // 2 b4 + b5 of its four, then the four waves meet in LDS
+  {
+    const bool b = lane & 16;
+    const double s0 = b ? rm[0] : rm[2], s1 = b ? rm[1] : rm[3];
+    const double k0 = b ? rm[2] : rm[0], k1 = b ? rm[3] : rm[1];
+    rm[0] = fmax(k0, __shfl_xor(s0, 16, 64));
+    rm[1] = fmax(k1, __shfl_xor(s1, 16, 64));
+  }
+  {
+    const bool b = lane & 32;
+    const double s = b ? rm[0] : rm[1], k = b ? rm[1] : rm[0];
+    rm[0] = fmax(k, __shfl_xor(s, 32, 64));
+  }
+  red[wave][4 * (lane & 15) + 2 * ((lane >> 4) & 1) + ((lane >> 5) & 1)] = rm[0];
+  __syncthreads();
+  if (wave == 0) {
+    const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+    const int i = tl.i0 + lane;
+    if (i < n && m > 0.0) atomicMax(&row_e[i], tile::exp_tag(gen, ilogb(m)));
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    build_block_rows_kernel(int n, double *__restrict__ Bout, long ldb,
+                            const double *__restrict__ coef, int nterms,
+                            const double *__restrict__ X1, long ldx1,
+                            const double *__restrict__ bank, long stride, long ldbank,
+                            int *row_e, int gen, bool sources_aligned) {
+  __shared__ double red[4][64];
+  const Tile64 tl(n, n, Bout, ldb);
+  if (tl.full && sources_aligned)
+    build_block_rows_body<true>(tl, n, Bout, ldb, coef, nterms, X1, ldx1, bank, stride, ldbank,
+                                row_e, gen, red);
+  else
+    build_block_rows_body<false>(tl, n, Bout, ldb, coef, nterms, X1, ldx1, bank, stride, ldbank,
+                                 row_e, gen, red);
+}
+
 __global__ void __launch_bounds__(256)
     max_exponent_kernel(int rows, int cols, const double *__restrict__ X, long ldx, int *row_e,
                         int *col_e, int gen) {
@@ -1108,6 +1202,20 @@ hipError_t launch_clear_exponents(hipStream_t stream, int n, int *a, int *b, int
   if (n <= 0 || (!a && !b && !c && !d)) return hipSuccess;
   hipLaunchKernelGGL(clear_exponents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, b, c, d,
                      n);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_block_rows(hipStream_t stream, int n, double *Bout, long ldb,
+                                   const double *coef, int nterms, const double *X1, long ldx1,
+                                   const double *bank, long stride, long ldbank, int *row_e,
+                                   int gen) {
+  if (n < 1) return hipSuccess;
+  auto a16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const bool aligned = (nterms < 2 || (a16(X1) && (ldx1 & 1) == 0)) &&
+                       (nterms < 3 || (a16(bank) && (ldbank & 1) == 0 && (stride & 1) == 0));
+  const dim3 grid((n + 63) / 64, (n + 63) / 64), block(256);
+  hipLaunchKernelGGL(build_block_rows_kernel, grid, block, 0, stream, n, Bout, ldb, coef, nterms,
+                     X1, ldx1, bank, stride, ldbank, row_e, gen, aligned);
   return hipGetLastError();
 }
 
